@@ -1,0 +1,307 @@
+// dedisp_api.cpp -- the dedisp-compatible C ABI (include/dedisp.h).
+//
+// Replaces the external dedisp library at the call sites
+// src/apps/casm_hella_refactored.cu:205-210 and
+// src/hella/src/dedisperse.cpp:32-48.  Errors are returned as dedisp_error
+// values (the reference turns them into std::runtime_error at
+// dedisperse.cpp:40-46); nothing here exits the process.
+#include <algorithm>
+#include <cstring>
+
+#include "../../include/dedisp.h"
+#include "internal.hpp"
+
+using namespace hamd;
+
+struct dedisp_plan_struct {
+  PlanHost h;
+  hipStream_t stream = nullptr;
+  std::vector<dedisp_bool> killmask_pub;
+  // device-side copies, rebuilt when the DM list or the killmask change
+  bool dirty = true;
+  DevBuf d_delta, d_live;
+  int nlive = 0;
+  std::vector<int32_t> delta_host, live_host;
+  DDTile tile{16, 32};
+  int max_spread = 0;
+  DevBuf scratch_in, scratch_ft, scratch_out;
+};
+
+namespace hamd {
+// Tile choice shared with the pipeline: the largest channel chunk whose LDS
+// footprint fits the budget, shrinking the DM block when even 8 channels do
+// not fit.
+DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
+                      int lanes_pref, size_t lds_budget)
+{
+  static const int gs[] = {8, 16, 32, 64};
+  for (int gi = 0; gi < 4; ++gi) {
+    int G = gs[gi];
+    if (G < lanes_pref) continue;
+    int nd = 256 / G;
+    int spread = dd_max_spread(delta, ndm, nlive, nd);
+    for (int cc : {32, 16, 8}) {
+      DDTile t{G, cc};
+      if (dd_lds_bytes(t, spread) <= lds_budget) {
+        *spread_out = spread;
+        return t;
+      }
+    }
+  }
+  DDTile t{64, 8};
+  *spread_out = dd_max_spread(delta, ndm, nlive, 4);
+  return t;
+}
+}  // namespace hamd
+
+namespace {
+
+dedisp_error refresh_device(dedisp_plan plan)
+{
+  if (!plan->dirty) return DEDISP_NO_ERROR;
+  plan->h.delays_live(0, plan->h.dm_list.size(), plan->delta_host, plan->live_host);
+  plan->nlive = (int)plan->live_host.size();
+  const int ndm = (int)plan->h.dm_list.size();
+  if (plan->nlive > 0) {
+    plan->tile = choose_dd_tile(plan->delta_host.data(), ndm, plan->nlive, &plan->max_spread,
+                                16, 64 * 1024);
+    plan->d_delta.ensure(plan->delta_host.size() * 4);
+    plan->d_live.ensure(plan->live_host.size() * 4);
+    HAMD_CHECK(hipMemcpy(plan->d_delta.p, plan->delta_host.data(), plan->delta_host.size() * 4,
+                         hipMemcpyHostToDevice));
+    HAMD_CHECK(hipMemcpy(plan->d_live.p, plan->live_host.data(), plan->live_host.size() * 4,
+                         hipMemcpyHostToDevice));
+  }
+  plan->dirty = false;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error gpu_error(const HipError &e)
+{
+  set_last_error(std::string(e.what) + ": " + hipGetErrorString(e.err));
+  if (e.err == hipErrorOutOfMemory) return DEDISP_MEM_ALLOC_FAILED;
+  return DEDISP_INTERNAL_GPU_ERROR;
+}
+
+}  // namespace
+
+extern "C" {
+
+dedisp_error dedisp_create_plan(dedisp_plan *plan, dedisp_size nchans, dedisp_float dt,
+                                dedisp_float f0, dedisp_float df)
+{
+  if (!plan) return DEDISP_INVALID_POINTER;
+  *plan = nullptr;
+  if (nchans == 0 || !(dt > 0.f) || !(f0 > 0.f) || !(df < 0.f)) return DEDISP_UNKNOWN_ERROR;
+  if (nchans > (1u << 16)) return DEDISP_NCHANS_EXCEEDS_LIMIT;
+  auto *p = new (std::nothrow) dedisp_plan_struct();
+  if (!p) return DEDISP_MEM_ALLOC_FAILED;
+  p->h.nchans = nchans;
+  p->h.dt = dt;
+  p->h.f0 = f0;
+  p->h.df = df;
+  p->h.build_delay_table();
+  p->h.killmask.assign(nchans, 1);
+  p->killmask_pub.assign(nchans, 1);
+  *plan = p;
+  return DEDISP_NO_ERROR;
+}
+
+void dedisp_destroy_plan(dedisp_plan plan) { delete plan; }
+
+dedisp_error dedisp_generate_dm_list(dedisp_plan plan, dedisp_float dm_start, dedisp_float dm_end,
+                                     dedisp_float ti, dedisp_float tol)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (!(dm_end >= dm_start) || !(tol > 1.f) || !(ti >= 0.f)) return DEDISP_UNKNOWN_ERROR;
+  plan->h.generate_dm_list(dm_start, dm_end, (double)ti, (double)tol);
+  plan->dirty = true;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_set_dm_list(dedisp_plan plan, const dedisp_float *dm_list, dedisp_size count)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (!dm_list || count == 0) return DEDISP_INVALID_POINTER;
+  for (dedisp_size i = 0; i < count; ++i)
+    if (!(dm_list[i] >= 0.f) || (i && dm_list[i] < dm_list[i - 1])) return DEDISP_UNKNOWN_ERROR;
+  plan->h.set_dm_list(dm_list, count);
+  plan->dirty = true;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_set_killmask(dedisp_plan plan, const dedisp_bool *killmask)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  for (size_t c = 0; c < plan->h.nchans; ++c) {
+    int v = killmask ? (killmask[c] != 0) : 1;
+    plan->h.killmask[c] = v;
+    plan->killmask_pub[c] = v;
+  }
+  plan->dirty = true;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_set_gulp_size(dedisp_plan plan, dedisp_size gulp_size)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (gulp_size == 0) return DEDISP_UNKNOWN_ERROR;
+  plan->h.gulp_size = gulp_size;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_size dedisp_get_gulp_size(dedisp_plan plan) { return plan ? plan->h.gulp_size : 0; }
+const dedisp_float *dedisp_get_dm_list(const dedisp_plan plan)
+{
+  return (plan && !plan->h.dm_list.empty()) ? plan->h.dm_list.data() : nullptr;
+}
+dedisp_size dedisp_get_dm_count(const dedisp_plan plan) { return plan ? plan->h.dm_list.size() : 0; }
+dedisp_size dedisp_get_max_delay(const dedisp_plan plan) { return plan ? plan->h.max_delay : 0; }
+dedisp_size dedisp_get_dm_delay(const dedisp_plan plan, int dm_trial)
+{
+  if (!plan || dm_trial < 0 || (size_t)dm_trial >= plan->h.dm_list.size()) return 0;
+  return (dedisp_size)plan->h.delay_of((size_t)dm_trial, plan->h.nchans - 1);
+}
+dedisp_size dedisp_get_channel_count(const dedisp_plan plan) { return plan ? plan->h.nchans : 0; }
+const dedisp_bool *dedisp_get_killmask(const dedisp_plan plan)
+{
+  return plan ? plan->killmask_pub.data() : nullptr;
+}
+dedisp_float dedisp_get_dt(const dedisp_plan plan) { return plan ? plan->h.dt : 0.f; }
+dedisp_float dedisp_get_f0(const dedisp_plan plan) { return plan ? plan->h.f0 : 0.f; }
+dedisp_float dedisp_get_df(const dedisp_plan plan) { return plan ? plan->h.df : 0.f; }
+
+dedisp_error dedisp_set_stream(dedisp_plan plan, void *stream)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  plan->stream = static_cast<hipStream_t>(stream);
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, const dedisp_byte *in,
+                                 dedisp_size in_nbits, dedisp_size in_stride, dedisp_byte *out,
+                                 dedisp_size out_nbits, dedisp_size out_stride,
+                                 dedisp_size first_dm_idx, dedisp_size dm_count, unsigned flags)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (plan->h.dm_list.empty()) return DEDISP_NO_DM_LIST_SET;
+  if (!in || !out) return DEDISP_INVALID_POINTER;
+  if ((flags & DEDISP_HOST_POINTERS) && (flags & DEDISP_DEVICE_POINTERS))
+    return DEDISP_INVALID_FLAG_COMBINATION;
+  if ((flags & DEDISP_WAIT) && (flags & DEDISP_ASYNC)) return DEDISP_INVALID_FLAG_COMBINATION;
+  if (in_nbits != 8) return DEDISP_UNSUPPORTED_IN_NBITS;
+  if (out_nbits != 32) return DEDISP_UNSUPPORTED_OUT_NBITS;
+  const size_t nch = plan->h.nchans;
+  const size_t md = plan->h.max_delay;
+  if (nsamps <= md) return DEDISP_TOO_FEW_NSAMPS;
+  if (first_dm_idx + dm_count > plan->h.dm_list.size() || dm_count == 0)
+    return DEDISP_UNKNOWN_ERROR;
+  const size_t nout = nsamps - md;
+  if (in_stride < nch) return DEDISP_INVALID_STRIDE;
+  if (out_stride < nout * 4 || out_stride % 4) return DEDISP_INVALID_STRIDE;
+  if (nout > (size_t)INT32_MAX || dm_count > (size_t)INT32_MAX) return DEDISP_TOO_FEW_NSAMPS;
+  const bool dev = (flags & DEDISP_DEVICE_POINTERS) != 0;
+  try {
+    refresh_device(plan);
+    hipStream_t s = plan->stream;
+    const uint8_t *d_in = in;
+    if (!dev) {
+      plan->scratch_in.ensure(nsamps * in_stride);
+      HAMD_CHECK(hipMemcpyAsync(plan->scratch_in.p, in, nsamps * in_stride, hipMemcpyHostToDevice, s));
+      d_in = plan->scratch_in.as<uint8_t>();
+    }
+    // transpose to channel-major rows long enough for every tile read
+    const int V = dd_samples_per_thread();
+    const size_t T = (size_t)plan->tile.lanes_per_dm * V;
+    const size_t ntiles = (nout + T - 1) / T;
+    const size_t ring = round_up(ntiles * T + md + (size_t)plan->max_spread + 64, 64);
+    const size_t row_stride = ring;
+    plan->scratch_ft.ensure(nch * row_stride);
+    transpose_tf_to_ft(d_in, (int64_t)nsamps, (int64_t)in_stride, (int)nch,
+                       plan->scratch_ft.as<uint8_t>(), (int64_t)row_stride, (int64_t)ring, s);
+    void *d_out = out;
+    if (!dev) {
+      plan->scratch_out.ensure(dm_count * out_stride);
+      d_out = plan->scratch_out.p;
+    }
+    DDLaunch L{};
+    L.src = DDSource{plan->scratch_ft.as<uint8_t>(), (int64_t)row_stride, (int64_t)ring, 0};
+    L.tab = DDDeviceTables{plan->d_delta.as<int32_t>() + first_dm_idx * (size_t)plan->nlive,
+                           plan->d_live.as<int32_t>(), plan->nlive, (int)dm_count};
+    L.nout = (int)nout;
+    L.out = d_out;
+    L.out_stride = (int64_t)(out_stride / 4);
+    L.out_type = DDOut::F32;
+    L.tile = plan->tile;
+    L.max_spread = plan->max_spread;
+    dd_launch(L, s);
+    if (!dev)
+      HAMD_CHECK(hipMemcpyAsync(out, d_out, dm_count * out_stride, hipMemcpyDeviceToHost, s));
+    if (!(flags & DEDISP_ASYNC)) HAMD_CHECK(hipStreamSynchronize(s));
+  } catch (const HipError &e) {
+    return gpu_error(e);
+  } catch (...) {
+    return DEDISP_UNKNOWN_ERROR;
+  }
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_execute_adv(const dedisp_plan plan, dedisp_size nsamps, const dedisp_byte *in,
+                                dedisp_size in_nbits, dedisp_size in_stride, dedisp_byte *out,
+                                dedisp_size out_nbits, dedisp_size out_stride, unsigned flags)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  return dedisp_execute_guru(plan, nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, 0,
+                             plan->h.dm_list.size(), flags);
+}
+
+dedisp_error dedisp_execute(const dedisp_plan plan, dedisp_size nsamps, const dedisp_byte *in,
+                            dedisp_size in_nbits, dedisp_byte *out, dedisp_size out_nbits,
+                            unsigned flags)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (nsamps <= plan->h.max_delay) return DEDISP_TOO_FEW_NSAMPS;
+  const size_t in_stride = plan->h.nchans * in_nbits / 8;
+  const size_t out_stride = (nsamps - plan->h.max_delay) * out_nbits / 8;
+  return dedisp_execute_adv(plan, nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, flags);
+}
+
+dedisp_error dedisp_set_device(int device_idx)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return DEDISP_PRIOR_GPU_ERROR;
+  if (device_idx < 0 || device_idx >= n) return DEDISP_INVALID_DEVICE_INDEX;
+  if (hipSetDevice(device_idx) != hipSuccess) return DEDISP_INTERNAL_GPU_ERROR;
+  return DEDISP_NO_ERROR;
+}
+
+dedisp_error dedisp_sync(void)
+{
+  return hipDeviceSynchronize() == hipSuccess ? DEDISP_NO_ERROR : DEDISP_PRIOR_GPU_ERROR;
+}
+
+const char *dedisp_get_error_string(dedisp_error error)
+{
+  switch (error) {
+    case DEDISP_NO_ERROR: return "No error";
+    case DEDISP_MEM_ALLOC_FAILED: return "Memory allocation failed";
+    case DEDISP_MEM_COPY_FAILED: return "Memory copy failed";
+    case DEDISP_NCHANS_EXCEEDS_LIMIT: return "No. channels exceeds internal limit";
+    case DEDISP_INVALID_PLAN: return "Invalid plan";
+    case DEDISP_INVALID_POINTER: return "Invalid pointer";
+    case DEDISP_INVALID_STRIDE: return "Invalid stride";
+    case DEDISP_NO_DM_LIST_SET: return "No DM list has been set";
+    case DEDISP_TOO_FEW_NSAMPS: return "No. samples < maximum delay";
+    case DEDISP_INVALID_FLAG_COMBINATION: return "Invalid flag combination";
+    case DEDISP_UNSUPPORTED_IN_NBITS: return "Unsupported in_nbits value";
+    case DEDISP_UNSUPPORTED_OUT_NBITS: return "Unsupported out_nbits value";
+    case DEDISP_INVALID_DEVICE_INDEX: return "Invalid device index";
+    case DEDISP_DEVICE_ALREADY_SET: return "Device is already set and cannot be changed";
+    case DEDISP_PRIOR_GPU_ERROR: return "There was a previous GPU error";
+    case DEDISP_INTERNAL_GPU_ERROR: return "Internal GPU error";
+    case DEDISP_UNKNOWN_ERROR: return "Unknown error";
+  }
+  return "Invalid error code";
+}
+
+}  // extern "C"

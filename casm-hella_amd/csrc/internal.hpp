@@ -1,0 +1,132 @@
+// internal.hpp -- shared declarations of libhella_amd (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hamd {
+
+// ---------------------------------------------------------------- errors
+struct HipError {
+  hipError_t err;
+  const char *what;
+  int line;
+};
+
+#define HAMD_CHECK(expr)                                   \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) throw ::hamd::HipError{e_, #expr, __LINE__}; \
+  } while (0)
+
+void set_last_error(const std::string &msg);
+const char *last_error();
+
+inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Device buffer owned by the library (hipMalloc/hipFree).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    release();
+    p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0;
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // grows (never shrinks); contents are not preserved
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (b == 0) return;
+    HAMD_CHECK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------- plan
+// Host description of a dedispersion plan (dedisp semantics).
+struct PlanHost {
+  size_t nchans = 0;
+  float dt = 0.f, f0 = 0.f, df = 0.f;
+  std::vector<float> delay_table;   // per channel, float
+  std::vector<float> dm_list;       // ascending
+  std::vector<int> killmask;        // 1 = live
+  size_t max_delay = 0;
+  size_t gulp_size = 65536;
+
+  void build_delay_table();
+  void generate_dm_list(float dm_start, float dm_end, double ti_us, double tol);
+  void set_dm_list(const float *dms, size_t n);
+  void compute_max_delay();
+  // delta[d*nlive + i] for live channels only; live[i] = channel index
+  void delays_live(size_t d_first, size_t nd, std::vector<int32_t> &delta,
+                   std::vector<int32_t> &live) const;
+  int32_t delay_of(size_t d, size_t c) const {
+    float p = dm_list[d] * delay_table[c];
+    return (int32_t)((double)p + 0.5);
+  }
+};
+
+// ---------------------------------------------------------------- dedisp kernel
+// Source rows: channel c of the FT u8 store begins at base + c*row_stride;
+// column x of the store is (start + x) mod ring (ring is a multiple of 16).
+struct DDSource {
+  const uint8_t *base;
+  int64_t row_stride;
+  int64_t ring;
+  int64_t start;
+};
+
+// Tile shape of the dedispersion kernel.
+struct DDTile {
+  int lanes_per_dm;  // G: threads sharing one DM trial (time groups)
+  int chunk;         // CC: channels staged in LDS per step (divides 256)
+};
+
+struct DDDeviceTables {
+  const int32_t *delta;   // [ndm][nlive] delays of live channels
+  const int32_t *live;    // [nlive] channel index of each live channel
+  int nlive;
+  int ndm;
+};
+
+// Launch the dedispersion kernel: out[d*out_stride + t], t < nout.
+// OutT = uint32_t or float.
+enum class DDOut { U32, F32 };
+struct DDLaunch {
+  DDSource src;
+  DDDeviceTables tab;
+  int nout;
+  void *out;
+  int64_t out_stride;   // elements
+  DDOut out_type;
+  DDTile tile;
+  int max_spread;       // max over DM blocks of delay(last)-delay(first) for any channel
+};
+void dd_launch(const DDLaunch &L, hipStream_t s);
+int dd_samples_per_thread();
+// largest per-channel spread of delays inside any block of nd consecutive DMs
+int dd_max_spread(const int32_t *delta, int ndm, int nlive, int nd);
+size_t dd_lds_bytes(const DDTile &t, int max_spread);
+
+// u8 time-major [nsamps][in_stride] -> FT rows (row_stride), zero tail to ring
+void transpose_tf_to_ft(const uint8_t *in, int64_t nsamps, int64_t in_stride,
+                        int nch, uint8_t *out, int64_t row_stride, int64_t ring,
+                        hipStream_t s);
+
+}  // namespace hamd
