@@ -25,6 +25,9 @@ struct dedisp_plan_struct {
   DDTile tile{16, 32};
   int max_spread = 0;
   DevBuf scratch_in, scratch_ft, scratch_out;
+  DevBuf d_beam, d_tiles;
+  std::vector<DDTileDesc> tiles;
+  size_t tiles_nout = 0, tiles_first = 0, tiles_count = 0;
 };
 
 namespace hamd {
@@ -211,8 +214,7 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
       d_in = plan->scratch_in.as<uint8_t>();
     }
     // transpose to channel-major rows long enough for every tile read
-    const int V = dd_samples_per_thread();
-    const size_t T = (size_t)plan->tile.lanes_per_dm * V;
+    const size_t T = (size_t)dd_tile_T(plan->tile);
     const size_t ntiles = (nout + T - 1) / T;
     const size_t ring = round_up(ntiles * T + md + (size_t)plan->max_spread + 64, 64);
     const size_t row_stride = ring;
@@ -224,17 +226,37 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
       plan->scratch_out.ensure(dm_count * out_stride);
       d_out = plan->scratch_out.p;
     }
-    DDLaunch L{};
-    L.src = DDSource{plan->scratch_ft.as<uint8_t>(), (int64_t)row_stride, (int64_t)ring, 0};
-    L.tab = DDDeviceTables{plan->d_delta.as<int32_t>() + first_dm_idx * (size_t)plan->nlive,
-                           plan->d_live.as<int32_t>(), plan->nlive, (int)dm_count};
-    L.nout = (int)nout;
-    L.out = d_out;
-    L.out_stride = (int64_t)(out_stride / 4);
-    L.out_type = DDOut::F32;
-    L.tile = plan->tile;
-    L.max_spread = plan->max_spread;
-    dd_launch(L, s);
+    DDBeamDev B{};
+    B.base = plan->scratch_ft.as<uint8_t>();
+    B.row_stride = (int64_t)row_stride;
+    B.ring = (int64_t)ring;
+    B.start = 0;
+    B.delta = plan->d_delta.as<int32_t>() + first_dm_idx * (size_t)plan->nlive;
+    B.live = plan->d_live.as<int32_t>();
+    B.nlive = plan->nlive;
+    B.ndm = (int)dm_count;
+    B.nout = (int)nout;
+    B.out_f32 = 1;
+    B.out = d_out;
+    B.out_stride = (int64_t)(out_stride / 4);
+    if (plan->nlive == 0) {
+      fill_zero_rows(d_out, B.out_stride, (int)nout, (int)dm_count, s);
+    } else {
+      if (plan->tiles.empty() || plan->tiles_nout != nout || plan->tiles_first != first_dm_idx ||
+          plan->tiles_count != dm_count) {
+        dd_build_tiles(std::vector<DDBeamDev>{B}, plan->tile, plan->tiles);
+        plan->d_tiles.ensure(plan->tiles.size() * sizeof(DDTileDesc));
+        HAMD_CHECK(hipMemcpy(plan->d_tiles.p, plan->tiles.data(),
+                             plan->tiles.size() * sizeof(DDTileDesc), hipMemcpyHostToDevice));
+        plan->tiles_nout = nout;
+        plan->tiles_first = first_dm_idx;
+        plan->tiles_count = dm_count;
+      }
+      plan->d_beam.ensure(sizeof(DDBeamDev));
+      HAMD_CHECK(hipMemcpyAsync(plan->d_beam.p, &B, sizeof(DDBeamDev), hipMemcpyHostToDevice, s));
+      dd_launch(plan->d_beam.as<DDBeamDev>(), plan->d_tiles.as<DDTileDesc>(),
+                (int)plan->tiles.size(), plan->tile, plan->max_spread, s);
+    }
     if (!dev)
       HAMD_CHECK(hipMemcpyAsync(out, d_out, dm_count * out_stride, hipMemcpyDeviceToHost, s));
     if (!(flags & DEDISP_ASYNC)) HAMD_CHECK(hipStreamSynchronize(s));

@@ -36,28 +36,28 @@ __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
   return __builtin_amdgcn_perm(hi, lo, sel);
 }
 
-template <int G, typename OutT>
-__global__ __launch_bounds__(NT) void k_dedisp(DDSource src,
-                                               const int32_t *__restrict__ delta,
-                                               const int32_t *__restrict__ live,
-                                               int nlive, int ndm, int nout,
-                                               OutT *__restrict__ out,
-                                               int64_t out_stride, int chunk,
+template <int G>
+__global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ beams,
+                                               const DDTileDesc *__restrict__ tiles, int chunk,
                                                int lrow)
 {
   constexpr int ND = NT / G;
   constexpr int T = G * V;
   extern __shared__ __align__(16) uint8_t lds[];
-  uint8_t *sdat = lds;                                        // [chunk][lrow]
+  const DDTileDesc td = tiles[blockIdx.x];
+  if (td.beam < 0) return;
+  const DDBeamDev B = beams[td.beam];
+  uint8_t *sdat = lds;                                                      // [chunk][lrow]
   int32_t *sdly = reinterpret_cast<int32_t *>(lds + (size_t)chunk * lrow);  // [chunk][ND]
-  int32_t *soff = sdly + chunk * ND;                          // [chunk]
+  int32_t *soff = sdly + chunk * ND;                                        // [chunk]
 
   const int tid = threadIdx.x;
   const int dl = tid / G;
   const int g = tid % G;
-  const int d0 = blockIdx.y * ND;
-  const int t0 = blockIdx.x * T;
+  const int d0 = td.d0;
+  const int t0 = td.t0;
   const int wave = tid >> 6, lane = tid & 63;
+  const int nlive = B.nlive, ndm = B.ndm;
 
   uint32_t acc[V / 2];
   uint32_t tot[V];
@@ -73,23 +73,22 @@ __global__ __launch_bounds__(NT) void k_dedisp(DDSource src,
     for (int i = tid; i < nc * ND; i += NT) {
       const int cc = i / ND, dd = i - (i / ND) * ND;
       const int dm = min(d0 + dd, ndm - 1);
-      sdly[cc * ND + dd] = delta[(int64_t)dm * nlive + k0 + cc];
+      sdly[cc * ND + dd] = B.delta[(int64_t)dm * nlive + k0 + cc];
     }
     __syncthreads();
-    // stage windows: one wave per channel
+    // stage windows: one wave per channel, 16-byte units
     for (int cc = wave; cc < nc; cc += NT / 64) {
       const int base_d = sdly[cc * ND];
       const int spread = sdly[cc * ND + ND - 1] - base_d;
-      int64_t first = src.start + t0 + base_d;
-      if (first >= src.ring) first -= src.ring;
+      const int64_t first = (B.start + t0 + base_d) % B.ring;
       const int64_t al = first & ~(int64_t)15;
       const int off = (int)(first - al);
       const int n16 = (off + T + spread + 4 + 15) >> 4;
-      const uint8_t *row = src.base + (int64_t)live[k0 + cc] * src.row_stride;
+      const uint8_t *row = B.base + (int64_t)B.live[k0 + cc] * B.row_stride;
       uint8_t *dst = sdat + (size_t)cc * lrow;
       for (int u = lane; u < n16; u += 64) {
         int64_t col = al + 16 * (int64_t)u;
-        if (col >= src.ring) col -= src.ring;
+        if (col >= B.ring) col %= B.ring;
         *reinterpret_cast<uint4 *>(dst + 16 * u) = *reinterpret_cast<const uint4 *>(row + col);
       }
       if (lane == 0) soff[cc] = off;
@@ -128,41 +127,42 @@ __global__ __launch_bounds__(NT) void k_dedisp(DDSource src,
   const int d = d0 + dl;
   if (d >= ndm) return;
   const int tb = t0 + g * V;
-  OutT *o = out + (int64_t)d * out_stride + tb;
-  if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+  const int nout = B.nout;
+  if (B.out_f32) {
+    float *o = static_cast<float *>(B.out) + (int64_t)d * B.out_stride + tb;
+    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
 #pragma unroll
-    for (int k = 0; k < V / 4; ++k) {
-      if constexpr (std::is_same<OutT, float>::value) {
-        float4 v = make_float4((float)tot[4 * k], (float)tot[4 * k + 1], (float)tot[4 * k + 2],
-                               (float)tot[4 * k + 3]);
-        reinterpret_cast<float4 *>(o)[k] = v;
-      } else {
-        uint4 v = make_uint4(tot[4 * k], tot[4 * k + 1], tot[4 * k + 2], tot[4 * k + 3]);
-        reinterpret_cast<uint4 *>(o)[k] = v;
-      }
+      for (int k = 0; k < V / 4; ++k)
+        reinterpret_cast<float4 *>(o)[k] = make_float4((float)tot[4 * k], (float)tot[4 * k + 1],
+                                                       (float)tot[4 * k + 2], (float)tot[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; ++j)
+        if (tb + j < nout) o[j] = (float)tot[j];
     }
   } else {
+    uint32_t *o = static_cast<uint32_t *>(B.out) + (int64_t)d * B.out_stride + tb;
+    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
 #pragma unroll
-    for (int j = 0; j < V; ++j)
-      if (tb + j < nout) o[j] = (OutT)tot[j];
+      for (int k = 0; k < V / 4; ++k)
+        reinterpret_cast<uint4 *>(o)[k] =
+            make_uint4(tot[4 * k], tot[4 * k + 1], tot[4 * k + 2], tot[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; ++j)
+        if (tb + j < nout) o[j] = tot[j];
+    }
   }
 }
 
 template <int G>
-void launch_g(const DDLaunch &L, hipStream_t s, int lrow)
+void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int chunk,
+              int lrow, hipStream_t s)
 {
   constexpr int ND = NT / G;
-  constexpr int T = G * V;
-  dim3 grid((unsigned)((L.nout + T - 1) / T), (unsigned)((L.tab.ndm + ND - 1) / ND));
-  size_t lds = (size_t)L.tile.chunk * lrow + (size_t)L.tile.chunk * ND * 4 + (size_t)L.tile.chunk * 4;
-  if (L.out_type == DDOut::U32)
-    hipLaunchKernelGGL((k_dedisp<G, uint32_t>), grid, dim3(NT), lds, s, L.src, L.tab.delta,
-                       L.tab.live, L.tab.nlive, L.tab.ndm, L.nout,
-                       static_cast<uint32_t *>(L.out), L.out_stride, L.tile.chunk, lrow);
-  else
-    hipLaunchKernelGGL((k_dedisp<G, float>), grid, dim3(NT), lds, s, L.src, L.tab.delta,
-                       L.tab.live, L.tab.nlive, L.tab.ndm, L.nout,
-                       static_cast<float *>(L.out), L.out_stride, L.tile.chunk, lrow);
+  const size_t lds = (size_t)chunk * lrow + (size_t)chunk * ND * 4 + (size_t)chunk * 4;
+  hipLaunchKernelGGL(k_dedisp<G>, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles,
+                     chunk, lrow);
   HAMD_CHECK(hipGetLastError());
 }
 
@@ -226,26 +226,51 @@ size_t dd_lds_bytes(const DDTile &t, int max_spread)
   return (size_t)t.chunk * lrow_for(t, max_spread) + (size_t)t.chunk * ND * 4 + (size_t)t.chunk * 4;
 }
 
-void dd_launch(const DDLaunch &L, hipStream_t s)
+void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
+                    std::vector<DDTileDesc> &tiles, int order)
 {
-  if (L.nout <= 0 || L.tab.ndm <= 0) return;
-  if (L.tab.nlive <= 0) {
-    // every channel killed: all sums are zero (a fill kernel, not a memset,
-    // so the launch can sit inside a captured graph)
-    dim3 grid((unsigned)((L.nout + 255) / 256), (unsigned)L.tab.ndm);
-    hipLaunchKernelGGL(k_fill_rows, grid, dim3(256), 0, s, static_cast<uint32_t *>(L.out),
-                       L.out_stride, L.nout);
-    HAMD_CHECK(hipGetLastError());
-    return;
+  const int T = dd_tile_T(t), ND = dd_tile_ND(t);
+  std::vector<DDTileDesc> list;
+  for (size_t b = 0; b < beams.size(); ++b) {
+    const DDBeamDev &B = beams[b];
+    if (B.nout <= 0 || B.ndm <= 0 || B.nlive <= 0) continue;
+    if (order == 0) {
+      for (int d0 = 0; d0 < B.ndm; d0 += ND)
+        for (int t0 = 0; t0 < B.nout; t0 += T) list.push_back(DDTileDesc{(int)b, d0, t0, 0});
+    } else {
+      for (int t0 = 0; t0 < B.nout; t0 += T)
+        for (int d0 = 0; d0 < B.ndm; d0 += ND) list.push_back(DDTileDesc{(int)b, d0, t0, 0});
+    }
   }
-  const int lrow = lrow_for(L.tile, L.max_spread);
-  switch (L.tile.lanes_per_dm) {
-    case 8: launch_g<8>(L, s, lrow); break;
-    case 16: launch_g<16>(L, s, lrow); break;
-    case 32: launch_g<32>(L, s, lrow); break;
-    case 64: launch_g<64>(L, s, lrow); break;
+  // the dispatcher hands workgroup i to XCD i % 8: give each XCD a contiguous
+  // run of the list so neighbouring time tiles (overlapping windows) share
+  // that XCD's L2
+  const size_t nx = 8;
+  const size_t q = (list.size() + nx - 1) / nx;
+  tiles.assign(q * nx, DDTileDesc{-1, 0, 0, 0});
+  for (size_t k = 0; k < list.size(); ++k) tiles[(k % q) * nx + k / q] = list[k];
+}
+
+void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t,
+               int max_spread, hipStream_t s)
+{
+  if (ntiles <= 0) return;
+  const int lrow = lrow_for(t, max_spread);
+  switch (t.lanes_per_dm) {
+    case 8: launch_g<8>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
+    case 16: launch_g<16>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
+    case 32: launch_g<32>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
+    case 64: launch_g<64>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
     default: throw HipError{hipErrorInvalidValue, "lanes_per_dm", __LINE__};
   }
+}
+
+void fill_zero_rows(void *out, int64_t stride, int n, int rows, hipStream_t s)
+{
+  if (n <= 0 || rows <= 0) return;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)rows);
+  hipLaunchKernelGGL(k_fill_rows, grid, dim3(256), 0, s, static_cast<uint32_t *>(out), stride, n);
+  HAMD_CHECK(hipGetLastError());
 }
 
 void transpose_tf_to_ft(const uint8_t *in, int64_t nsamps, int64_t in_stride, int nch,

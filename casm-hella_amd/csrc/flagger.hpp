@@ -1,0 +1,64 @@
+// flagger.hpp -- device descriptors and host runner of the FP32 flagger.
+#pragma once
+
+#include "internal.hpp"
+
+namespace hamd {
+
+constexpr int FLAG_BLOCK = 4096;  // canonical reduction block (samples)
+
+enum FlagInFmt { FLAG_IN_U8_TF = 0, FLAG_IN_F32_TF = 1, FLAG_IN_F16_FT = 2 };
+enum FlagOp { FOP_NONE = 0, FOP_DIVBP = 1, FOP_NORM = 2, FOP_SPEC = 3, FOP_NORM_QUANT = 4 };
+enum FlagPart { FPART_NONE = 0, FPART_SUM = 1, FPART_SQDEV = 2 };
+
+// per-beam scalars kept on the device
+struct FScal {
+  double mu;    // mean of the last statistics pass
+  double inv;   // 1 / (double)sigf
+  float sigf;   // (float)sqrt(variance)
+  float pad;
+};
+
+// One beam of a gulp as seen by the flagger kernels.
+struct FlagBeamDev {
+  float *X;            // work plane [nch][xs]
+  int64_t xs;          // row stride of X and mask (elements)
+  int nch;
+  int nblk;            // ceil(W / FLAG_BLOCK)
+  const void *in;      // input gulp on the device
+  int in_fmt;          // FlagInFmt
+  int flag1, flag2;    // channel range zeroed by every scrunch
+  int64_t in_stride;   // TF: elements between samples; FT: elements between channels
+  uint8_t *ring;       // quantised output, channel-major
+  int64_t ring_stride; // bytes between channel rows
+  int64_t ring_len;    // columns (multiple of 16); column = (col0 + t) mod ring_len
+  int64_t col0;
+  double *part;        // [nch][nblk] block partial sums
+  float *bp;           // [nch] bandpass
+  FScal *scal;
+  uint32_t *flagcount; // [nch] grown-mask counts (accumulated)
+  float *ts;           // [W] time series of the final pass
+  uint8_t *mask;       // [nch][xs] threshold mask scratch
+};
+
+struct FlagScrunch {
+  int tscrunch, fscrunch;
+  float thresh;
+};
+
+struct FlagParams {
+  float spec_min = -0.05f, spec_max = 0.15f;
+  int nscrunch = 0;
+  FlagScrunch scrunch[8];
+};
+
+struct FlagRunner {
+  // Runs the whole flagger for nbeam beams described by d_beams (device
+  // array); max_nch is the largest channel count among them.
+  static void run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W, const FlagParams &P,
+                  hipStream_t s);
+};
+
+void zero_u32(uint32_t *p, int n, hipStream_t s);
+
+}  // namespace hamd

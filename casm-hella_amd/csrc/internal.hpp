@@ -83,50 +83,61 @@ struct PlanHost {
 };
 
 // ---------------------------------------------------------------- dedisp kernel
-// Source rows: channel c of the FT u8 store begins at base + c*row_stride;
-// column x of the store is (start + x) mod ring (ring is a multiple of 16).
-struct DDSource {
+// One beam as seen by the dedispersion kernel.  Source rows: channel c of the
+// FT u8 store begins at base + c*row_stride; window column x is stored at
+// column (start + x) mod ring (ring is a multiple of 16).
+struct DDBeamDev {
   const uint8_t *base;
   int64_t row_stride;
   int64_t ring;
   int64_t start;
+  const int32_t *delta;  // [ndm][nlive] delays of live channels
+  const int32_t *live;   // [nlive] channel index of each live channel
+  int nlive;
+  int ndm;
+  int nout;              // output samples per DM row
+  int out_f32;           // 1: float output, 0: u32
+  void *out;
+  int64_t out_stride;    // elements between DM rows
+};
+
+// One workgroup of work: beam, first DM trial, first output sample.  beam < 0
+// marks a padding entry.
+struct DDTileDesc {
+  int beam, d0, t0, pad;
 };
 
 // Tile shape of the dedispersion kernel.
 struct DDTile {
-  int lanes_per_dm;  // G: threads sharing one DM trial (time groups)
+  int lanes_per_dm;  // G: threads sharing one DM trial (time groups); ND = 256 / G
   int chunk;         // CC: channels staged in LDS per step (divides 256)
 };
 
-struct DDDeviceTables {
-  const int32_t *delta;   // [ndm][nlive] delays of live channels
-  const int32_t *live;    // [nlive] channel index of each live channel
-  int nlive;
-  int ndm;
-};
-
-// Launch the dedispersion kernel: out[d*out_stride + t], t < nout.
-// OutT = uint32_t or float.
-enum class DDOut { U32, F32 };
-struct DDLaunch {
-  DDSource src;
-  DDDeviceTables tab;
-  int nout;
-  void *out;
-  int64_t out_stride;   // elements
-  DDOut out_type;
-  DDTile tile;
-  int max_spread;       // max over DM blocks of delay(last)-delay(first) for any channel
-};
-void dd_launch(const DDLaunch &L, hipStream_t s);
 int dd_samples_per_thread();
+inline int dd_tile_T(const DDTile &t) { return t.lanes_per_dm * dd_samples_per_thread(); }
+inline int dd_tile_ND(const DDTile &t) { return 256 / t.lanes_per_dm; }
 // largest per-channel spread of delays inside any block of nd consecutive DMs
 int dd_max_spread(const int32_t *delta, int ndm, int nlive, int nd);
 size_t dd_lds_bytes(const DDTile &t, int max_spread);
+// Tile table over beams: entries ordered so that the workgroups that the
+// dispatcher sends to one XCD cover consecutive time tiles of one DM block.
+// order 0: DM-block-major (consecutive time tiles of one DM block per XCD run);
+// order 1: time-major (all DM blocks of a time tile per XCD run)
+void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
+                    std::vector<DDTileDesc> &tiles, int order = 0);
+// Launch over a device tile table.
+void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t,
+               int max_spread, hipStream_t s);
 
 // u8 time-major [nsamps][in_stride] -> FT rows (row_stride), zero tail to ring
 void transpose_tf_to_ft(const uint8_t *in, int64_t nsamps, int64_t in_stride,
                         int nch, uint8_t *out, int64_t row_stride, int64_t ring,
                         hipStream_t s);
+
+// zero rows [rows][n] of 32-bit words with a kernel (graph-capturable)
+void fill_zero_rows(void *out, int64_t stride, int n, int rows, hipStream_t s);
+
+DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
+                      int lanes_pref, size_t lds_budget);
 
 }  // namespace hamd

@@ -1,0 +1,837 @@
+// pipeline.cpp -- hella pipeline context: configuration, per-beam plans and
+// rings, the plain per-beam gulp path and the batched gulp path.
+//
+// Control flow restated from src/apps/casm_hella_refactored.cu:
+//   sizing            casm:182-224 (NTIME, ntime_dd, ntime_out; FILE input: one-shot)
+//   per-gulp loop     casm:659-835 (flag all beams; once the window is full,
+//                     per beam dedisperse -> smooth -> find_peaks while the
+//                     candidate cap and the time budget allow)
+//   rewind            casm:771-797 is replaced by a circular window: absolute
+//                     sample a of a beam lives at ring column a mod NTIME, so
+//                     no data moves between gulps.
+#include "pipeline.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <sstream>
+
+namespace hamd {
+
+// ------------------------------------------------------------------ config
+static bool parse_line_kv(const std::string &line, std::string &k, std::string &v)
+{
+  std::istringstream is(line);
+  if (!(is >> k)) return false;
+  if (!(is >> v)) v.clear();
+  return true;
+}
+
+bool Config::parse(const std::string &text, std::string &err, bool beam_only)
+{
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::string k, v;
+    if (!parse_line_kv(line, k, v) || k[0] == '#') continue;
+    auto F = [&]() { return (float)std::atof(v.c_str()); };
+    auto D = [&]() { return std::atof(v.c_str()); };
+    auto I = [&]() { return std::atoi(v.c_str()); };
+    if (k == "DM_MIN") dm_min = F();
+    else if (k == "DM_MAX") dm_max = F();
+    else if (k == "DM_TOL") dm_tol = F();
+    else if (k == "NCHAN") nchan = I();
+    else if (k == "FCH1") fch1 = D();
+    else if (k == "FOFF") foff = D();
+    else if (k == "TSAMP") tsamp = D();
+    else if (beam_only) continue;
+    else if (k == "SNR") snr = F();
+    else if (k == "WIDTH_MIN") width_min = I();
+    else if (k == "WIDTH_MAX") width_max = I();
+    else if (k == "GULP") gulp = std::atoll(v.c_str());
+    else if (k == "FLAG1") flag1 = I();
+    else if (k == "FLAG2") flag2 = I();
+    else if (k == "SPEC_MIN") spec_min = F();
+    else if (k == "SPEC_MAX") spec_max = F();
+    else if (k == "NBEAM") nbeam = I();
+    else if (k == "GPU") gpu = I();
+    else if (k == "PULSE_WIDTH") pulse_width_us = F();
+    else if (k == "MAX_GIANTS") max_giants = I();
+    else if (k == "CAND_CAP") cand_cap = I();
+    else if (k == "CLUSTER_DT") cluster_dt = I();
+    else if (k == "CLUSTER_DDM") cluster_ddm = I();
+    else if (k == "TIME_LIMIT") time_limit = D();
+    else if (k == "INPUT") {
+      // casm:74-80: DADA (fp16 FT), FILE (one window), FILTERBANK (u8 TF)
+      if (v == "DADA") { input_fmt = FLAG_IN_F16_FT; oneshot = false; }
+      else if (v == "FILE") { input_fmt = FLAG_IN_U8_TF; oneshot = true; }
+      else if (v == "FILTERBANK") { input_fmt = FLAG_IN_U8_TF; oneshot = false; }
+      else { err = "unsupported INPUT " + v; return false; }
+    } else if (k == "INPUT_DTYPE") {
+      if (v == "U8") input_fmt = FLAG_IN_U8_TF;
+      else if (v == "F32") input_fmt = FLAG_IN_F32_TF;
+      else if (v == "F16") input_fmt = FLAG_IN_F16_FT;
+      else { err = "unsupported INPUT_DTYPE " + v; return false; }
+    } else if (k == "SEARCH") {
+      if (v == "BOXCAR") search_mode = SEARCH_BOXCAR;
+      else if (v == "HELLA") search_mode = SEARCH_HELLA;
+      else { err = "unsupported SEARCH " + v; return false; }
+    } else if (k == "SCRUNCH") {
+      // casm:157-167: n lines "ts fs thresh nits" follow
+      int n = I();
+      if (n < 0 || n > 8) { err = "SCRUNCH count out of range (0..8)"; return false; }
+      flag.nscrunch = n;
+      for (int i = 0; i < n; ++i) {
+        if (!std::getline(in, line)) { err = "SCRUNCH: missing lines"; return false; }
+        int ts = 0, fs = 0, nits = 0;
+        float th = 0;
+        if (std::sscanf(line.c_str(), "%d %d %f %d", &ts, &fs, &th, &nits) < 3 || ts < 1 || fs < 1) {
+          err = "SCRUNCH: bad line '" + line + "'";
+          return false;
+        }
+        flag.scrunch[i] = FlagScrunch{ts, fs, th};
+      }
+    }
+  }
+  flag.spec_min = spec_min;
+  flag.spec_max = spec_max;
+  return true;
+}
+
+// ------------------------------------------------------------------ build
+static void alloc_zero(DevBuf &b, size_t bytes, hipStream_t s)
+{
+  b.ensure(std::max<size_t>(bytes, 16));
+  zero_u32(b.as<uint32_t>(), (int)(round_up(b.bytes, 4) / 4), s);
+}
+
+void ctx_build(hella_ctx c)
+{
+  if (c->built) return;
+  Config &cfg = c->cfg;
+  const int W = (int)cfg.gulp;
+  c->W = W;
+  const int maxW = cfg.width_max, minW = cfg.width_min;
+  c->nw = search_nwidth(cfg.search_mode, minW, maxW);
+  if (c->nw <= 0 || c->nw > SEARCH_MAXW || maxW > 1024 || (maxW & (maxW - 1)) ||
+      (minW & (minW - 1)))
+    throw std::runtime_error("WIDTH_MIN/WIDTH_MAX must be powers of two, maxW <= 1024");
+  hipStream_t s = c->stream;
+  int64_t in_off = 0;
+  const int esz = cfg.input_fmt == FLAG_IN_U8_TF ? 1 : (cfg.input_fmt == FLAG_IN_F32_TF ? 4 : 2);
+  c->max_nch = 0;
+  c->max_rows = 0;
+  int ntdd = -1, nout = -1;
+  for (auto &B : c->beams) {
+    if (B.nch <= 0) throw std::runtime_error("NCHAN must be positive");
+    B.plan = PlanHost{};
+    B.plan.nchans = (size_t)B.nch;
+    B.plan.dt = (float)B.dt;
+    B.plan.f0 = (float)B.f0;
+    B.plan.df = (float)B.df;
+    if (!(B.plan.df < 0.f) || !(B.plan.dt > 0.f)) throw std::runtime_error("FOFF must be < 0, TSAMP > 0");
+    B.plan.build_delay_table();
+    B.plan.generate_dm_list(B.dm_lo, B.dm_hi, (double)cfg.pulse_width_us, (double)B.dm_tol);
+    if (B.killmask.empty()) B.killmask.assign((size_t)B.nch, 1);
+    B.plan.killmask = B.killmask;
+    B.plan.delays_live(0, B.plan.dm_list.size(), B.delta_host, B.live_host);
+    B.nlive = (int)B.live_host.size();
+    B.ndm = (int)B.plan.dm_list.size();
+    B.md = (int64_t)B.plan.max_delay;
+    if (cfg.oneshot) {
+      B.ntime = W;
+      B.ntime_dd = (int)(W - B.md);
+      B.nout = B.ntime_dd - maxW;
+      B.min_gulp = 0;
+    } else {
+      B.ntime = 32 * ((W + B.md + maxW) / 32 + 1);
+      B.ntime_dd = W + maxW;
+      B.nout = W;
+      B.min_gulp = (int)(B.ntime / W);
+    }
+    if (B.nout <= 0) throw std::runtime_error("gulp too short for the maximum DM delay");
+    if (ntdd >= 0 && (ntdd != B.ntime_dd || nout != B.nout))
+      throw std::runtime_error("one-shot sizing needs the same max delay in every beam");
+    ntdd = B.ntime_dd;
+    nout = B.nout;
+    B.rows = cfg.search_mode == SEARCH_HELLA ? B.ndm - 2 : B.ndm;
+    if (B.rows <= 0) throw std::runtime_error("too few DM trials for the search mode");
+    const int64_t ring_len = cfg.oneshot ? (int64_t)round_up((size_t)W, 16) : B.ntime;
+    B.ring_stride = (int64_t)round_up((size_t)ring_len, 64);
+    B.ntime = cfg.oneshot ? ring_len : B.ntime;
+    B.ring.ensure((size_t)B.nch * B.ring_stride);
+    alloc_zero(B.ring, (size_t)B.nch * B.ring_stride, s);
+    B.dstride = (int64_t)round_up((size_t)B.ntime_dd, 64);
+    B.in_off = in_off;
+    B.in_bytes = (int64_t)W * B.nch * esz;
+    in_off += B.in_bytes;
+    if (B.nlive > 0) {
+      B.d_delta.ensure(B.delta_host.size() * 4);
+      B.d_live.ensure(B.live_host.size() * 4);
+      HAMD_CHECK(hipMemcpy(B.d_delta.p, B.delta_host.data(), B.delta_host.size() * 4,
+                           hipMemcpyHostToDevice));
+      HAMD_CHECK(hipMemcpy(B.d_live.p, B.live_host.data(), B.live_host.size() * 4,
+                           hipMemcpyHostToDevice));
+    }
+    // flagger state
+    const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
+    B.f_part.ensure((size_t)B.nch * nblk * 8);
+    B.f_bp.ensure((size_t)B.nch * 4);
+    alloc_zero(B.f_scal, sizeof(FScal), s);
+    alloc_zero(B.f_count, (size_t)B.nch * 4, s);
+    B.f_ts.ensure((size_t)W * 4);
+    // search state
+    const int sblk = (B.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
+    const size_t ncnt = (size_t)c->nw * B.rows * sblk;
+    B.s_part.ensure(ncnt * 8);
+    alloc_zero(B.s_scal, sizeof(SScal), s);
+    B.s_counts.ensure(ncnt * 4);
+    B.s_offs.ensure(ncnt * 4);
+    B.s_tiles.ensure(((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4);
+    B.s_cands.ensure((size_t)cfg.cand_cap * sizeof(Cand));
+    alloc_zero(B.s_ncand, 4, s);
+    c->max_nch = std::max(c->max_nch, B.nch);
+    c->max_rows = std::max(c->max_rows, B.rows);
+  }
+  c->gulp_bytes = in_off;
+  // tile shape shared by all beams: the preferred lane count whose LDS
+  // footprint fits every beam with the largest channel chunk
+  if (!c->tile_forced) {
+    DDTile best{64, 8};
+    for (int G : {16, 32, 64, 8}) {
+      int cc_ok = 0;
+      for (int cc : {32, 16, 8}) {
+        bool fits = true;
+        for (auto &B : c->beams) {
+          if (B.nlive == 0) continue;
+          const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
+          if (dd_lds_bytes(DDTile{G, cc}, sp) > 64 * 1024) { fits = false; break; }
+        }
+        if (fits) { cc_ok = cc; break; }
+      }
+      if (cc_ok) { best = DDTile{G, cc_ok}; break; }
+    }
+    c->tile = best;
+  }
+  for (auto &B : c->beams)
+    B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(c->tile)) : 0;
+  // shared work buffers (plain path)
+  c->xs = (int64_t)round_up((size_t)W, 64);
+  c->X.ensure((size_t)c->max_nch * c->xs * 4);
+  if (cfg.flag.nscrunch > 0) c->mask.ensure((size_t)c->max_nch * c->xs);
+  size_t dmax = 0, pmax = 0;
+  for (auto &B : c->beams) {
+    dmax = std::max(dmax, (size_t)B.ndm * B.dstride * 4);
+    pmax = std::max(pmax, (size_t)c->nw * B.rows * round_up((size_t)B.nout, 64) * 4);
+  }
+  c->D.ensure(dmax);
+  if (cfg.search_mode == SEARCH_HELLA) {
+    c->planes.ensure(pmax);
+    std::vector<float> all, k;
+    c->kofs.clear();
+    for (int i = 0; i < c->nw; ++i) {
+      hella_kernel_host(minW << i, k);
+      c->kofs.push_back((int)all.size());
+      all.insert(all.end(), k.begin(), k.end());
+    }
+    c->d_kern.ensure(all.size() * 4);
+    HAMD_CHECK(hipMemcpy(c->d_kern.p, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  }
+  // descriptors
+  const size_t nb = c->beams.size();
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_fdesc, nb * sizeof(FlagBeamDev), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, nb * sizeof(uint32_t), hipHostMallocDefault));
+  c->d_fdesc.ensure(nb * sizeof(FlagBeamDev));
+  c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
+  c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
+  c->d_in.ensure((size_t)c->gulp_bytes);
+  // search constants
+  SearchCommon &C = c->common;
+  C.nout = c->beams[0].nout;
+  C.ntime_dd = c->beams[0].ntime_dd;
+  C.maxW = maxW;
+  C.minW = minW;
+  C.nw = c->nw;
+  C.mode = cfg.search_mode;
+  C.snr = cfg.snr;
+  C.cap = cfg.cand_cap;
+  C.nblk = (C.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
+  C.wp = (C.nout / 256) * 256;
+  if (C.wp == 0) C.wp = C.nout;
+  // per-beam tile tables of the plain path (beam index 0 of a one-entry array)
+  for (auto &B : c->beams) {
+    if (B.nlive == 0) continue;
+    DDBeamDev dd{};
+    dd.ndm = B.ndm;
+    dd.nout = B.ntime_dd;
+    dd.nlive = B.nlive;
+    std::vector<DDTileDesc> tiles;
+    dd_build_tiles(std::vector<DDBeamDev>{dd}, c->tile, tiles, c->tile_order);
+    B.d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
+    HAMD_CHECK(hipMemcpy(B.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
+                         hipMemcpyHostToDevice));
+    B.ntiles = (int)tiles.size();
+  }
+  HAMD_CHECK(hipStreamSynchronize(s));
+  c->built = true;
+}
+
+// ------------------------------------------------------------------ descriptors
+static bool beam_ready(hella_ctx c, const BeamState &B)
+{
+  return c->cfg.oneshot || c->gulp_index >= B.min_gulp;
+}
+
+static int64_t write_col(hella_ctx c, const BeamState &B)
+{
+  if (c->cfg.oneshot) return 0;
+  return (int64_t)((c->gulp_index * (long long)c->W) % B.ntime);
+}
+
+static int64_t window_start(hella_ctx c, const BeamState &B)
+{
+  if (c->cfg.oneshot) return 0;
+  return (int64_t)(((c->gulp_index + 1) * (long long)c->W) % B.ntime);
+}
+
+static long long window_abs0(hella_ctx c, const BeamState &B)
+{
+  if (c->cfg.oneshot) return 0;
+  return (c->gulp_index + 1) * (long long)c->W - B.ntime;
+}
+
+static FlagBeamDev make_fdesc(hella_ctx c, BeamState &B, bool own, const void *d_gulp)
+{
+  FlagBeamDev f{};
+  f.X = own ? B.X.as<float>() : c->X.as<float>();
+  f.xs = c->xs;
+  f.nch = B.nch;
+  f.nblk = (c->W + FLAG_BLOCK - 1) / FLAG_BLOCK;
+  f.in = static_cast<const uint8_t *>(d_gulp) + B.in_off;
+  f.in_fmt = c->cfg.input_fmt;
+  f.flag1 = c->cfg.flag1;
+  f.flag2 = c->cfg.flag2;
+  f.in_stride = c->cfg.input_fmt == FLAG_IN_F16_FT ? c->W : B.nch;
+  f.ring = B.ring.as<uint8_t>();
+  f.ring_stride = B.ring_stride;
+  f.ring_len = B.ntime;
+  f.col0 = write_col(c, B);
+  f.part = B.f_part.as<double>();
+  f.bp = B.f_bp.as<float>();
+  f.scal = B.f_scal.as<FScal>();
+  f.flagcount = B.f_count.as<uint32_t>();
+  f.ts = B.f_ts.as<float>();
+  f.mask = own ? B.mask.as<uint8_t>() : c->mask.as<uint8_t>();
+  return f;
+}
+
+static DDBeamDev make_ddesc(hella_ctx c, BeamState &B, bool own)
+{
+  DDBeamDev d{};
+  d.base = B.ring.as<uint8_t>();
+  d.row_stride = B.ring_stride;
+  d.ring = B.ntime;
+  d.start = window_start(c, B);
+  d.delta = B.d_delta.as<int32_t>();
+  d.live = B.d_live.as<int32_t>();
+  d.nlive = B.nlive;
+  d.ndm = B.ndm;
+  d.nout = B.ntime_dd;
+  d.out_f32 = 0;
+  d.out = own ? B.D.p : c->D.p;
+  d.out_stride = B.dstride;
+  return d;
+}
+
+static SearchBeamDev make_sdesc(hella_ctx c, BeamState &B, bool own)
+{
+  SearchBeamDev s{};
+  s.D = own ? B.D.as<uint32_t>() : c->D.as<uint32_t>();
+  s.ds = B.dstride;
+  s.rows = B.rows;
+  s.row0 = c->cfg.search_mode == SEARCH_HELLA ? 1 : 0;
+  s.planes = own ? B.planes.as<float>() : c->planes.as<float>();
+  s.ps = (int64_t)round_up((size_t)B.nout, 64);
+  s.part = B.s_part.as<double>();
+  s.scal = B.s_scal.as<SScal>();
+  s.counts = B.s_counts.as<uint32_t>();
+  s.offs = B.s_offs.as<uint32_t>();
+  s.tiles = B.s_tiles.as<uint32_t>();
+  s.cands = B.s_cands.as<Cand>();
+  s.ncand = B.s_ncand.as<uint32_t>();
+  return s;
+}
+
+// ------------------------------------------------------------------ timing
+struct StageTimer {
+  hella_ctx c;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int idx;
+  StageTimer(hella_ctx c_, int i) : c(c_), idx(i)
+  {
+    if (!c->timing) return;
+    HAMD_CHECK(hipEventCreate(&e0));
+    HAMD_CHECK(hipEventCreate(&e1));
+    HAMD_CHECK(hipEventRecord(e0, c->stream));
+  }
+  ~StageTimer()
+  {
+    if (!c->timing) return;
+    float ms = 0.f;
+    if (hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+      c->stage_ms[(size_t)idx] += ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+};
+
+// ------------------------------------------------------------------ plain stages
+const void *ctx_stage_input(hella_ctx c, const void *gulp, bool host_ptr)
+{
+  if (!host_ptr) return gulp;
+  HAMD_CHECK(hipMemcpyAsync(c->d_in.p, gulp, (size_t)c->gulp_bytes, hipMemcpyHostToDevice,
+                            c->stream));
+  return c->d_in.p;
+}
+
+void ctx_flag_beam(hella_ctx c, int b, const void *d_gulp)
+{
+  BeamState &B = c->beams[(size_t)b];
+  StageTimer t(c, 0);
+  c->h_fdesc[b] = make_fdesc(c, B, false, d_gulp);
+  FlagBeamDev *dd = c->d_fdesc.as<FlagBeamDev>() + b;
+  HAMD_CHECK(hipMemcpyAsync(dd, &c->h_fdesc[b], sizeof(FlagBeamDev), hipMemcpyHostToDevice,
+                            c->stream));
+  FlagRunner::run(dd, 1, B.nch, c->W, c->cfg.flag, c->stream);
+}
+
+void ctx_dedisperse(hella_ctx c, int b)
+{
+  BeamState &B = c->beams[(size_t)b];
+  StageTimer t(c, 1);
+  c->h_ddesc[b] = make_ddesc(c, B, false);
+  if (B.nlive == 0) {
+    fill_zero_rows(c->D.p, B.dstride, B.ntime_dd, B.ndm, c->stream);
+  } else {
+    DDBeamDev *dd = c->d_ddesc.as<DDBeamDev>() + b;
+    HAMD_CHECK(hipMemcpyAsync(dd, &c->h_ddesc[b], sizeof(DDBeamDev), hipMemcpyHostToDevice,
+                              c->stream));
+    dd_launch(dd, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile, B.spread, c->stream);
+  }
+  c->cur_beam = b;
+}
+
+void ctx_smooth(hella_ctx c, int b)
+{
+  BeamState &B = c->beams[(size_t)b];
+  StageTimer t(c, 2);
+  c->h_sdesc[b] = make_sdesc(c, B, false);
+  SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
+  HAMD_CHECK(hipMemcpyAsync(sd, &c->h_sdesc[b], sizeof(SearchBeamDev), hipMemcpyHostToDevice,
+                            c->stream));
+  SearchRunner::smooth(sd, 1, B.rows, c->common, c->d_kern.as<float>(), c->kofs, c->stream);
+}
+
+// append beam b's first candidates to the gulp list under the caps of
+// peaks.cu:295-296 (per beam) and peaks.cu:319-334 (per gulp)
+static void append_peaks(hella_ctx c, int b, uint32_t ncand, const std::vector<Cand> &cs)
+{
+  BeamState &B = c->beams[(size_t)b];
+  B.crossings = ncand;
+  const long long a0 = window_abs0(c, B) + c->cfg.width_max / 2;
+  for (const Cand &x : cs) {
+    if ((int)c->peaks.size() >= c->cfg.max_giants) break;
+    hella_candidate h;
+    h.snr = x.snr;
+    h.beam = b;
+    h.samp = x.samp;
+    h.width = x.width;
+    h.dm_idx = x.dm_idx;
+    h.dm = B.plan.dm_list[(size_t)x.dm_idx];
+    h.abs_samp = a0 + x.samp;
+    c->peaks.push_back(h);
+  }
+}
+
+static size_t beam_take(hella_ctx c, uint32_t ncand)
+{
+  size_t n = std::min<size_t>(ncand, (size_t)c->cfg.cand_cap);
+  return std::min<size_t>(n, (size_t)c->cfg.max_giants);
+}
+
+int ctx_find_peaks(hella_ctx c, int b)
+{
+  BeamState &B = c->beams[(size_t)b];
+  uint32_t nc = 0;
+  std::vector<Cand> cs;
+  {
+    StageTimer t(c, 3);
+    SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
+    SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
+    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[b], B.s_ncand.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HAMD_CHECK(hipStreamSynchronize(c->stream));
+    nc = c->h_ncand[b];
+    cs.resize(beam_take(c, nc));
+    if (!cs.empty()) {
+      HAMD_CHECK(hipMemcpyAsync(cs.data(), B.s_cands.p, cs.size() * sizeof(Cand),
+                                hipMemcpyDeviceToHost, c->stream));
+      HAMD_CHECK(hipStreamSynchronize(c->stream));
+    }
+  }
+  append_peaks(c, b, nc, cs);
+  return (int)cs.size();
+}
+
+void ctx_end_gulp(hella_ctx c) { c->gulp_index += 1; }
+
+void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  c->batch_mode = false;
+  c->stage_ms.fill(0.f);
+  c->peaks.clear();
+  c->clusters.clear();
+  const void *d_gulp = ctx_stage_input(c, gulp, host_ptr);
+  for (size_t b = 0; b < c->beams.size(); ++b) ctx_flag_beam(c, (int)b, d_gulp);
+  for (size_t b = 0; b < c->beams.size(); ++b) {
+    BeamState &B = c->beams[b];
+    if (!beam_ready(c, B)) continue;
+    if ((int)c->peaks.size() >= c->cfg.max_giants) break;
+    const double el =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (el >= c->cfg.time_limit) break;
+    ctx_dedisperse(c, (int)b);
+    ctx_smooth(c, (int)b);
+    ctx_find_peaks(c, (int)b);
+  }
+  HAMD_CHECK(hipStreamSynchronize(c->stream));
+  ctx_cluster(c);
+  ctx_end_gulp(c);
+}
+
+// ------------------------------------------------------------------ batched path
+static void batch_alloc(hella_ctx c)
+{
+  for (auto &B : c->beams) {
+    B.X.ensure((size_t)B.nch * c->xs * 4);
+    if (c->cfg.flag.nscrunch > 0) B.mask.ensure((size_t)B.nch * c->xs);
+    B.D.ensure((size_t)B.ndm * B.dstride * 4);
+    if (c->cfg.search_mode == SEARCH_HELLA)
+      B.planes.ensure((size_t)c->nw * B.rows * round_up((size_t)B.nout, 64) * 4);
+  }
+}
+
+static void batch_destroy_graph(Batch &bt)
+{
+  if (bt.exec) (void)hipGraphExecDestroy(bt.exec);
+  bt.exec = nullptr;
+  bt.graph_ok = false;
+}
+
+// (re)build the searchable-beam set, its tile table and descriptors
+static void batch_prepare(hella_ctx c)
+{
+  std::vector<int> ready;
+  for (size_t b = 0; b < c->beams.size(); ++b)
+    if (beam_ready(c, c->beams[b])) ready.push_back((int)b);
+  Batch &bt = c->batch;
+  if (bt.valid && ready == bt.beams) return;
+  batch_destroy_graph(bt);
+  bt.beams = ready;
+  std::vector<DDBeamDev> dd;
+  int spread = 0;
+  for (int b : ready) {
+    dd.push_back(make_ddesc(c, c->beams[(size_t)b], true));
+    spread = std::max(spread, c->beams[(size_t)b].spread);
+  }
+  std::vector<DDTileDesc> tiles;
+  dd_build_tiles(dd, c->tile, tiles, c->tile_order);
+  bt.d_tiles.ensure(std::max<size_t>(tiles.size(), 1) * sizeof(DDTileDesc));
+  if (!tiles.empty())
+    HAMD_CHECK(hipMemcpy(bt.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
+                         hipMemcpyHostToDevice));
+  bt.ntiles = (int)tiles.size();
+  bt.tile = c->tile;
+  bt.spread = spread;
+  bt.valid = true;
+}
+
+// record the device work of one batched gulp on the context stream
+static void batch_record(hella_ctx c)
+{
+  Batch &bt = c->batch;
+  const size_t nb = c->beams.size();
+  hipStream_t s = c->stream;
+  HAMD_CHECK(hipMemcpyAsync(c->d_fdesc.p, c->h_fdesc, nb * sizeof(FlagBeamDev),
+                            hipMemcpyHostToDevice, s));
+  FlagRunner::run(c->d_fdesc.as<FlagBeamDev>(), (int)nb, c->max_nch, c->W, c->cfg.flag, s);
+  const int nr = (int)bt.beams.size();
+  if (nr == 0) return;
+  HAMD_CHECK(hipMemcpyAsync(c->d_ddesc.p, c->h_ddesc, (size_t)nr * sizeof(DDBeamDev),
+                            hipMemcpyHostToDevice, s));
+  HAMD_CHECK(hipMemcpyAsync(c->d_sdesc.p, c->h_sdesc, (size_t)nr * sizeof(SearchBeamDev),
+                            hipMemcpyHostToDevice, s));
+  for (int i = 0; i < nr; ++i) {
+    const BeamState &B = c->beams[(size_t)bt.beams[(size_t)i]];
+    if (B.nlive == 0) fill_zero_rows(B.D.p, B.dstride, B.ntime_dd, B.ndm, s);
+  }
+  dd_launch(c->d_ddesc.as<DDBeamDev>(), bt.d_tiles.as<DDTileDesc>(), bt.ntiles, bt.tile, bt.spread,
+            s);
+  int max_rows = 0;
+  for (int b : bt.beams) max_rows = std::max(max_rows, c->beams[(size_t)b].rows);
+  SearchRunner::run(c->d_sdesc.as<SearchBeamDev>(), nr, max_rows, c->common,
+                    c->d_kern.as<float>(), c->kofs, s);
+  for (int i = 0; i < nr; ++i) {
+    const BeamState &B = c->beams[(size_t)bt.beams[(size_t)i]];
+    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[i], B.s_ncand.p, 4, hipMemcpyDeviceToHost, s));
+  }
+}
+
+void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
+{
+  c->batch_mode = true;
+  c->stage_ms.fill(0.f);
+  c->peaks.clear();
+  c->clusters.clear();
+  batch_alloc(c);
+  batch_prepare(c);
+  Batch &bt = c->batch;
+  const void *d_gulp = ctx_stage_input(c, gulp, host_ptr);
+  // refresh the per-gulp descriptor values (read by the graph's copy nodes)
+  for (size_t b = 0; b < c->beams.size(); ++b)
+    c->h_fdesc[b] = make_fdesc(c, c->beams[b], true, d_gulp);
+  for (size_t i = 0; i < bt.beams.size(); ++i) {
+    BeamState &B = c->beams[(size_t)bt.beams[i]];
+    c->h_ddesc[i] = make_ddesc(c, B, true);
+    c->h_sdesc[i] = make_sdesc(c, B, true);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    HAMD_CHECK(hipEventCreate(&e0));
+    HAMD_CHECK(hipEventCreate(&e1));
+    HAMD_CHECK(hipEventRecord(e0, c->stream));
+  }
+  if (!bt.graph_ok) {
+    hipGraph_t g = nullptr;
+    HAMD_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      batch_record(c);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HAMD_CHECK(hipStreamEndCapture(c->stream, &g));
+    HAMD_CHECK(hipGraphInstantiate(&bt.exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    bt.graph_ok = true;
+  }
+  HAMD_CHECK(hipGraphLaunch(bt.exec, c->stream));
+  if (c->timing) HAMD_CHECK(hipEventRecord(e1, c->stream));
+  HAMD_CHECK(hipStreamSynchronize(c->stream));
+  if (c->timing) {
+    float ms = 0.f;
+    HAMD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    c->stage_ms[0] = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  // candidates: only beams with crossings are copied back
+  std::vector<std::vector<Cand>> per(bt.beams.size());
+  for (size_t i = 0; i < bt.beams.size(); ++i) {
+    const size_t n = beam_take(c, c->h_ncand[i]);
+    if (!n) continue;
+    per[i].resize(n);
+    HAMD_CHECK(hipMemcpyAsync(per[i].data(), c->beams[(size_t)bt.beams[i]].s_cands.p,
+                              n * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
+  }
+  HAMD_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < bt.beams.size(); ++i) {
+    if ((int)c->peaks.size() >= c->cfg.max_giants) break;
+    append_peaks(c, bt.beams[i], c->h_ncand[i], per[i]);
+  }
+  ctx_cluster(c);
+  ctx_end_gulp(c);
+}
+
+// ------------------------------------------------------------------ clustering
+// Cells of CLUSTER_DT samples x CLUSTER_DDM DM trials per beam; 8-connected
+// occupied cells form one cluster, represented by its highest-S/N member
+// (lowest list index on ties); clusters ordered by representative index.
+void ctx_cluster(hella_ctx c)
+{
+  c->clusters.clear();
+  const int n = (int)c->peaks.size();
+  if (!n) return;
+  const int ct = std::max(1, c->cfg.cluster_dt), cd = std::max(1, c->cfg.cluster_ddm);
+  struct KI { int64_t beam; int64_t key; int idx; };
+  auto key = [&](int64_t cdm, int64_t cts) { return (cdm << 32) | (int64_t)(uint32_t)cts; };
+  std::vector<KI> v((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const hella_candidate &h = c->peaks[(size_t)i];
+    v[(size_t)i] = KI{h.beam, key(h.dm_idx / cd, h.samp / ct), i};
+  }
+  std::sort(v.begin(), v.end(), [](const KI &a, const KI &b) {
+    if (a.beam != b.beam) return a.beam < b.beam;
+    if (a.key != b.key) return a.key < b.key;
+    return a.idx < b.idx;
+  });
+  std::vector<std::pair<int64_t, int64_t>> cells;  // (beam, key)
+  std::vector<int> cell_of((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    if (i == 0 || v[(size_t)i].beam != v[(size_t)i - 1].beam || v[(size_t)i].key != v[(size_t)i - 1].key)
+      cells.emplace_back(v[(size_t)i].beam, v[(size_t)i].key);
+    cell_of[(size_t)v[(size_t)i].idx] = (int)cells.size() - 1;
+  }
+  const int nc = (int)cells.size();
+  std::vector<int> par((size_t)nc);
+  for (int i = 0; i < nc; ++i) par[(size_t)i] = i;
+  std::function<int(int)> find = [&](int x) {
+    while (par[(size_t)x] != x) {
+      par[(size_t)x] = par[(size_t)par[(size_t)x]];
+      x = par[(size_t)x];
+    }
+    return x;
+  };
+  for (int i = 0; i < nc; ++i) {
+    const int64_t bm = cells[(size_t)i].first, k = cells[(size_t)i].second;
+    const int64_t cdm = k >> 32, cts = (int32_t)(uint32_t)k;
+    for (int dd = -1; dd <= 1; ++dd)
+      for (int dt = -1; dt <= 1; ++dt) {
+        if (!dd && !dt) continue;
+        const auto it = std::lower_bound(cells.begin(), cells.end(),
+                                         std::make_pair(bm, key(cdm + dd, cts + dt)));
+        if (it != cells.end() && it->first == bm && it->second == key(cdm + dd, cts + dt)) {
+          int a = find(i), b = find((int)(it - cells.begin()));
+          if (a != b) par[(size_t)std::max(a, b)] = std::min(a, b);
+        }
+      }
+  }
+  std::vector<int> rep((size_t)nc, -1);
+  for (int i = 0; i < n; ++i) {
+    const int r = find(cell_of[(size_t)i]);
+    const int cur = rep[(size_t)r];
+    if (cur < 0 || c->peaks[(size_t)i].snr > c->peaks[(size_t)cur].snr) rep[(size_t)r] = i;
+  }
+  std::vector<int> reps;
+  for (int i = 0; i < nc; ++i)
+    if (find(i) == i && rep[(size_t)i] >= 0) reps.push_back(rep[(size_t)i]);
+  std::sort(reps.begin(), reps.end());
+  std::vector<int> slot((size_t)nc, -1);
+  for (size_t q = 0; q < reps.size(); ++q) {
+    slot[(size_t)find(cell_of[(size_t)reps[q]])] = (int)q;
+    hella_cluster h{};
+    h.rep = reps[q];
+    h.samp_min = h.dm_min = 0x7fffffff;
+    h.samp_max = h.dm_max = -1;
+    h.beam = c->peaks[(size_t)reps[q]].beam;
+    c->clusters.push_back(h);
+  }
+  for (int i = 0; i < n; ++i) {
+    hella_cluster &h = c->clusters[(size_t)slot[(size_t)find(cell_of[(size_t)i])]];
+    const hella_candidate &p = c->peaks[(size_t)i];
+    h.count++;
+    h.samp_min = std::min(h.samp_min, p.samp);
+    h.samp_max = std::max(h.samp_max, p.samp);
+    h.dm_min = std::min(h.dm_min, p.dm_idx);
+    h.dm_max = std::max(h.dm_max, p.dm_idx);
+  }
+}
+
+// ------------------------------------------------------------------ autotune
+void ctx_autotune(hella_ctx c, const char *path)
+{
+  ctx_build(c);
+  batch_alloc(c);
+  // time the dedispersion of every beam's current window with each shape
+  std::vector<DDBeamDev> dd;
+  for (auto &B : c->beams)
+    if (B.nlive > 0) dd.push_back(make_ddesc(c, B, true));
+  if (dd.empty()) return;
+  DevBuf d_beams;
+  d_beams.ensure(dd.size() * sizeof(DDBeamDev));
+  HAMD_CHECK(hipMemcpy(d_beams.p, dd.data(), dd.size() * sizeof(DDBeamDev), hipMemcpyHostToDevice));
+  struct Res { int G, cc, order; float ms; };
+  std::vector<Res> res;
+  hipEvent_t e0, e1;
+  HAMD_CHECK(hipEventCreate(&e0));
+  HAMD_CHECK(hipEventCreate(&e1));
+  DevBuf d_tiles;
+  for (int G : {8, 16, 32, 64})
+    for (int cc : {8, 16, 32})
+      for (int order : {0, 1}) {
+        DDTile t{G, cc};
+        int spread = 0;
+        bool fits = true;
+        for (auto &B : c->beams) {
+          if (!B.nlive) continue;
+          const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
+          spread = std::max(spread, sp);
+          if (dd_lds_bytes(t, sp) > 64 * 1024) fits = false;
+        }
+        if (!fits) continue;
+        std::vector<DDTileDesc> tiles;
+        dd_build_tiles(dd, t, tiles, order);
+        d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
+        HAMD_CHECK(hipMemcpy(d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
+                             hipMemcpyHostToDevice));
+        dd_launch(d_beams.as<DDBeamDev>(), d_tiles.as<DDTileDesc>(), (int)tiles.size(), t, spread,
+                  c->stream);
+        HAMD_CHECK(hipEventRecord(e0, c->stream));
+        const int reps = 3;
+        for (int r = 0; r < reps; ++r)
+          dd_launch(d_beams.as<DDBeamDev>(), d_tiles.as<DDTileDesc>(), (int)tiles.size(), t,
+                    spread, c->stream);
+        HAMD_CHECK(hipEventRecord(e1, c->stream));
+        HAMD_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HAMD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        res.push_back(Res{G, cc, order, ms / reps});
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (res.empty()) return;
+  const Res best = *std::min_element(res.begin(), res.end(),
+                                     [](const Res &a, const Res &b) { return a.ms < b.ms; });
+  c->tile = DDTile{best.G, best.cc};
+  c->tile_order = best.order;
+  c->tile_forced = true;
+  for (auto &B : c->beams) {
+    B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(c->tile)) : 0;
+    if (!B.nlive) continue;
+    DDBeamDev one{};
+    one.ndm = B.ndm;
+    one.nout = B.ntime_dd;
+    one.nlive = B.nlive;
+    std::vector<DDTileDesc> tiles;
+    dd_build_tiles(std::vector<DDBeamDev>{one}, c->tile, tiles, c->tile_order);
+    B.d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
+    HAMD_CHECK(hipMemcpy(B.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
+                         hipMemcpyHostToDevice));
+    B.ntiles = (int)tiles.size();
+  }
+  c->batch.valid = false;
+  batch_destroy_graph(c->batch);
+  if (path && *path) {
+    FILE *f = std::fopen(path, "w");
+    if (f) {
+      std::fprintf(f, "{\"best\": {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"ms\": %.4f},\n",
+                   best.G, best.cc, best.order, best.ms);
+      std::fprintf(f, " \"nbeam\": %zu, \"nchan_max\": %d, \"gulp\": %d,\n \"trials\": [", c->beams.size(),
+                   c->max_nch, c->W);
+      for (size_t i = 0; i < res.size(); ++i)
+        std::fprintf(f, "%s\n  {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"ms\": %.4f}",
+                     i ? "," : "", res[i].G, res[i].cc, res[i].order, res[i].ms);
+      std::fprintf(f, "]}\n");
+      std::fclose(f);
+    }
+  }
+}
+
+}  // namespace hamd

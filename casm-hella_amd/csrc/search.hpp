@@ -1,0 +1,78 @@
+// search.hpp -- boxcar / hella-kernel search over dedispersed planes:
+// per-width statistics, normalisation, threshold and ordered compaction.
+#pragma once
+
+#include "internal.hpp"
+
+namespace hamd {
+
+constexpr int SEARCH_BLOCK = 4096;  // canonical reduction block (samples)
+constexpr int SEARCH_MAXW = 16;     // maximum number of width trials
+
+enum SearchMode { SEARCH_BOXCAR = 0, SEARCH_HELLA = 1 };
+
+struct Cand {
+  float snr;
+  int samp;    // output sample index within the gulp
+  int width;   // width trial index
+  int dm_idx;  // DM trial index
+};
+
+struct SScal {
+  double mu[SEARCH_MAXW];
+  double inv[SEARCH_MAXW];
+  float sigf[SEARCH_MAXW];
+  double mu0;      // mean of the normalised first width plane
+  float sig0;      // its standard deviation
+  float thr;       // snr * sig0
+  int gated;       // 1 when sig0 is outside (0.1, 5): no candidates
+  int pad;
+};
+
+struct SearchBeamDev {
+  const uint32_t *D;  // dedispersed sums [ndm][ds]
+  int64_t ds;
+  int rows;           // searched rows (boxcar: ndm; hella: ndm - 2)
+  int row0;           // first D row searched
+  float *planes;      // hella mode: raw smoothed planes [nw][rows][ps]
+  int64_t ps;
+  double *part;       // [nw][rows][nblk]
+  SScal *scal;
+  uint32_t *counts;   // [nw][rows][nblk]
+  uint32_t *offs;     // exclusive scan of counts
+  uint32_t *tiles;    // scan scratch [ceil(n/SEARCH_BLOCK)]
+  Cand *cands;        // ordered candidates (first cap stored)
+  uint32_t *ncand;    // total crossings (uncapped)
+};
+
+struct SearchCommon {
+  int nout;      // searched samples per row
+  int ntime_dd;  // dedispersed samples available per row
+  int maxW, minW;
+  int nw;
+  int mode;
+  float snr;
+  int cap;       // candidate buffer capacity per beam
+  int nblk;      // ceil(nout / SEARCH_BLOCK)
+  int wp;        // statistics columns
+};
+
+int search_nwidth(int mode, int minW, int maxW);
+void hella_kernel_host(int sm, std::vector<float> &k);
+
+struct SearchRunner {
+  // Full search for nbeam beams (device descriptors).  max_rows is the
+  // largest row count.  kern: hella kernels on the device (mode HELLA),
+  // concatenated per width; kofs: their offsets (host).
+  static void run(const SearchBeamDev *d_beams, int nbeam, int max_rows, const SearchCommon &C,
+                  const float *d_kern, const std::vector<int> &kofs, hipStream_t s);
+  // the two halves of run(): width trials + per-width statistics (smooth.cpp),
+  // then sigma gate, threshold and ordered compaction (peaks.cu)
+  static void smooth(const SearchBeamDev *d_beams, int nbeam, int max_rows,
+                     const SearchCommon &C, const float *d_kern, const std::vector<int> &kofs,
+                     hipStream_t s);
+  static void peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows, const SearchCommon &C,
+                    hipStream_t s);
+};
+
+}  // namespace hamd

@@ -1,0 +1,451 @@
+"""ctypes binding of libhella_amd.so (the C ABI declared in include/dedisp.h
+and include/hella.h).
+
+The library is the product: every compute call below runs hand-written HIP
+kernels for gfx950.  Loading fails loudly if the library has not been built
+(``python -c "import __graft_entry__ as g; g.build()"``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhella_amd.so")
+
+_lib = None
+
+
+def lib():
+    """Load libhella_amd.so once; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libhella_amd.so not built: {LIB_PATH}")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+# dedisp flags / errors (include/dedisp.h)
+DEDISP_NO_ERROR = 0
+DEDISP_HOST_POINTERS = 1 << 1
+DEDISP_DEVICE_POINTERS = 1 << 2
+DEDISP_WAIT = 1 << 3
+DEDISP_ASYNC = 1 << 4
+
+_VP = C.c_void_p
+_SZ = C.c_ulong
+_F = C.c_float
+_I = C.c_int
+
+
+def _declare(L):
+    def d(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    d("dedisp_create_plan", _I, C.POINTER(_VP), _SZ, _F, _F, _F)
+    d("dedisp_destroy_plan", None, _VP)
+    d("dedisp_generate_dm_list", _I, _VP, _F, _F, _F, _F)
+    d("dedisp_set_dm_list", _I, _VP, C.POINTER(_F), _SZ)
+    d("dedisp_set_killmask", _I, _VP, C.POINTER(_I))
+    d("dedisp_set_gulp_size", _I, _VP, _SZ)
+    d("dedisp_get_gulp_size", _SZ, _VP)
+    d("dedisp_get_dm_list", C.POINTER(_F), _VP)
+    d("dedisp_get_dm_count", _SZ, _VP)
+    d("dedisp_get_max_delay", _SZ, _VP)
+    d("dedisp_get_dm_delay", _SZ, _VP, _I)
+    d("dedisp_get_channel_count", _SZ, _VP)
+    d("dedisp_get_killmask", C.POINTER(_I), _VP)
+    d("dedisp_get_dt", _F, _VP)
+    d("dedisp_get_f0", _F, _VP)
+    d("dedisp_get_df", _F, _VP)
+    d("dedisp_execute_adv", _I, _VP, _SZ, _VP, _SZ, _SZ, _VP, _SZ, _SZ, C.c_uint)
+    d("dedisp_execute", _I, _VP, _SZ, _VP, _SZ, _VP, _SZ, C.c_uint)
+    d("dedisp_execute_guru", _I, _VP, _SZ, _VP, _SZ, _SZ, _VP, _SZ, _SZ, _SZ, _SZ, C.c_uint)
+    d("dedisp_set_device", _I, _I)
+    d("dedisp_sync", _I)
+    d("dedisp_get_error_string", C.c_char_p, _I)
+    d("dedisp_set_stream", _I, _VP, _VP)
+
+
+class DedispError(RuntimeError):
+    pass
+
+
+def _chk(code):
+    if code != DEDISP_NO_ERROR:
+        msg = lib().dedisp_get_error_string(code).decode()
+        raise DedispError(f"dedisp error {code}: {msg}")
+
+
+class DedispPlan:
+    """Python view of a dedisp_plan (create / DM list / killmask / execute)."""
+
+    def __init__(self, nchans, dt, f0, df):
+        self._p = _VP()
+        _chk(lib().dedisp_create_plan(C.byref(self._p), nchans, dt, f0, df))
+        self.nchans = nchans
+
+    def close(self):
+        if self._p:
+            lib().dedisp_destroy_plan(self._p)
+            self._p = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate_dm_list(self, dm_start, dm_end, ti_us, tol):
+        _chk(lib().dedisp_generate_dm_list(self._p, dm_start, dm_end, ti_us, tol))
+
+    def set_dm_list(self, dms):
+        a = np.ascontiguousarray(dms, dtype=np.float32)
+        _chk(lib().dedisp_set_dm_list(self._p, a.ctypes.data_as(C.POINTER(_F)), a.size))
+
+    def set_killmask(self, mask):
+        if mask is None:
+            _chk(lib().dedisp_set_killmask(self._p, None))
+            return
+        a = np.ascontiguousarray(mask, dtype=np.int32)
+        _chk(lib().dedisp_set_killmask(self._p, a.ctypes.data_as(C.POINTER(_I))))
+
+    @property
+    def dm_count(self):
+        return int(lib().dedisp_get_dm_count(self._p))
+
+    @property
+    def max_delay(self):
+        return int(lib().dedisp_get_max_delay(self._p))
+
+    def dm_list(self):
+        n = self.dm_count
+        ptr = lib().dedisp_get_dm_list(self._p)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+
+    def execute_adv(self, data_tf, nsamps=None, in_stride=None, out=None, flags=0):
+        """Host-pointer execute: data_tf is u8 [nsamps][in_stride]."""
+        data_tf = np.ascontiguousarray(data_tf, dtype=np.uint8)
+        if nsamps is None:
+            nsamps = data_tf.shape[0]
+        if in_stride is None:
+            in_stride = data_tf.shape[1]
+        nout = nsamps - self.max_delay
+        if out is None:
+            out = np.zeros((self.dm_count, nout), dtype=np.float32)
+        _chk(lib().dedisp_execute_adv(self._p, nsamps, data_tf.ctypes.data, 8, in_stride,
+                                      out.ctypes.data, 32, out.strides[0], flags))
+        return out
+
+    def execute_guru(self, data_tf, first_dm, dm_count, flags=0):
+        data_tf = np.ascontiguousarray(data_tf, dtype=np.uint8)
+        nsamps, in_stride = data_tf.shape
+        nout = nsamps - self.max_delay
+        out = np.zeros((dm_count, nout), dtype=np.float32)
+        _chk(lib().dedisp_execute_guru(self._p, nsamps, data_tf.ctypes.data, 8, in_stride,
+                                       out.ctypes.data, 32, out.strides[0], first_dm, dm_count,
+                                       flags))
+        return out
+
+
+# ---------------------------------------------------------------- hella pipeline
+class HellaError(RuntimeError):
+    pass
+
+
+class _Info(C.Structure):
+    _fields_ = [("nchan", _I), ("ndm", _I), ("max_delay", C.c_longlong), ("ntime", C.c_longlong),
+                ("ntime_dd", _I), ("ntime_out", _I), ("nwidth", _I), ("gulp", C.c_longlong),
+                ("min_gulp", _I), ("gulp_index", C.c_longlong), ("rows", _I), ("nlive", _I)]
+
+
+class _Cand(C.Structure):
+    _fields_ = [("snr", _F), ("beam", _I), ("samp", _I), ("width", _I), ("dm_idx", _I),
+                ("dm", _F), ("abs_samp", C.c_longlong)]
+
+
+class _Cluster(C.Structure):
+    _fields_ = [("rep", _I), ("count", _I), ("samp_min", _I), ("samp_max", _I),
+                ("dm_min", _I), ("dm_max", _I), ("beam", _I)]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("seed", C.c_ulonglong), ("nchan", _I), ("nsamp", C.c_longlong),
+                ("t_offset", C.c_longlong), ("fch1", C.c_double), ("foff", C.c_double),
+                ("tsamp", C.c_double), ("dtype", _I), ("mean", _F), ("sigma", _F),
+                ("n_rfi_chan", _I), ("rfi_chan", C.POINTER(_I)), ("rfi_sigma_mult", _F),
+                ("n_pulse", _I), ("pulse_t", C.POINTER(C.c_double)), ("pulse_dm", C.POINTER(_F)),
+                ("pulse_fwhm_ms", C.POINTER(_F)), ("pulse_snr", C.POINTER(_F)),
+                ("burst_period", C.c_longlong), ("burst_phase", C.c_longlong), ("burst_len", _I),
+                ("burst_amp", _F), ("bp_slope", _F)]
+
+
+CAND_DTYPE = np.dtype([("snr", "f4"), ("beam", "i4"), ("samp", "i4"), ("width", "i4"),
+                       ("dm_idx", "i4"), ("dm", "f4"), ("abs_samp", "i8")])
+CLUSTER_DTYPE = np.dtype([("rep", "i4"), ("count", "i4"), ("samp_min", "i4"), ("samp_max", "i4"),
+                          ("dm_min", "i4"), ("dm_max", "i4"), ("beam", "i4")])
+
+
+def _declare_hella(L):
+    def d(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    P = _VP
+    d("hella_create", _I, C.c_char_p, C.POINTER(P))
+    d("hella_destroy", None, P)
+    d("hella_last_error", C.c_char_p)
+    d("hella_set_beam", _I, P, _I, C.c_char_p)
+    d("hella_set_killmask", _I, P, _I, C.POINTER(_I))
+    d("hella_get_info", _I, P, _I, C.POINTER(_Info))
+    d("hella_get_dm_list", _I, P, _I, C.POINTER(_F), _I, C.POINTER(_I))
+    d("hella_get_stream", P, P)
+    d("hella_fast_flagger", _I, P, P, _I)
+    d("hella_dedisperse", _I, P, _I)
+    d("hella_smooth", _I, P, _I)
+    d("hella_find_peaks", _I, P, _I)
+    d("hella_clear_peaks", _I, P)
+    d("hella_end_gulp", _I, P)
+    d("hella_process_gulp", _I, P, P, _I, C.POINTER(_I))
+    d("hella_process_gulp_batched", _I, P, P, _I, C.POINTER(_I))
+    d("hella_get_peaks", _I, P, C.POINTER(_Cand), _I)
+    d("hella_get_clusters", _I, P, C.POINTER(_Cluster), _I)
+    d("hella_last_crossings", C.c_longlong, P, _I)
+    d("hella_copy_window", _I, P, _I, P, C.c_size_t)
+    d("hella_copy_dedisp", _I, P, _I, P, C.c_size_t)
+    d("hella_get_search_stats", _I, P, _I, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F),
+      C.POINTER(_I))
+    d("hella_stage_times", _I, P, C.POINTER(_F))
+    d("hella_set_timing", _I, P, _I)
+    d("hella_autotune", _I, P, C.c_char_p)
+    d("hella_set_tile", _I, P, _I, _I, _I)
+    d("hella_device_count", _I)
+    d("hella_dev_alloc", P, C.c_size_t)
+    d("hella_dev_free", None, P)
+    d("hella_memcpy", _I, P, P, C.c_size_t, _I)
+    d("hella_device_sync", _I)
+    d("hella_synth_filterbank", _I, C.POINTER(SynthParams), P)
+
+
+_orig_declare = _declare
+
+
+def _declare(L):  # noqa: F811 -- extend the dedisp declarations
+    _orig_declare(L)
+    _declare_hella(L)
+
+
+def _hchk(code):
+    if code != 0:
+        raise HellaError(f"hella error {code}: {lib().hella_last_error().decode()}")
+
+
+def config_text(cfg):
+    """dict -> KEY VALUE text; SCRUNCH takes a list of (ts, fs, thresh)."""
+    if isinstance(cfg, str):
+        return cfg
+    lines = []
+    for k, v in cfg.items():
+        if k == "SCRUNCH":
+            lines.append(f"SCRUNCH {len(v)}")
+            lines += [f"{ts} {fs} {th} 1" for ts, fs, th in v]
+        else:
+            lines.append(f"{k} {v}")
+    return "\n".join(lines) + "\n"
+
+
+class DeviceBuffer:
+    """Device memory from the library's allocator (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().hella_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise HellaError("device allocation failed")
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _hchk(lib().hella_memcpy(self.ptr, a.ctypes.data, a.nbytes, 1))
+
+    def download(self, arr):
+        _hchk(lib().hella_memcpy(arr.ctypes.data, self.ptr, arr.nbytes, 2))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            lib().hella_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Hella:
+    """One pipeline context (hella_create ... hella_destroy)."""
+
+    def __init__(self, cfg):
+        self._c = _VP()
+        _hchk(lib().hella_create(config_text(cfg).encode(), C.byref(self._c)))
+
+    def close(self):
+        if self._c:
+            lib().hella_destroy(self._c)
+            self._c = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_beam(self, beam, **kw):
+        _hchk(lib().hella_set_beam(self._c, beam, config_text(kw).encode()))
+
+    def set_killmask(self, beam, mask):
+        a = np.ascontiguousarray(mask, np.int32)
+        _hchk(lib().hella_set_killmask(self._c, beam, a.ctypes.data_as(C.POINTER(_I))))
+
+    def set_tile(self, lanes, chunk, order=0):
+        _hchk(lib().hella_set_tile(self._c, lanes, chunk, order))
+
+    def set_timing(self, on=True):
+        _hchk(lib().hella_set_timing(self._c, 1 if on else 0))
+
+    def info(self, beam=0):
+        i = _Info()
+        _hchk(lib().hella_get_info(self._c, beam, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _Info._fields_}
+
+    def dm_list(self, beam=0):
+        n = _I()
+        _hchk(lib().hella_get_dm_list(self._c, beam, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _hchk(lib().hella_get_dm_list(self._c, beam, out.ctypes.data_as(C.POINTER(_F)), n.value,
+                                      C.byref(n)))
+        return out
+
+    @staticmethod
+    def _ptr(gulp):
+        if isinstance(gulp, DeviceBuffer):
+            return gulp.ptr, 0
+        a = np.ascontiguousarray(gulp)
+        return a.ctypes.data, 1, a
+
+    def process_gulp(self, gulp, batched=False):
+        keep = self._ptr(gulp)
+        n = _I()
+        f = lib().hella_process_gulp_batched if batched else lib().hella_process_gulp
+        _hchk(f(self._c, keep[0], keep[1], C.byref(n)))
+        return n.value
+
+    def fast_flagger(self, gulp):
+        keep = self._ptr(gulp)
+        _hchk(lib().hella_fast_flagger(self._c, keep[0], keep[1]))
+
+    def dedisperse(self, beam):
+        _hchk(lib().hella_dedisperse(self._c, beam))
+
+    def smooth(self, scale=1):
+        _hchk(lib().hella_smooth(self._c, scale))
+
+    def find_peaks(self, beam):
+        _hchk(lib().hella_find_peaks(self._c, beam))
+
+    def clear_peaks(self):
+        _hchk(lib().hella_clear_peaks(self._c))
+
+    def end_gulp(self):
+        _hchk(lib().hella_end_gulp(self._c))
+
+    def peaks(self):
+        n = lib().hella_get_peaks(self._c, None, 0)
+        out = np.zeros(n, CAND_DTYPE)
+        if n:
+            lib().hella_get_peaks(self._c, out.ctypes.data_as(C.POINTER(_Cand)), n)
+        return out
+
+    def clusters(self):
+        n = lib().hella_get_clusters(self._c, None, 0)
+        out = np.zeros(n, CLUSTER_DTYPE)
+        if n:
+            lib().hella_get_clusters(self._c, out.ctypes.data_as(C.POINTER(_Cluster)), n)
+        return out
+
+    def crossings(self, beam):
+        return int(lib().hella_last_crossings(self._c, beam))
+
+    def window(self, beam=0):
+        i = self.info(beam)
+        out = np.zeros((i["nchan"], i["ntime"]), np.uint8)
+        _hchk(lib().hella_copy_window(self._c, beam, out.ctypes.data, out.nbytes))
+        return out
+
+    def dedisp_plane(self, beam=0):
+        i = self.info(beam)
+        out = np.zeros((i["ndm"], i["ntime_dd"]), np.uint32)
+        _hchk(lib().hella_copy_dedisp(self._c, beam, out.ctypes.data, out.nbytes))
+        return out
+
+    def search_stats(self, beam=0):
+        nw = self.info(beam)["nwidth"]
+        mu = np.zeros(nw, np.float32)
+        sg = np.zeros(nw, np.float32)
+        s0 = _F()
+        gated = _I()
+        _hchk(lib().hella_get_search_stats(self._c, beam, mu.ctypes.data_as(C.POINTER(_F)),
+                                           sg.ctypes.data_as(C.POINTER(_F)), C.byref(s0),
+                                           C.byref(gated)))
+        return dict(mu=mu, sigma=sg, sigma0=s0.value, gated=bool(gated.value))
+
+    def stage_times(self):
+        ms = (_F * 4)()
+        _hchk(lib().hella_stage_times(self._c, ms))
+        return list(ms)
+
+    def autotune(self, path=None):
+        _hchk(lib().hella_autotune(self._c, path.encode() if path else None))
+
+    @property
+    def stream(self):
+        return lib().hella_get_stream(self._c)
+
+
+def device_count():
+    return int(lib().hella_device_count())
+
+
+def device_sync():
+    _hchk(lib().hella_device_sync())
+
+
+def synth(nchan, nsamp, fch1, foff, tsamp, seed=1, t_offset=0, dtype="u8", mean=128.0,
+          sigma=16.0, rfi_chan=(), rfi_mult=1.0, pulses=(), burst_period=0, burst_phase=0,
+          burst_len=0, burst_amp=0.0, bp_slope=0.0):
+    """Seeded synthetic filterbank [nsamp][nchan] (u8 or f32).
+    pulses: iterable of (t_seconds, dm, fwhm_ms, snr)."""
+    rc = np.ascontiguousarray(rfi_chan, np.int32)
+    pulses = list(pulses)
+    pt = np.array([p[0] for p in pulses], np.float64)
+    pdm = np.array([p[1] for p in pulses], np.float32)
+    pw = np.array([p[2] for p in pulses], np.float32)
+    ps = np.array([p[3] for p in pulses], np.float32)
+    prm = SynthParams(seed, nchan, nsamp, t_offset, fch1, foff, tsamp, 0 if dtype == "u8" else 1,
+                      mean, sigma, rc.size, rc.ctypes.data_as(C.POINTER(_I)), rfi_mult,
+                      len(pulses), pt.ctypes.data_as(C.POINTER(C.c_double)),
+                      pdm.ctypes.data_as(C.POINTER(_F)), pw.ctypes.data_as(C.POINTER(_F)),
+                      ps.ctypes.data_as(C.POINTER(_F)), burst_period, burst_phase, burst_len,
+                      burst_amp, bp_slope)
+    out = np.zeros((nsamp, nchan), np.uint8 if dtype == "u8" else np.float32)
+    _hchk(lib().hella_synth_filterbank(C.byref(prm), out.ctypes.data))
+    return out

@@ -1,0 +1,197 @@
+/*
+ * hella.h -- C ABI of the MI355X FRB-search pipeline (libhella_amd.so).
+ *
+ * Mirrors casm-hella's pipeline-stage interface (namespace hella, all taking
+ * the pipeline state hella::pinfo_t, src/hella/definitions.h:84-173):
+ *   hella_fast_flagger  <- hella::fast_flagger(p)      flagger.h:75, flagger.cu:345-411
+ *   hella_dedisperse    <- hella::dedisperse(p, beam)  dedisperse.h:17, dedisperse.cpp:17-48
+ *   hella_smooth        <- hella::smooth(p, scale)     smooth.h:16, smooth.cpp:21-127
+ *   hella_find_peaks    <- hella::find_peaks(p, bm)    peaks.h:17, peaks.cu:31-117
+ *   hella_clear_peaks   <- hella::clear_peaks(p)       peaks.h:20, peaks.cu:119-127
+ *   hella_process_gulp  <- the per-gulp loop of src/apps/casm_hella_refactored.cu:659-835
+ *                          (flag all beams, then per beam dedisperse -> smooth -> find_peaks)
+ *   hella_create        <- initialize() (casm:54-309): the same KEY VALUE configuration
+ *                          vocabulary (casm:72-169), plus the geometry keys that are
+ *                          compile-time constants in the reference (definitions.h:43-56).
+ * Opaque context instead of pinfo_t; status codes instead of exit(); all device
+ * work is stream-ordered on the context's stream.
+ */
+#ifndef HELLA_AMD_HELLA_H
+#define HELLA_AMD_HELLA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hella_ctx_s *hella_ctx;
+
+typedef enum {
+  HELLA_OK = 0,
+  HELLA_ERR_INVALID = 1,   /* bad argument */
+  HELLA_ERR_CONFIG = 2,    /* bad or inconsistent configuration */
+  HELLA_ERR_ALLOC = 3,     /* device or host allocation failed */
+  HELLA_ERR_GPU = 4,       /* HIP runtime error */
+  HELLA_ERR_STATE = 5,     /* call out of order (e.g. search before the ring is full) */
+  HELLA_ERR_CAPACITY = 6   /* a fixed-capacity buffer overflowed (data still valid up to it) */
+} hella_status;
+
+/* one threshold crossing (peaks.cu:101-116 fields plus absolute sample) */
+typedef struct {
+  float snr;
+  int beam;
+  int samp;        /* sample within the gulp's output window */
+  int width;       /* width trial index */
+  int dm_idx;      /* DM trial index into the beam's DM list */
+  float dm;        /* DM value of that trial */
+  long long abs_samp; /* absolute sample index (stream time) */
+} hella_candidate;
+
+typedef struct {
+  int rep;         /* index of the representative (max S/N) candidate in the gulp list */
+  int count;       /* crossings in the cluster */
+  int samp_min, samp_max, dm_min, dm_max;
+  int beam;
+} hella_cluster;
+
+typedef struct {
+  int nchan;
+  int ndm;
+  long long max_delay;
+  long long ntime;     /* ring length NTIME (casm:215-224) */
+  int ntime_dd;        /* dedispersed samples per gulp (gulp + maxW) */
+  int ntime_out;       /* searched samples per gulp */
+  int nwidth;
+  long long gulp;
+  int min_gulp;        /* first gulp index with a full ring (casm:632) */
+  long long gulp_index;/* gulps ingested so far */
+  int rows;            /* searched DM rows */
+  int nlive;           /* live channels */
+} hella_info;
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* config: newline-separated "KEY VALUE" lines.  Keys (defaults in brackets):
+ *   reference keys:  DM_MIN [0] DM_MAX [1000] DM_TOL [1.25] WIDTH_MIN [1]
+ *     WIDTH_MAX [32] SNR [7] GULP [32768] FLAG1 [-1] FLAG2 [-1]
+ *     SPEC_MIN [-0.05] SPEC_MAX [0.15] NBEAM [1] GPU [0]
+ *     INPUT FILTERBANK|DADA|FILE [FILTERBANK]  (FILE = one-shot window sizing)
+ *     SCRUNCH n, followed by n lines "ts fs thresh nits"
+ *   geometry:  NCHAN [256] FCH1 [500] (MHz) FOFF [-0.390625] (MHz) TSAMP [256e-6] (s)
+ *     PULSE_WIDTH [40] (us, dedisp ti)
+ *   search:    SEARCH BOXCAR|HELLA [BOXCAR]  MAX_GIANTS [10000]  CAND_CAP [1048576]
+ *     CLUSTER_DT [64] CLUSTER_DDM [16]  TIME_LIMIT [100.1] (s, casm:723)
+ *   input:     INPUT_DTYPE U8|F32|F16 (default from INPUT)
+ * Unknown keys are ignored, as in the reference. */
+hella_status hella_create(const char *config, hella_ctx *out);
+void hella_destroy(hella_ctx ctx);
+/* message of the last failing call on this thread */
+const char *hella_last_error(void);
+
+/* Per-beam geometry/DM overrides ("KEY VALUE" lines: NCHAN FCH1 FOFF TSAMP DM_MIN
+ * DM_MAX DM_TOL) for beam < NBEAM; call before the first gulp. */
+hella_status hella_set_beam(hella_ctx ctx, int beam, const char *overrides);
+/* killmask[c] != 0 keeps channel c of the beam (dedisp_set_killmask semantics) */
+hella_status hella_set_killmask(hella_ctx ctx, int beam, const int *killmask);
+
+hella_status hella_get_info(hella_ctx ctx, int beam, hella_info *out);
+/* copies up to cap DM values; returns the DM count in *n */
+hella_status hella_get_dm_list(hella_ctx ctx, int beam, float *out, int cap, int *n);
+/* the hipStream_t (as void*) all work of the context is ordered on */
+void *hella_get_stream(hella_ctx ctx);
+
+/* ---- stages (plain per-beam path) --------------------------------------- */
+/* gulp: all beams' samples, beam-major ([nbeam][gulp][nchan] for U8/F32 time-major,
+ * [nbeam][nchan][gulp] for F16 channel-major).  host_ptr != 0: host memory.
+ * Flags every beam into its ring (casm:701). */
+hella_status hella_fast_flagger(hella_ctx ctx, const void *gulp, int host_ptr);
+/* dedisperse the current window of a beam into the context's plane (casm:726) */
+hella_status hella_dedisperse(hella_ctx ctx, int beam);
+/* width trials + per-width statistics of the last dedispersed beam (casm:731);
+ * scale != 1 keeps the reference's unnormalised mode unsupported -> HELLA_ERR_INVALID */
+hella_status hella_smooth(hella_ctx ctx, int scale);
+/* sigma gate, threshold, ordered compaction; appends to the gulp's list (casm:736) */
+hella_status hella_find_peaks(hella_ctx ctx, int beam);
+hella_status hella_clear_peaks(hella_ctx ctx);
+/* advance every beam's ring by one gulp (the rewind of casm:771-797) */
+hella_status hella_end_gulp(hella_ctx ctx);
+
+/* Whole gulp on the plain path: flag all beams, then per beam
+ * dedisperse -> smooth -> find_peaks while the candidate cap and the time limit
+ * allow, then advance.  *ncand receives the gulp's candidate count. */
+hella_status hella_process_gulp(hella_ctx ctx, const void *gulp, int host_ptr, int *ncand);
+/* Same result through the batched path: every beam of the gulp in one launch
+ * chain (flag all, dedisperse all, search all), captured once as a hipGraph
+ * and replayed. */
+hella_status hella_process_gulp_batched(hella_ctx ctx, const void *gulp, int host_ptr,
+                                        int *ncand);
+
+/* the gulp's candidates (reference order: beam, width, DM row, sample) */
+int hella_get_peaks(hella_ctx ctx, hella_candidate *out, int cap);
+/* clusters of the gulp's candidates (8-connected cells of CLUSTER_DT samples x
+ * CLUSTER_DDM trials, per beam); returns the count, fills up to cap */
+int hella_get_clusters(hella_ctx ctx, hella_cluster *out, int cap);
+/* total crossings of the last searched beam before any cap */
+long long hella_last_crossings(hella_ctx ctx, int beam);
+
+/* ---- inspection ---------------------------------------------------------- */
+/* copy the beam's ring (u8, [nchan][ntime], oldest sample first) */
+hella_status hella_copy_window(hella_ctx ctx, int beam, void *host, size_t bytes);
+/* copy the last dedispersed plane of a beam ([ndm][ntime_dd] u32) */
+hella_status hella_copy_dedisp(hella_ctx ctx, int beam, void *host, size_t bytes);
+/* per-width mean/sigma (nwidth each), sigma0, gate flag of the beam's last search */
+hella_status hella_get_search_stats(hella_ctx ctx, int beam, float *mu, float *sigma,
+                                    float *sigma0, int *gated);
+/* per-stage device time of the last gulp in ms: flag, dedisperse, search, peaks (4) */
+hella_status hella_stage_times(hella_ctx ctx, float *ms);
+/* enable per-stage event timing (adds syncs between stages) */
+hella_status hella_set_timing(hella_ctx ctx, int on);
+
+/* ---- tuning -------------------------------------------------------------- */
+/* time the dedispersion tile shapes on the current rings and keep the fastest;
+ * the choice and its timings are written as JSON to path (may be NULL) */
+hella_status hella_autotune(hella_ctx ctx, const char *path);
+/* force a tile shape: lanes per DM trial (8/16/32/64), channels per LDS chunk
+ * (8/16/32), tile order (0 DM-block-major, 1 time-major) */
+hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
+
+/* ---- device helpers (test / bench plumbing) ------------------------------ */
+int hella_device_count(void);
+void *hella_dev_alloc(size_t bytes);
+void hella_dev_free(void *p);
+/* kind: 1 host->device, 2 device->host, 3 device->device */
+hella_status hella_memcpy(void *dst, const void *src, size_t bytes, int kind);
+hella_status hella_device_sync(void);
+
+/* ---- synthetic filterbank generator (seeded, host) ---------------------- */
+typedef struct {
+  unsigned long long seed;
+  int nchan;
+  long long nsamp;        /* rows to generate */
+  long long t_offset;     /* absolute index of the first row (streams stay continuous) */
+  double fch1, foff, tsamp;
+  int dtype;              /* 0: u8 (rounded, clipped to [0,255]); 1: f32 */
+  float mean, sigma;
+  int n_rfi_chan;         /* channels with noise sigma * rfi_sigma_mult */
+  const int *rfi_chan;
+  float rfi_sigma_mult;
+  int n_pulse;            /* dispersed Gaussian pulses */
+  const double *pulse_t;  /* arrival time (s) at fch1 */
+  const float *pulse_dm;
+  const float *pulse_fwhm_ms;
+  const float *pulse_snr; /* matched-filter S/N of the band-summed pulse */
+  long long burst_period; /* broadband bursts: every period samples (0: none) */
+  long long burst_phase;
+  int burst_len;
+  float burst_amp;        /* in units of sigma */
+  float bp_slope;         /* gain = 1 + bp_slope * (c/(nchan-1) - 0.5) */
+} hella_synth_params;
+
+/* out: [nsamp][nchan] of dtype.  Pulses use 4.15e3 s MHz^2 and the float delay
+ * of the dedisp plan, so injection and search agree. */
+hella_status hella_synth_filterbank(const hella_synth_params *p, void *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -112,6 +112,8 @@ void orc_dedisperse(const uint8_t *in, size_t in_tstride, size_t in_cstride,
                     const uint8_t *killmask, size_t nout,
                     uint32_t *out, size_t out_stride)
 {
+  /* DM rows are independent: threads only change which core computes a row */
+#pragma omp parallel for schedule(dynamic, 1)
   for (size_t d = 0; d < ndm; ++d) {
     uint32_t *o = out + d * out_stride;
     for (size_t t = 0; t < nout; ++t) o[t] = 0;
@@ -135,7 +137,7 @@ void orc_stats(const float *x, size_t h, size_t w, size_t stride,
 {
   size_t wp = stats_width(w);
   double *tmp = (double *)malloc((wp ? wp : 1) * sizeof(double));
-  double *rr = (double *)malloc((h ? h : 1) * sizeof(double));
+  double *rr = (double *)calloc(h ? h : 1, sizeof(double));
   for (size_t r = 0; r < h; ++r) {
     for (size_t j = 0; j < wp; ++j) tmp[j] = (double)x[r * stride + j];
     rr[r] = orc_csum(tmp, wp) / (double)wp;
