@@ -1,0 +1,239 @@
+"""FRB-search hot path benchmark on one MI355X.
+
+Workload (the production single-beam configuration): 1 beam of uint8
+filterbank data, 1024 channels over 500->400 MHz, tsamp 131.072 us, 65536-sample
+chunks plus the max-DM overlap tail, DM 0-2000 (tolerance 1.25, pulse width
+64 us), boxcar widths 1..256 in powers of two, S/N threshold 7.  The synthetic
+stream carries 20 dispersed pulses (DM U(50,1800), FWHM logU(0.2,10) ms, S/N
+U(8,50)), 5% persistent narrowband RFI channels at 10 sigma and 10-sample
+broadband bursts every 2 s at +6 sigma.
+
+One step = one chunk through flag -> dedisperse -> boxcar search -> peaks on the
+plain per-beam path, input already resident in HBM.  value = DM-trial samples
+per second of that step.  Also reported: the batched graph path on several
+beams, per-stage device times, the dedispersion kernel's integer-add roofline,
+the flagger's HBM roofline and the oracle (CPU) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+CHUNK = 65536
+NCH = 1024
+TSAMP = 131.072e-6
+FOFF = -100.0 / NCH
+MAXW = 256
+SNR = 7.0
+PEAK_HBM_GBS = 8000.0
+PEAK_ADD_T = 78.6  # packed u16 integer adds/s (256 CU x 64 lanes x 2 x 2.4 GHz), 1e12
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "hella_amd", os.path.join(ROOT, "casm-hella_amd", "hella_amd.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["hella_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def config(nbeam=1):
+    return dict(INPUT="FILTERBANK", NCHAN=NCH, FCH1=500.0, FOFF=FOFF, TSAMP=TSAMP, GULP=CHUNK,
+                DM_MIN=0, DM_MAX=2000, DM_TOL=1.25, PULSE_WIDTH=64, WIDTH_MIN=1, WIDTH_MAX=MAXW,
+                SNR=SNR, NBEAM=nbeam, SCRUNCH=[(1, 1, 6.0)])
+
+
+def stream_params(seed=20261205, duration_s=60.0):
+    rng = np.random.default_rng(seed)
+    pulses = []
+    for _ in range(20):
+        pulses.append((float(rng.uniform(0.5, duration_s - 0.5)), float(rng.uniform(50, 1800)),
+                       float(np.exp(rng.uniform(np.log(0.2), np.log(10.0)))),
+                       float(rng.uniform(8, 50))))
+    rfi = rng.choice(NCH, int(0.05 * NCH), replace=False)
+    return pulses, rfi
+
+
+def make_chunk(hamd, g, pulses, rfi, seed=20261205):
+    return hamd.synth(NCH, CHUNK, 500.0, FOFF, TSAMP, seed=seed, t_offset=g * CHUNK,
+                      rfi_chan=rfi, rfi_mult=10.0, pulses=pulses,
+                      burst_period=int(round(2.0 / TSAMP)), burst_phase=1000, burst_len=10,
+                      burst_amp=6.0)
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def cpu_baseline(hamd, chunk_tf, sample_rows):
+    """Oracle flag on the whole chunk plus dedisperse+search on a sample of DM
+    rows, scaled to the whole chunk (DM rows cost the same)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as orc
+
+    plan = orc.Plan(NCH, TSAMP, 500.0, FOFF, 0.0, 2000.0, 64.0, 1.25)
+    ndm = plan.dms.size
+    t0 = time.perf_counter()
+    x = orc.ingest_u8_tf(chunk_tf)
+    flagged, _ = orc.flag(x, -0.05, 0.15, -1, -1, ((1, 1, 6.0),))
+    t_flag = time.perf_counter() - t0
+    # window: the flagged chunk followed by itself stands in for chunk + tail
+    ntime_dd = CHUNK + MAXW
+    reps = (ntime_dd + plan.max_delay + CHUNK - 1) // CHUNK
+    win = np.tile(flagged, (1, reps))
+    rows = np.linspace(0, ndm - 1, sample_rows).astype(int)
+    t1 = time.perf_counter()
+    D = orc.dedisperse_ft(win, plan.delta[rows], ntime_dd)
+    t_dd = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    orc.search(D, CHUNK, MAXW, 1, 0, SNR)
+    t_s = time.perf_counter() - t2
+    t_chunk = t_flag + (t_dd + t_s) * ndm / sample_rows
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": ndm * CHUNK / t_chunk, "unit": "DM-trial·samples/s", "cores": cores,
+            "kind": "port",
+            "sample": (f"oracle (C, -O3, OpenMP over DM rows; flagger single-threaded): flagger on "
+                       f"one full chunk ({t_flag:.2f} s), dedispersion + boxcar search on "
+                       f"{sample_rows} of {ndm} DM trials ({t_dd + t_s:.2f} s), scaled to all "
+                       f"trials: {t_chunk:.1f} s per chunk")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--batch-beams", type=int, default=4)
+    ap.add_argument("--cpu-rows", type=int, default=96)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this build has a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    hamd = load_pkg()
+    if hamd.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    pulses, rfi = stream_params()
+    h = hamd.Hella(config(1))
+    info = h.info(0)
+    need = max(args.warmup, info["min_gulp"] + 1)
+    nchunks = need + args.steps
+    ndistinct = min(nchunks, 8)
+    t_gen = time.perf_counter()
+    host_chunks = [make_chunk(hamd, g, pulses, rfi) for g in range(ndistinct)]
+    dev = []
+    for c in host_chunks:
+        b = hamd.DeviceBuffer(c.nbytes)
+        b.upload(c)
+        dev.append(b)
+    print(f"# generated {ndistinct} chunks in {time.perf_counter() - t_gen:.1f} s; "
+          f"ndm={info['ndm']} max_delay={info['max_delay']} ntime={info['ntime']}",
+          file=sys.stderr, flush=True)
+
+    # ---------------- plain path
+    for g in range(need):
+        h.process_gulp(dev[g % ndistinct])
+    lat = []
+    hamd.device_sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        h.process_gulp(dev[(need + k) % ndistinct])
+        lat.append((time.perf_counter() - ts) * 1e3)
+    hamd.device_sync()
+    t_plain = (time.perf_counter() - t0) / args.steps
+    ncand = len(h.peaks())
+    print(f"# plain: {t_plain * 1e3:.2f} ms/chunk, {ncand} candidates", file=sys.stderr,
+          flush=True)
+    # per-stage device times (one extra step with events between stages)
+    h.set_timing(True)
+    h.process_gulp(dev[0])
+    stages = dict(zip(["flag", "dedisperse", "search", "peaks"], h.stage_times()))
+    h.set_timing(False)
+    # live kernel timing with HIP events on the context stream
+    ms_dd = h.bench_stage(0, 1, reps=5)
+    ms_flag = h.bench_stage(0, 0, reps=3)
+    ms_search = h.bench_stage(0, 2, reps=3)
+    adds = info["ndm"] * info["ntime_dd"] * info["nlive"]
+    achieved_add = adds / (ms_dd * 1e-3) / 1e12
+    flag_bytes = 3 * NCH * CHUNK * 1  # one stats read, one read+write apply (u8 equivalents)
+    h.close()
+
+    # ---------------- batched path (graph-captured), several beams
+    nb = args.batch_beams
+    hb = hamd.Hella(config(nb))
+    gulp_dev = []
+    for i in range(ndistinct):
+        buf = hamd.DeviceBuffer(host_chunks[i].nbytes * nb)
+        buf.upload(np.concatenate([host_chunks[(i + j) % ndistinct].ravel() for j in range(nb)]))
+        gulp_dev.append(buf)
+    for g in range(need):
+        hb.process_gulp(gulp_dev[g % ndistinct], batched=True)
+    if args.tune:
+        hb.autotune(os.path.join(ROOT, "gpurun_out", "autotune_c2.json"))
+        hb.process_gulp(gulp_dev[0], batched=True)
+    blat = []
+    hamd.device_sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        hb.process_gulp(gulp_dev[(need + k) % ndistinct], batched=True)
+        blat.append((time.perf_counter() - ts) * 1e3)
+    hamd.device_sync()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    hb.close()
+
+    chunk_s = CHUNK * TSAMP
+    value = info["ndm"] * CHUNK / t_plain
+    out = {
+        "metric": "real-time beams/GPU; DM-trial·samples/s; p99 chunk latency, batched vs plain",
+        "value": value,
+        "unit": "DM-trial·samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": need,
+        "ms_per_step": t_plain * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generator: Gaussian noise, dispersed pulses, RFI)",
+        "config": {"workload": "plain single beam: 1024 ch 500->400 MHz, 131.072 us, "
+                               "65536-sample chunks + max-DM tail, DM 0-2000 "
+                               f"({info['ndm']} trials), boxcar widths 1..256",
+                   "ndm": info["ndm"], "max_delay": info["max_delay"], "chunk": CHUNK,
+                   "nchan": NCH, "scrunch": "1 1 6"},
+        "realtime_factor": chunk_s / t_plain,
+        "realtime_beams": {"plain": int(chunk_s / t_plain),
+                           "batched": int(nb * chunk_s / t_batch), "batched_beams_run": nb},
+        "latency_ms": {"plain": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                       "batched": {"p50": pct(blat, 50), "p99": pct(blat, 99), "beams": nb}},
+        "stages_ms": stages,
+        "kernels_ms": {"dedisperse": ms_dd, "flagger": ms_flag, "search": ms_search},
+        "roofline": {"bound": "valu", "kernel": "k_dedisp", "achieved": achieved_add,
+                     "peak": PEAK_ADD_T, "unit": "Tadd/s", "frac": achieved_add / PEAK_ADD_T,
+                     "traffic": None},
+        "roofline_flagger": {"bound": "hbm", "achieved": flag_bytes / (ms_flag * 1e-3) / 1e9,
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": flag_bytes / (ms_flag * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                             "traffic": None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(hamd, host_chunks[0], args.cpu_rows)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -336,6 +336,14 @@ hella_status hella_set_timing(hella_ctx c, int on)
   return HELLA_OK;
 }
 
+hella_status hella_bench_stage(hella_ctx c, int beam, int which, int reps, float *ms)
+{
+  if (!valid_beam(c, beam) || !ms || reps <= 0 || which < 0 || which > 2)
+    return fail(HELLA_ERR_INVALID, "bad argument");
+  if (!c->built || c->batch_mode) return fail(HELLA_ERR_STATE, "run a plain gulp first");
+  return guard([&] { *ms = ctx_bench_stage(c, beam, which, reps); });
+}
+
 hella_status hella_autotune(hella_ctx c, const char *path)
 {
   if (!c) return fail(HELLA_ERR_INVALID, "ctx is NULL");

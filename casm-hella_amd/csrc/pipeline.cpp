@@ -744,6 +744,38 @@ void ctx_cluster(hella_ctx c)
   }
 }
 
+// ------------------------------------------------------------------ stage timing
+float ctx_bench_stage(hella_ctx c, int b, int which, int reps)
+{
+  BeamState &B = c->beams[(size_t)b];
+  hipStream_t s = c->stream;
+  auto launch = [&]() {
+    if (which == 0) {
+      FlagRunner::run(c->d_fdesc.as<FlagBeamDev>() + b, 1, B.nch, c->W, c->cfg.flag, s);
+    } else if (which == 1) {
+      if (B.nlive)
+        dd_launch(c->d_ddesc.as<DDBeamDev>() + b, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile,
+                  B.spread, s);
+    } else {
+      SearchRunner::run(c->d_sdesc.as<SearchBeamDev>() + b, 1, B.rows, c->common,
+                        c->d_kern.as<float>(), c->kofs, s);
+    }
+  };
+  launch();  // warm
+  hipEvent_t e0, e1;
+  HAMD_CHECK(hipEventCreate(&e0));
+  HAMD_CHECK(hipEventCreate(&e1));
+  HAMD_CHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  HAMD_CHECK(hipEventRecord(e1, s));
+  HAMD_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HAMD_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / (float)reps;
+}
+
 // ------------------------------------------------------------------ autotune
 void ctx_autotune(hella_ctx c, const char *path)
 {

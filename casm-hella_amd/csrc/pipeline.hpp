@@ -124,5 +124,6 @@ void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr);
 void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr);
 void ctx_cluster(hella_ctx c);
 void ctx_autotune(hella_ctx c, const char *path);
+float ctx_bench_stage(hella_ctx c, int b, int which, int reps);
 const void *ctx_stage_input(hella_ctx c, const void *gulp, bool host_ptr);
 }  // namespace hamd

@@ -225,6 +225,7 @@ def _declare_hella(L):
     d("hella_stage_times", _I, P, C.POINTER(_F))
     d("hella_set_timing", _I, P, _I)
     d("hella_autotune", _I, P, C.c_char_p)
+    d("hella_bench_stage", _I, P, _I, _I, _I, C.POINTER(_F))
     d("hella_set_tile", _I, P, _I, _I, _I)
     d("hella_device_count", _I)
     d("hella_dev_alloc", P, C.c_size_t)
@@ -412,6 +413,12 @@ class Hella:
         ms = (_F * 4)()
         _hchk(lib().hella_stage_times(self._c, ms))
         return list(ms)
+
+    def bench_stage(self, beam, which, reps=5):
+        """mean device ms of one stage re-run (0 flag, 1 dedisperse, 2 search)"""
+        ms = _F()
+        _hchk(lib().hella_bench_stage(self._c, beam, which, reps, C.byref(ms)))
+        return ms.value
 
     def autotune(self, path=None):
         _hchk(lib().hella_autotune(self._c, path.encode() if path else None))

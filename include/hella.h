@@ -147,6 +147,13 @@ hella_status hella_stage_times(hella_ctx ctx, float *ms);
 /* enable per-stage event timing (adds syncs between stages) */
 hella_status hella_set_timing(hella_ctx ctx, int on);
 
+/* Re-run one stage of the plain path for a beam on the data of its last run
+ * (same descriptors, so the call is idempotent) reps times, timed with HIP
+ * events on the context stream.  which: 0 flagger, 1 dedispersion kernel,
+ * 2 search (width trials, statistics, threshold, compaction).  *ms receives
+ * the mean duration of one repetition. */
+hella_status hella_bench_stage(hella_ctx ctx, int beam, int which, int reps, float *ms);
+
 /* ---- tuning -------------------------------------------------------------- */
 /* time the dedispersion tile shapes on the current rings and keep the fastest;
  * the choice and its timings are written as JSON to path (may be NULL) */

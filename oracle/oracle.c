@@ -445,8 +445,26 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
     }
     /* per-width mean / variance (smooth.cpp:186-194 -> calculate_stddev) */
     double *rawplane = (double *)malloc(sizeof(double) * plane);
-    for (int r = 0; r < rows; ++r)
-      for (int s = 0; s < nout; ++s) rawplane[(size_t)r * nout + s] = plane_raw(&p, r, s);
+    if (cfg->mode == ORC_MODE_BOXCAR) {
+      /* exact integer box sums from a u64 prefix sum of each row */
+      const int c0 = cfg->maxW / 2, w = p.w, lo = w / 2;
+#pragma omp parallel for schedule(static)
+      for (int r = 0; r < rows; ++r) {
+        const uint32_t *row = D + (size_t)(r + row0) * d_stride;
+        uint64_t *P = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(cfg->ntime_dd + 1));
+        P[0] = 0;
+        for (int t = 0; t < cfg->ntime_dd; ++t) P[t + 1] = P[t] + row[t];
+        for (int s = 0; s < nout; ++s) {
+          int a = s + c0 - lo;
+          rawplane[(size_t)r * nout + s] = (double)(P[a + w] - P[a]);
+        }
+        free(P);
+      }
+    } else {
+#pragma omp parallel for schedule(static)
+      for (int r = 0; r < rows; ++r)
+        for (int s = 0; s < nout; ++s) rawplane[(size_t)r * nout + s] = plane_raw(&p, r, s);
+    }
     for (int r = 0; r < rows; ++r) rr[r] = orc_csum(rawplane + (size_t)r * nout, wp) / (double)wp;
     double m = orc_csum(rr, (size_t)rows) / (double)rows;
     for (int r = 0; r < rows; ++r) {
