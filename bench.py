@@ -116,6 +116,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="skip the batched path and the CPU baseline (profiling runs)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has a single MI355X "
@@ -145,6 +149,9 @@ def main():
     # ---------------- plain path
     for g in range(need):
         h.process_gulp(dev[g % ndistinct])
+    if args.tune:
+        h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_plain.json"))
+        h.process_gulp(dev[0])
     lat = []
     hamd.device_sync()
     t0 = time.perf_counter()
@@ -170,6 +177,19 @@ def main():
     achieved_add = adds / (ms_dd * 1e-3) / 1e12
     flag_bytes = 3 * NCH * CHUNK * 1  # one stats read, one read+write apply (u8 equivalents)
     h.close()
+    if args.calib_copy_mb:
+        nbytes = args.calib_copy_mb << 20
+        a, b = hamd.DeviceBuffer(nbytes), hamd.DeviceBuffer(nbytes)
+        hamd.lib().hella_memcpy(b.ptr, a.ptr, nbytes, 3)
+        hamd.device_sync()
+        a.free()
+        b.free()
+    if args.plain_only:
+        print(json.dumps({"ms_per_step": t_plain * 1e3, "stages_ms": stages,
+                          "kernels_ms": {"dedisperse": ms_dd, "flagger": ms_flag,
+                                         "search": ms_search},
+                          "dedisp_Tadd_s": achieved_add}), flush=True)
+        return
 
     # ---------------- batched path (graph-captured), several beams
     nb = args.batch_beams

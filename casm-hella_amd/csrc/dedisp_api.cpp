@@ -22,7 +22,7 @@ struct dedisp_plan_struct {
   DevBuf d_delta, d_live;
   int nlive = 0;
   std::vector<int32_t> delta_host, live_host;
-  DDTile tile{16, 32};
+  DDTile tile{16, 32, 1};
   int max_spread = 0;
   DevBuf scratch_in, scratch_ft, scratch_out;
   DevBuf d_beam, d_tiles;
@@ -44,14 +44,14 @@ DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
     int nd = 256 / G;
     int spread = dd_max_spread(delta, ndm, nlive, nd);
     for (int cc : {32, 16, 8}) {
-      DDTile t{G, cc};
+      DDTile t{G, cc, 1};
       if (dd_lds_bytes(t, spread) <= lds_budget) {
         *spread_out = spread;
         return t;
       }
     }
   }
-  DDTile t{64, 8};
+  DDTile t{64, 8, 1};
   *spread_out = dd_max_spread(delta, ndm, nlive, 4);
   return t;
 }
@@ -67,7 +67,7 @@ dedisp_error refresh_device(dedisp_plan plan)
   const int ndm = (int)plan->h.dm_list.size();
   if (plan->nlive > 0) {
     plan->tile = choose_dd_tile(plan->delta_host.data(), ndm, plan->nlive, &plan->max_spread,
-                                16, 64 * 1024);
+                                16, DD_LDS_BUDGET);
     plan->d_delta.ensure(plan->delta_host.size() * 4);
     plan->d_live.ensure(plan->live_host.size() * 4);
     HAMD_CHECK(hipMemcpy(plan->d_delta.p, plan->delta_host.data(), plan->delta_host.size() * 4,

@@ -31,9 +31,43 @@ constexpr int NT = 256;
 constexpr int V = 36;          // samples per thread
 constexpr int NW = V / 4 + 1;  // LDS dwords read per channel per thread
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
   return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// write one thread's V sums of DM row d starting at output sample tb
+__device__ __forceinline__ void store_out(const DDBeamDev &B, int d, int tb, const uint32_t *tot)
+{
+  if (d >= B.ndm) return;
+  const int nout = B.nout;
+  if (B.out_f32) {
+    float *o = static_cast<float *>(B.out) + (int64_t)d * B.out_stride + tb;
+    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k)
+        reinterpret_cast<float4 *>(o)[k] = make_float4((float)tot[4 * k], (float)tot[4 * k + 1],
+                                                       (float)tot[4 * k + 2], (float)tot[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; ++j)
+        if (tb + j < nout) o[j] = (float)tot[j];
+    }
+  } else {
+    uint32_t *o = static_cast<uint32_t *>(B.out) + (int64_t)d * B.out_stride + tb;
+    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k)
+        reinterpret_cast<uint4 *>(o)[k] =
+            make_uint4(tot[4 * k], tot[4 * k + 1], tot[4 * k + 2], tot[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < V; ++j)
+        if (tb + j < nout) o[j] = tot[j];
+    }
+  }
 }
 
 template <int G>
@@ -66,29 +100,39 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
 #pragma unroll
   for (int i = 0; i < V; ++i) tot[i] = 0;
 
+  // delays of the first chunk; later chunks are prefetched into registers
+  // during the previous chunk's arithmetic (one (channel, DM) pair per thread)
+  auto dly_load = [&](int k0, int i) -> int32_t {
+    const int nc = min(chunk, nlive - k0);
+    if (i >= nc * ND) return 0;
+    const int cc = i / ND, dd = i - (i / ND) * ND;
+    const int dm = min(d0 + dd, ndm - 1);
+    return B.delta[(int64_t)dm * nlive + k0 + cc];
+  };
+  int32_t pre = dly_load(0, tid);
+  const int ring = (int)B.ring;
   int since_flush = 0;
   for (int k0 = 0; k0 < nlive; k0 += chunk) {
     const int nc = min(chunk, nlive - k0);
     __syncthreads();
-    for (int i = tid; i < nc * ND; i += NT) {
-      const int cc = i / ND, dd = i - (i / ND) * ND;
-      const int dm = min(d0 + dd, ndm - 1);
-      sdly[cc * ND + dd] = B.delta[(int64_t)dm * nlive + k0 + cc];
-    }
+    if (tid < nc * ND) sdly[tid] = pre;
+    for (int i = tid + NT; i < nc * ND; i += NT) sdly[i] = dly_load(k0, i);
     __syncthreads();
+    if (k0 + chunk < nlive) pre = dly_load(k0 + chunk, tid);
     // stage windows: one wave per channel, 16-byte units
     for (int cc = wave; cc < nc; cc += NT / 64) {
       const int base_d = sdly[cc * ND];
       const int spread = sdly[cc * ND + ND - 1] - base_d;
-      const int64_t first = (B.start + t0 + base_d) % B.ring;
-      const int64_t al = first & ~(int64_t)15;
-      const int off = (int)(first - al);
+      int first = (int)B.start + t0 + base_d;
+      while (first >= ring) first -= ring;
+      const int al = first & ~15;
+      const int off = first - al;
       const int n16 = (off + T + spread + 4 + 15) >> 4;
       const uint8_t *row = B.base + (int64_t)B.live[k0 + cc] * B.row_stride;
       uint8_t *dst = sdat + (size_t)cc * lrow;
       for (int u = lane; u < n16; u += 64) {
-        int64_t col = al + 16 * (int64_t)u;
-        if (col >= B.ring) col %= B.ring;
+        int col = al + 16 * u;
+        while (col >= ring) col -= ring;
         *reinterpret_cast<uint4 *>(dst + 16 * u) = *reinterpret_cast<const uint4 *>(row + col);
       }
       if (lane == 0) soff[cc] = off;
@@ -124,45 +168,324 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
     }
   }
 
-  const int d = d0 + dl;
-  if (d >= ndm) return;
-  const int tb = t0 + g * V;
-  const int nout = B.nout;
-  if (B.out_f32) {
-    float *o = static_cast<float *>(B.out) + (int64_t)d * B.out_stride + tb;
-    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+  store_out(B, d0 + dl, t0 + g * V, tot);
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void glb_void_t;
+
+// Same arithmetic as k_dedisp; the channel windows of the next chunk are
+// copied global -> LDS by the DMA path of global_load_lds_dwordx4 (each lane
+// moves 16 bytes to LDS base + 16 * lane) while the current chunk is summed,
+// so the loads never wait in registers.  Two LDS buffers alternate.
+template <int G, int CC>
+__global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__ beams,
+                                                   const DDTileDesc *__restrict__ tiles,
+                                                   int lrow)
+{
+  constexpr int chunk = CC;
+  constexpr int ND = NT / G;
+  constexpr int T = G * V;
+  extern __shared__ __align__(16) uint8_t lds[];
+  const DDTileDesc td = tiles[blockIdx.x];
+  if (td.beam < 0) return;
+  const DDBeamDev B = beams[td.beam];
+  // layout: data[2][chunk][lrow] | rel[2][chunk][ND] (window offset of each DM)
+  uint8_t *sdat = lds;
+  int32_t *srel = reinterpret_cast<int32_t *>(lds + (size_t)2 * chunk * lrow);
+
+  const int tid = threadIdx.x;
+  const int dl = tid / G;
+  const int g = tid % G;
+  const int d0 = td.d0;
+  const int t0 = td.t0;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int nlive = B.nlive, ndm = B.ndm;
+  const int dlast = min(d0 + ND - 1, ndm - 1);
+
+  // Each wave stages channels wave, wave+4, ... of a chunk.  Their delays
+  // (first DM, last DM, this lane's DM) are loaded one chunk ahead into
+  // registers so issuing the DMA copies never waits on memory.
+  constexpr int MAXC = CC / (NT / 64);
+  const int ring = (int)B.ring;
+  int32_t pb[MAXC], pl[MAXC], pr[MAXC];
+  auto prefetch = [&](int k0) {
+    const int nc = min(chunk, nlive - k0);
 #pragma unroll
-      for (int k = 0; k < V / 4; ++k)
-        reinterpret_cast<float4 *>(o)[k] = make_float4((float)tot[4 * k], (float)tot[4 * k + 1],
-                                                       (float)tot[4 * k + 2], (float)tot[4 * k + 3]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < V; ++j)
-        if (tb + j < nout) o[j] = (float)tot[j];
+    for (int i = 0; i < MAXC; ++i) {
+      const int cc = wave + 4 * i;
+      if (cc < nc) {
+        pb[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)d0 * nlive + k0 + cc]);
+        pl[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)dlast * nlive + k0 + cc]);
+        pr[i] = lane < ND ? B.delta[(int64_t)min(d0 + lane, ndm - 1) * nlive + k0 + cc] : 0;
+      }
     }
-  } else {
-    uint32_t *o = static_cast<uint32_t *>(B.out) + (int64_t)d * B.out_stride + tb;
-    if (tb + V <= nout && ((reinterpret_cast<uintptr_t>(o) & 15) == 0)) {
+  };
+  auto stage = [&](int k0, int buf) {
+    const int nc = min(chunk, nlive - k0);
 #pragma unroll
-      for (int k = 0; k < V / 4; ++k)
-        reinterpret_cast<uint4 *>(o)[k] =
-            make_uint4(tot[4 * k], tot[4 * k + 1], tot[4 * k + 2], tot[4 * k + 3]);
-    } else {
+    for (int i = 0; i < MAXC; ++i) {
+      const int cc = wave + 4 * i;
+      if (cc >= nc) break;
+      const int base_d = pb[i];
+      const int spread = pl[i] - base_d;
+      int first = (int)B.start + t0 + base_d;
+      while (first >= ring) first -= ring;
+      const int al = first & ~15;
+      const int off = first - al;
+      const int n16 = (off + T + spread + 4 + 15) >> 4;
+      const uint8_t *row = B.base + (int64_t)B.live[k0 + cc] * B.row_stride;
+      uint8_t *dst = sdat + ((size_t)buf * chunk + cc) * lrow;
+      for (int j = 0; j * 64 < n16; ++j) {
+        const int u = j * 64 + lane;
+        if (u < n16) {
+          int col = al + 16 * u;
+          while (col >= ring) col -= ring;
+          __builtin_amdgcn_global_load_lds((glb_void_t *)(row + col), (lds_void_t *)(dst + 1024 * j),
+                                           16, 0, 0);
+        }
+      }
+      if (lane < ND) srel[((size_t)buf * chunk + cc) * ND + lane] = off + pr[i] - base_d;
+    }
+  };
+
+  uint32_t acc[V / 2];
+  uint32_t tot[V];
 #pragma unroll
-      for (int j = 0; j < V; ++j)
-        if (tb + j < nout) o[j] = tot[j];
+  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i) tot[i] = 0;
+
+  prefetch(0);
+  stage(0, 0);
+  if (chunk < nlive) prefetch(chunk);
+  int since_flush = 0, buf = 0;
+  for (int k0 = 0; k0 < nlive; k0 += chunk, buf ^= 1) {
+    const int nc = min(chunk, nlive - k0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (k0 + chunk < nlive) {
+      stage(k0 + chunk, buf ^ 1);
+      if (k0 + 2 * chunk < nlive) prefetch(k0 + 2 * chunk);
+    }
+    const uint8_t *dat = sdat + (size_t)buf * chunk * lrow;
+    const int32_t *rel = srel + (size_t)buf * chunk * ND;
+    for (int cc = 0; cc < nc; ++cc) {
+      const int p = g * V + rel[cc * ND + dl];
+      const uint32_t *wrow = reinterpret_cast<const uint32_t *>(dat + (size_t)cc * lrow) + (p >> 2);
+      const uint32_t s = (uint32_t)(p & 3);
+      const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
+      const uint32_t sel_o = (s + 1u) | 0x0c00u | ((s + 3u) << 16) | 0x0c000000u;
+      uint32_t w[NW];
+#pragma unroll
+      for (int i = 0; i < NW; ++i) w[i] = wrow[i];
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        acc[2 * k] += perm(w[k + 1], w[k], sel_e);
+        acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
+      }
+    }
+    since_flush += nc;
+    if (since_flush >= 256 || k0 + chunk >= nlive) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        tot[4 * k + 0] += acc[2 * k] & 0xffffu;
+        tot[4 * k + 2] += acc[2 * k] >> 16;
+        tot[4 * k + 1] += acc[2 * k + 1] & 0xffffu;
+        tot[4 * k + 3] += acc[2 * k + 1] >> 16;
+        acc[2 * k] = 0;
+        acc[2 * k + 1] = 0;
+      }
+      since_flush = 0;
     }
   }
+  store_out(B, d0 + dl, t0 + g * V, tot);
+}
+
+// Register-pipelined variant: while chunk k is summed from LDS, the 16-byte
+// units of chunk k+1's windows are already loading into registers (UPL units
+// per lane per staged channel); after the sums they are written to the single
+// LDS buffer.  Delays for the staging are loaded two chunks ahead.  Units
+// beyond UPL*64 of a window (very wide delay spreads) are copied directly.
+template <int G, int CC, int UPL>
+__global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ beams,
+                                                  const DDTileDesc *__restrict__ tiles, int lrow)
+{
+  constexpr int ND = NT / G;
+  constexpr int T = G * V;
+  constexpr int MAXC = CC / (NT / 64);
+  extern __shared__ __align__(16) uint8_t lds[];
+  const DDTileDesc td = tiles[blockIdx.x];
+  if (td.beam < 0) return;
+  const DDBeamDev B = beams[td.beam];
+  uint8_t *sdat = lds;                                                   // [CC][lrow]
+  int32_t *srel = reinterpret_cast<int32_t *>(lds + (size_t)CC * lrow);  // [CC][ND]
+
+  const int tid = threadIdx.x;
+  const int dl = tid / G;
+  const int g = tid % G;
+  const int d0 = td.d0;
+  const int t0 = td.t0;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int nlive = B.nlive, ndm = B.ndm;
+  const int dlast = min(d0 + ND - 1, ndm - 1);
+  const int ring = (int)B.ring;
+
+  // per staged channel slot: delays (first DM, last DM, this lane's DM) and
+  // the registers holding its window units
+  int32_t pb[MAXC], pl[MAXC], pr[MAXC];
+  u32x4 ureg[MAXC * UPL];
+  int uoff[MAXC], un16[MAXC], ualgn[MAXC], urel[MAXC];
+  int64_t urow[MAXC];
+
+#define HAMD_PF_PREFETCH_DLY(K0)                                                           \
+  {                                                                                        \
+    const int nc_ = min(CC, nlive - (K0));                                                 \
+    _Pragma("unroll") for (int i = 0; i < MAXC; ++i) {                                     \
+      const int cc = wave + 4 * i;                                                         \
+      if (cc < nc_) {                                                                      \
+        pb[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)d0 * nlive + (K0) + cc]);   \
+        pl[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)dlast * nlive + (K0) + cc]);\
+        pr[i] = lane < ND ? B.delta[(int64_t)min(d0 + lane, ndm - 1) * nlive + (K0) + cc] : 0; \
+      }                                                                                    \
+    }                                                                                      \
+  }
+#define HAMD_PF_LOAD(K0)                                                                   \
+  {                                                                                        \
+    const int nc_ = min(CC, nlive - (K0));                                                 \
+    _Pragma("unroll") for (int i = 0; i < MAXC; ++i) {                                     \
+      const int cc = wave + 4 * i;                                                         \
+      if (cc < nc_) {                                                                      \
+        int first = (int)B.start + t0 + pb[i];                                             \
+        first -= first >= ring ? ring : 0;                                                 \
+        const int al = first & ~15;                                                        \
+        uoff[i] = first - al;                                                              \
+        urel[i] = uoff[i] + pr[i] - pb[i];                                                 \
+        ualgn[i] = al;                                                                     \
+        un16[i] = (uoff[i] + T + (pl[i] - pb[i]) + 4 + 15) >> 4;                           \
+        urow[i] = (int64_t)B.live[(K0) + cc] * B.row_stride;                               \
+        _Pragma("unroll") for (int j = 0; j < UPL; ++j) {                                  \
+          const int u = j * 64 + lane;                                                     \
+          int col = al + 16 * u;                                                           \
+          col -= col >= ring ? ring : 0;                                                   \
+          if (u < un16[i])                                                                 \
+            ureg[i * UPL + j] = *reinterpret_cast<const u32x4 *>(B.base + urow[i] + col);  \
+        }                                                                                  \
+      }                                                                                    \
+    }                                                                                      \
+  }
+
+  uint32_t acc[V / 2];
+  uint32_t tot[V];
+#pragma unroll
+  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i) tot[i] = 0;
+
+  HAMD_PF_PREFETCH_DLY(0)
+  HAMD_PF_LOAD(0)
+  if (CC < nlive) HAMD_PF_PREFETCH_DLY(CC)
+  int since_flush = 0;
+  for (int k0 = 0; k0 < nlive; k0 += CC) {
+    const int nc = min(CC, nlive - k0);
+    __syncthreads();  // everyone is done with the previous chunk
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int cc = wave + 4 * i;
+      if (cc < nc) {
+        uint8_t *dst = sdat + (size_t)cc * lrow;
+#pragma unroll
+        for (int j = 0; j < UPL; ++j) {
+          const int u = j * 64 + lane;
+          if (u < un16[i]) *reinterpret_cast<u32x4 *>(dst + 16 * u) = ureg[i * UPL + j];
+        }
+        for (int u = UPL * 64 + lane; u < un16[i]; u += 64) {
+          int col = ualgn[i] + 16 * u;
+          col -= col >= ring ? ring : 0;
+          *reinterpret_cast<uint4 *>(dst + 16 * u) =
+              *reinterpret_cast<const uint4 *>(B.base + urow[i] + col);
+        }
+        if (lane < ND) srel[cc * ND + lane] = urel[i];
+      }
+    }
+    __syncthreads();
+    if (k0 + CC < nlive) {
+      HAMD_PF_LOAD(k0 + CC)  // in flight during the sums below
+      if (k0 + 2 * CC < nlive) HAMD_PF_PREFETCH_DLY(k0 + 2 * CC)
+    }
+    for (int cc = 0; cc < nc; ++cc) {
+      const int p = g * V + srel[cc * ND + dl];
+      const uint32_t *wrow = reinterpret_cast<const uint32_t *>(sdat + (size_t)cc * lrow) + (p >> 2);
+      const uint32_t s = (uint32_t)(p & 3);
+      const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
+      const uint32_t sel_o = sel_e + 0x00010001u;
+      uint32_t w[NW];
+#pragma unroll
+      for (int i = 0; i < NW; ++i) w[i] = wrow[i];
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        acc[2 * k] += perm(w[k + 1], w[k], sel_e);
+        acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
+      }
+    }
+    since_flush += nc;
+    if (since_flush >= 256 || k0 + CC >= nlive) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        tot[4 * k + 0] += acc[2 * k] & 0xffffu;
+        tot[4 * k + 2] += acc[2 * k] >> 16;
+        tot[4 * k + 1] += acc[2 * k + 1] & 0xffffu;
+        tot[4 * k + 3] += acc[2 * k + 1] >> 16;
+        acc[2 * k] = 0;
+        acc[2 * k + 1] = 0;
+      }
+      since_flush = 0;
+    }
+  }
+#undef HAMD_PF_PREFETCH_DLY
+#undef HAMD_PF_LOAD
+  store_out(B, d0 + dl, t0 + g * V, tot);
+}
+
+template <int G, int CC>
+void launch_pf(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int lrow,
+               hipStream_t s)
+{
+  constexpr int ND = NT / G;
+  const size_t lds = (size_t)CC * lrow + (size_t)CC * ND * 4;
+  const int upl = (lrow + 1023) / 1024;
+  if (upl <= 1)
+    hipLaunchKernelGGL((k_dedisp_pf<G, CC, 1>), dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams,
+                       d_tiles, lrow);
+  else if (upl == 2)
+    hipLaunchKernelGGL((k_dedisp_pf<G, CC, 2>), dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams,
+                       d_tiles, lrow);
+  else
+    hipLaunchKernelGGL((k_dedisp_pf<G, CC, 3>), dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams,
+                       d_tiles, lrow);
 }
 
 template <int G>
 void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int chunk,
-              int lrow, hipStream_t s)
+              int lrow, int dma, hipStream_t s)
 {
   constexpr int ND = NT / G;
-  const size_t lds = (size_t)chunk * lrow + (size_t)chunk * ND * 4 + (size_t)chunk * 4;
-  hipLaunchKernelGGL(k_dedisp<G>, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles,
-                     chunk, lrow);
+  if (dma == 2) {
+    if (chunk == 8) launch_pf<G, 8>(d_beams, d_tiles, ntiles, lrow, s);
+    else launch_pf<G, 16>(d_beams, d_tiles, ntiles, lrow, s);
+  } else if (dma == 1) {
+    const size_t lds = (size_t)2 * chunk * lrow + (size_t)2 * chunk * ND * 4;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles, lrow);
+    };
+    if (chunk == 8) go(k_dedisp_dma<G, 8>);
+    else if (chunk == 16) go(k_dedisp_dma<G, 16>);
+    else go(k_dedisp_dma<G, 32>);
+  } else {
+    const size_t lds = (size_t)chunk * lrow + (size_t)chunk * ND * 4 + (size_t)chunk * 4;
+    hipLaunchKernelGGL(k_dedisp<G>, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles,
+                       chunk, lrow);
+  }
   HAMD_CHECK(hipGetLastError());
 }
 
@@ -223,7 +546,9 @@ static int lrow_for(const DDTile &t, int max_spread)
 size_t dd_lds_bytes(const DDTile &t, int max_spread)
 {
   const int ND = NT / t.lanes_per_dm;
-  return (size_t)t.chunk * lrow_for(t, max_spread) + (size_t)t.chunk * ND * 4 + (size_t)t.chunk * 4;
+  const size_t one = (size_t)t.chunk * lrow_for(t, max_spread) + (size_t)t.chunk * ND * 4 +
+                     (size_t)t.chunk * 4;
+  return t.dma == 1 ? 2 * one : one;
 }
 
 void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
@@ -251,16 +576,26 @@ void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
   for (size_t k = 0; k < list.size(); ++k) tiles[(k % q) * nx + k / q] = list[k];
 }
 
-void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t,
-               int max_spread, hipStream_t s)
+void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t_in,
+               int max_spread, hipStream_t s, int64_t min_ring)
 {
   if (ntiles <= 0) return;
+  DDTile t = t_in;
+  // shrink the channel chunk (then drop the second DMA buffer) until the
+  // workgroup's LDS fits the launch limit
+  while (dd_lds_bytes(t, max_spread) > DD_LDS_BUDGET && t.chunk > 8) t.chunk /= 2;
+  if (dd_lds_bytes(t, max_spread) > DD_LDS_BUDGET && t.dma == 1) t.dma = 0;
+  if (dd_lds_bytes(t, max_spread) > DD_LDS_BUDGET)
+    throw HipError{hipErrorInvalidValue, "dedispersion window exceeds LDS (delay spread too wide)",
+                   __LINE__};
   const int lrow = lrow_for(t, max_spread);
+  // the register-prefetch loader wraps columns with a single subtraction
+  if (t.dma == 2 && (min_ring < (int64_t)lrow + 16 || t.chunk > 16)) t.dma = 0;
   switch (t.lanes_per_dm) {
-    case 8: launch_g<8>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
-    case 16: launch_g<16>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
-    case 32: launch_g<32>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
-    case 64: launch_g<64>(d_beams, d_tiles, ntiles, t.chunk, lrow, s); break;
+    case 8: launch_g<8>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;
+    case 16: launch_g<16>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;
+    case 32: launch_g<32>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;
+    case 64: launch_g<64>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;
     default: throw HipError{hipErrorInvalidValue, "lanes_per_dm", __LINE__};
   }
 }

@@ -12,27 +12,13 @@
 // Elementwise passes use the same (row, block, lane) mapping, which lets the
 // reduction of a pass's output ride along in the pass that writes it.
 #include "flagger.hpp"
+#include "reduce.hpp"
 
 namespace hamd {
 
 namespace {
 
 constexpr int NT = 256;
-
-__device__ __forceinline__ double lane_tree(double v, double *sh)
-{
-  const int l = threadIdx.x;
-  sh[l] = v;
-  __syncthreads();
-#pragma unroll
-  for (int s = NT / 2; s >= 1; s >>= 1) {
-    if (l < s) sh[l] += sh[l + s];
-    __syncthreads();
-  }
-  double r = sh[0];
-  __syncthreads();
-  return r;
-}
 
 __device__ __forceinline__ uint8_t quant(float x)
 {
@@ -81,9 +67,9 @@ __global__ __launch_bounds__(256) void k_ingest(const FlagBeamDev *__restrict__ 
 }
 
 // ---- generic elementwise pass + canonical row partial of its output
+// a0/a1: SPEC -> channel-mean range; NORM_FLAG1 -> +/- threshold
 __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ beams, int W,
-                                              int op, int part, int npart, float smin,
-                                              float smax)
+                                              int op, int part, int npart, float a0, float a1)
 {
   const FlagBeamDev &B = beams[blockIdx.z];
   const int c = blockIdx.y;
@@ -92,17 +78,18 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
   const int l = threadIdx.x;
   float *row = B.X + (int64_t)c * B.xs;
   const int base = blk * FLAG_BLOCK;
-  __shared__ double sh[NT];
+  __shared__ double sh[2 * 128];
   const FScal sc = *B.scal;
   float bpv = 0.f;
   bool bad = false;
-  if (op == FOP_DIVBP) bpv = B.bp[c];
+  if (op == FOP_DIVBP || op == FOP_DIVBP_TS) bpv = B.bp[c];
   if (op == FOP_SPEC) {
     const float b2 = B.bp[c];
-    bad = (b2 < smin) || (b2 > smax);
+    bad = (b2 < a0) || (b2 > a1);
   }
-  double acc = 0.0;
-  const double mu = (part == FPART_SQDEV) ? sc.mu : 0.0;
+  const bool fixed = c >= B.flag1 && c < B.flag2;
+  uint32_t cnt = 0;
+  double acc[2] = {0.0, 0.0};
 #pragma unroll 4
   for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
     const int t = base + l + NT * k;
@@ -119,19 +106,37 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
         B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
         break;
       }
+      case FOP_NORM_FLAG1: {
+        // box of 1 x 1: the box mean is the sample itself (kernel 1.0f)
+        x = (float)(((double)x - sc.mu) * sc.inv);
+        const bool g = (x > a0) || (x < a1);
+        cnt += g ? 1u : 0u;
+        if (g || fixed) x = 0.f;
+        x = x + 1.f;
+        row[t] = x;
+        break;
+      }
+      case FOP_DIVBP_TS: {
+        x = bpv > 0.f ? x / bpv : 0.f;
+        x = x / B.ts[t];
+        row[t] = x;
+        break;
+      }
       default: break;
     }
-    if (t < npart) {
-      if (part == FPART_SUM) acc += (double)x;
-      else if (part == FPART_SQDEV) {
-        const double d = (double)x - mu;
-        acc += d * d;
-      }
+    if (t < npart && part != FPART_NONE) {
+      const double d = (double)x;
+      acc[0] += d;
+      acc[1] += d * d;
     }
   }
+  if (cnt) atomicAdd(&B.flagcount[c], cnt);
   if (part == FPART_NONE) return;
-  const double s = lane_tree(acc, sh);
-  if (l == 0) B.part[(int64_t)c * B.nblk + blk] = s;
+  tree256_multi(acc, sh);
+  if (l == 0) {
+    B.part[(int64_t)c * B.nblk + blk] = acc[0];
+    if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk] = acc[1];
+  }
 }
 
 // bandpass from partials: bp[c] = (float)(sum_b part[c][b] / W)
@@ -145,36 +150,50 @@ __global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict_
   B.bp[c] = (float)(s / (double)W);
 }
 
-// global statistic over rows: value(r) = (sum_b part[r][b]) / n, combined by
-// the canonical blocked rule and divided by the row count.
-// which == 0: mean -> scal.mu; which == 1: variance -> scal.sigf, scal.inv
-__global__ __launch_bounds__(NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams,
-                                                     int n, int which)
+// Statistics of the plane from the block sums (calculate_stddev,
+// normalization.cu:125-162): S1_r, S2_r = ascending sums of row r's block
+// partials; mu = canonical sum over rows of S1_r / n, over the row count; the
+// variance is the same combination of ((S2_r - 2 mu S1_r) + n mu mu) / n.
+__device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool var, double *sh)
 {
-  const FlagBeamDev &B = beams[blockIdx.x];
-  __shared__ double sh[NT];
   const int l = threadIdx.x;
+  const double dn = (double)n;
   double total = 0.0;
   for (int r0 = 0; r0 < B.nch; r0 += FLAG_BLOCK) {
-    double acc = 0.0;
+    double acc[1] = {0.0};
     for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
       const int r = r0 + l + NT * k;
       if (r >= B.nch) break;
-      double s = 0.0;
-      for (int b = 0; b < B.nblk; ++b) s += B.part[(int64_t)r * B.nblk + b];
-      acc += s / (double)n;
+      double s1 = 0.0;
+      for (int b = 0; b < B.nblk; ++b) s1 += B.part[(int64_t)r * B.nblk + b];
+      if (!var) {
+        acc[0] += s1 / dn;
+      } else {
+        double s2 = 0.0;
+        for (int b = 0; b < B.nblk; ++b) s2 += B.part2[(int64_t)r * B.nblk + b];
+        acc[0] += ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
+      }
     }
-    total += lane_tree(acc, sh);
+    tree256_multi(acc, sh);
+    if (l == 0) sh[128] = acc[0];
+    __syncthreads();
+    total += sh[128];
+    __syncthreads();
   }
-  if (l == 0) {
-    const double v = total / (double)B.nch;
-    if (which == 0) {
-      B.scal->mu = v;
-    } else {
-      const float sf = (float)sqrt(v);
-      B.scal->sigf = sf;
-      B.scal->inv = 1.0 / (double)sf;
-    }
+  return total / (double)B.nch;
+}
+
+__global__ __launch_bounds__(NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams, int n)
+{
+  const FlagBeamDev &B = beams[blockIdx.x];
+  __shared__ double sh[136];
+  const double mu = flag_rows_csum(B, n, 0.0, false, sh);
+  const double var = flag_rows_csum(B, n, mu, true, sh);
+  if (threadIdx.x == 0) {
+    const float sf = (float)sqrt(var);
+    B.scal->mu = mu;
+    B.scal->sigf = sf;
+    B.scal->inv = 1.0 / (double)sf;
   }
 }
 
@@ -220,8 +239,8 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
   const int base = blockIdx.x * FLAG_BLOCK;
   const int l = threadIdx.x;
   const bool fixed = c >= B.flag1 && c < B.flag2;
-  __shared__ double sh[NT];
-  double acc = 0.0;
+  __shared__ double sh[128];
+  double acc[1] = {0.0};
   uint32_t cnt = 0;
   float *row = B.X + (int64_t)c * B.xs;
   for (int kk = 0; kk < FLAG_BLOCK / NT; ++kk) {
@@ -238,28 +257,32 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
     if (g || fixed) x = 0.f;
     x = x + 1.f;
     row[t] = x;
-    if (t < npart) acc += (double)x;
+    if (t < npart) acc[0] += (double)x;
   }
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
-  const double s = lane_tree(acc, sh);
-  if (l == 0) B.part[(int64_t)c * B.nblk + blockIdx.x] = s;
+  tree256_multi(acc, sh);
+  if (l == 0) B.part[(int64_t)c * B.nblk + blockIdx.x] = acc[0];
 }
 
-// ts_correct (src/hella/src/timeseries.cu:19-74): ts[t] = sum_c x / nch in
-// channel order (double), then every channel divided by ts[t]
-__global__ __launch_bounds__(NT) void k_ts_correct(const FlagBeamDev *__restrict__ beams, int W)
+// ts_correct (src/hella/src/timeseries.cu:19-74) on the bandpass-corrected
+// plane: ts[t] = sum_c (x / bp[c]) / nch in channel order (double); the
+// division by ts happens in the following FOP_DIVBP_TS pass.
+__global__ __launch_bounds__(64) void k_ts_sum(const FlagBeamDev *__restrict__ beams, int W)
 {
   const FlagBeamDev &B = beams[blockIdx.z];
-  const int t = blockIdx.x * NT + threadIdx.x;
+  const int t = blockIdx.x * 64 + threadIdx.x;
   if (t >= W) return;
   double s = 0.0;
-  for (int c = 0; c < B.nch; ++c) s += (double)B.X[(int64_t)c * B.xs + t];
-  const float tv = (float)(s / (double)B.nch);
-  B.ts[t] = tv;
+  const float *col = B.X + t;
+  // the sum stays sequential in channel order; unrolling only lets the loads
+  // of 16 channels be in flight together
+#pragma unroll 16
   for (int c = 0; c < B.nch; ++c) {
-    float *p = B.X + (int64_t)c * B.xs + t;
-    *p = *p / tv;
+    const float b = B.bp[c];
+    const float x = col[(int64_t)c * B.xs];
+    s += (double)(b > 0.f ? x / b : 0.f);
   }
+  B.ts[t] = (float)(s / (double)B.nch);
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -281,55 +304,50 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   const dim3 gplane((unsigned)nblk, (unsigned)max_nch, (unsigned)nbeam);
   const dim3 grow((unsigned)((max_nch + NT - 1) / NT), 1, (unsigned)nbeam);
   const dim3 gbeam((unsigned)nbeam);
-  auto apply = [&](int op, int part, int npart) {
-    hipLaunchKernelGGL(k_apply, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, P.spec_min,
-                       P.spec_max);
+  auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
+    hipLaunchKernelGGL(k_apply, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0, a1);
   };
   auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
-  auto fin = [&](int which) {
-    hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(NT), 0, s, d_beams, wp, which);
-  };
-  // normalise (normalize_data, flagger.cu:198-214) on a plane whose row
-  // partials of sum(x) over wp columns are already in place
-  auto stats_after_sum = [&]() {
-    fin(0);
-    apply(FOP_NONE, FPART_SQDEV, wp);
-    fin(1);
-  };
+  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(NT), 0, s, d_beams, wp); };
 
   hipLaunchKernelGGL(k_ingest, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                     (unsigned)nbeam),
                      dim3(256), 0, s, d_beams, W);
-  // bandpass_flag (flagger.cu:216-242)
+  // bandpass_flag (flagger.cu:216-242): bandpass_correct, normalize_data,
+  // channel means of the normalised data, spectral flag
   apply(FOP_NONE, FPART_SUM, n32);
   bp();
-  apply(FOP_DIVBP, FPART_SUM, wp);
-  stats_after_sum();
+  apply(FOP_DIVBP, FPART_SUM2, wp);
+  stats();
   apply(FOP_NORM, FPART_SUM, n32);
   bp();
-  apply(FOP_SPEC, FPART_SUM, n32);
+  apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
   bp();
   // configured scrunches (flagger.cu:365-384)
   for (int i = 0; i < P.nscrunch; ++i) {
     const FlagScrunch &S = P.scrunch[i];
-    apply(FOP_DIVBP, FPART_SUM, wp);
-    stats_after_sum();
-    apply(FOP_NORM, FPART_NONE, 0);
-    const float k = (float)(1. / S.tscrunch / S.fscrunch);
+    apply(FOP_DIVBP, FPART_SUM2, wp);
+    stats();
     const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
     const float nthr = (float)(-1. * (double)thr);
-    hipLaunchKernelGGL(k_box_thresh, gplane, dim3(NT), 0, s, d_beams, W, S.tscrunch, S.fscrunch,
-                       k, thr, nthr);
-    hipLaunchKernelGGL(k_grow_replace, gplane, dim3(NT), 0, s, d_beams, W, S.tscrunch,
-                       S.fscrunch, n32);
+    if (S.tscrunch == 1 && S.fscrunch == 1) {
+      apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
+    } else {
+      apply(FOP_NORM, FPART_NONE, 0);
+      const float k = (float)(1. / S.tscrunch / S.fscrunch);
+      hipLaunchKernelGGL(k_box_thresh, gplane, dim3(NT), 0, s, d_beams, W, S.tscrunch, S.fscrunch,
+                         k, thr, nthr);
+      hipLaunchKernelGGL(k_grow_replace, gplane, dim3(NT), 0, s, d_beams, W, S.tscrunch,
+                         S.fscrunch, n32);
+    }
     bp();
   }
-  // apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401)
-  apply(FOP_DIVBP, FPART_NONE, 0);
-  hipLaunchKernelGGL(k_ts_correct, dim3((unsigned)((W + NT - 1) / NT), 1, (unsigned)nbeam),
-                     dim3(NT), 0, s, d_beams, W);
-  apply(FOP_NONE, FPART_SUM, wp);
-  stats_after_sum();
+  // apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401): bandpass_correct,
+  // ts_correct, normalize_data
+  hipLaunchKernelGGL(k_ts_sum, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam), dim3(64), 0,
+                     s, d_beams, W);
+  apply(FOP_DIVBP_TS, FPART_SUM2, wp);
+  stats();
   // transpose_output_handler (transpose.cu:120-145): quantise into the ring
   apply(FOP_NORM_QUANT, FPART_NONE, 0);
   HAMD_CHECK(hipGetLastError());

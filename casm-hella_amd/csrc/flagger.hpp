@@ -8,8 +8,16 @@ namespace hamd {
 constexpr int FLAG_BLOCK = 4096;  // canonical reduction block (samples)
 
 enum FlagInFmt { FLAG_IN_U8_TF = 0, FLAG_IN_F32_TF = 1, FLAG_IN_F16_FT = 2 };
-enum FlagOp { FOP_NONE = 0, FOP_DIVBP = 1, FOP_NORM = 2, FOP_SPEC = 3, FOP_NORM_QUANT = 4 };
-enum FlagPart { FPART_NONE = 0, FPART_SUM = 1, FPART_SQDEV = 2 };
+enum FlagOp {
+  FOP_NONE = 0,
+  FOP_DIVBP = 1,       // x / bp[c] (0 when bp <= 0)
+  FOP_NORM = 2,        // (x - mu) / sigma
+  FOP_SPEC = 3,        // 1 if the channel mean is out of range, else x + 1
+  FOP_NORM_QUANT = 4,  // normalise and quantise into the ring
+  FOP_NORM_FLAG1 = 5,  // normalise, then the 1 x 1 scrunch: clip |x| > thr to 0, +1
+  FOP_DIVBP_TS = 6     // (x / bp[c]) / ts[t]
+};
+enum FlagPart { FPART_NONE = 0, FPART_SUM = 1, FPART_SUM2 = 2 };
 
 // per-beam scalars kept on the device
 struct FScal {
@@ -33,7 +41,8 @@ struct FlagBeamDev {
   int64_t ring_stride; // bytes between channel rows
   int64_t ring_len;    // columns (multiple of 16); column = (col0 + t) mod ring_len
   int64_t col0;
-  double *part;        // [nch][nblk] block partial sums
+  double *part;        // [nch][nblk] block partial sums of x
+  double *part2;       // [nch][nblk] block partial sums of x*x
   float *bp;           // [nch] bandpass
   FScal *scal;
   uint32_t *flagcount; // [nch] grown-mask counts (accumulated)

@@ -111,9 +111,13 @@ struct DDTileDesc {
 struct DDTile {
   int lanes_per_dm;  // G: threads sharing one DM trial (time groups); ND = 256 / G
   int chunk;         // CC: channels staged in LDS per step (divides 256)
+  int dma = 1;       // 0: loads through registers; 1: double-buffered global->LDS DMA;
+                     // 2: next chunk prefetched in registers (chunk 8 or 16)
 };
 
 int dd_samples_per_thread();
+// LDS bytes a dedispersion workgroup may use (the per-workgroup launch limit)
+constexpr size_t DD_LDS_BUDGET = 64 * 1024;
 inline int dd_tile_T(const DDTile &t) { return t.lanes_per_dm * dd_samples_per_thread(); }
 inline int dd_tile_ND(const DDTile &t) { return 256 / t.lanes_per_dm; }
 // largest per-channel spread of delays inside any block of nd consecutive DMs
@@ -126,8 +130,9 @@ size_t dd_lds_bytes(const DDTile &t, int max_spread);
 void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
                     std::vector<DDTileDesc> &tiles, int order = 0);
 // Launch over a device tile table.
+// min_ring: smallest ring length among the beams of the launch
 void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t,
-               int max_spread, hipStream_t s);
+               int max_spread, hipStream_t s, int64_t min_ring = INT64_MAX);
 
 // u8 time-major [nsamps][in_stride] -> FT rows (row_stride), zero tail to ring
 void transpose_tf_to_ft(const uint8_t *in, int64_t nsamps, int64_t in_stride,

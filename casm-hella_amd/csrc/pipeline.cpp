@@ -179,6 +179,7 @@ void ctx_build(hella_ctx c)
     // flagger state
     const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
     B.f_part.ensure((size_t)B.nch * nblk * 8);
+    B.f_part2.ensure((size_t)B.nch * nblk * 8);
     B.f_bp.ensure((size_t)B.nch * 4);
     alloc_zero(B.f_scal, sizeof(FScal), s);
     alloc_zero(B.f_count, (size_t)B.nch * 4, s);
@@ -187,6 +188,7 @@ void ctx_build(hella_ctx c)
     const int sblk = (B.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
     const size_t ncnt = (size_t)c->nw * B.rows * sblk;
     B.s_part.ensure(ncnt * 8);
+    B.s_part2.ensure(ncnt * 8);
     alloc_zero(B.s_scal, sizeof(SScal), s);
     B.s_counts.ensure(ncnt * 4);
     B.s_offs.ensure(ncnt * 4);
@@ -200,21 +202,21 @@ void ctx_build(hella_ctx c)
   // tile shape shared by all beams: the preferred lane count whose LDS
   // footprint fits every beam with the largest channel chunk
   if (!c->tile_forced) {
-    DDTile best{64, 8};
-    for (int G : {16, 32, 64, 8}) {
-      int cc_ok = 0;
-      for (int cc : {32, 16, 8}) {
-        bool fits = true;
-        for (auto &B : c->beams) {
-          if (B.nlive == 0) continue;
-          const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
-          if (dd_lds_bytes(DDTile{G, cc}, sp) > 64 * 1024) { fits = false; break; }
-        }
-        if (fits) { cc_ok = cc; break; }
+    // measured on MI355X (1024 ch, DM 0-2000): 32 lanes per DM trial, 8-channel
+    // chunks staged through registers, time-major tile order; wider lane
+    // groups take over when the delay spread of 8 DM trials does not fit
+    DDTile best{64, 8, 0};
+    for (int G : {32, 64}) {
+      bool fits = true;
+      for (auto &B : c->beams) {
+        if (B.nlive == 0) continue;
+        const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
+        if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
       }
-      if (cc_ok) { best = DDTile{G, cc_ok}; break; }
+      if (fits) { best = DDTile{G, 8, 0}; break; }
     }
     c->tile = best;
+    c->tile_order = 1;
   }
   for (auto &B : c->beams)
     B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(c->tile)) : 0;
@@ -322,6 +324,7 @@ static FlagBeamDev make_fdesc(hella_ctx c, BeamState &B, bool own, const void *d
   f.ring_len = B.ntime;
   f.col0 = write_col(c, B);
   f.part = B.f_part.as<double>();
+  f.part2 = B.f_part2.as<double>();
   f.bp = B.f_bp.as<float>();
   f.scal = B.f_scal.as<FScal>();
   f.flagcount = B.f_count.as<uint32_t>();
@@ -358,6 +361,7 @@ static SearchBeamDev make_sdesc(hella_ctx c, BeamState &B, bool own)
   s.planes = own ? B.planes.as<float>() : c->planes.as<float>();
   s.ps = (int64_t)round_up((size_t)B.nout, 64);
   s.part = B.s_part.as<double>();
+  s.part2 = B.s_part2.as<double>();
   s.scal = B.s_scal.as<SScal>();
   s.counts = B.s_counts.as<uint32_t>();
   s.offs = B.s_offs.as<uint32_t>();
@@ -422,7 +426,7 @@ void ctx_dedisperse(hella_ctx c, int b)
     DDBeamDev *dd = c->d_ddesc.as<DDBeamDev>() + b;
     HAMD_CHECK(hipMemcpyAsync(dd, &c->h_ddesc[b], sizeof(DDBeamDev), hipMemcpyHostToDevice,
                               c->stream));
-    dd_launch(dd, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile, B.spread, c->stream);
+    dd_launch(dd, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile, B.spread, c->stream, B.ntime);
   }
   c->cur_beam = b;
 }
@@ -581,8 +585,10 @@ static void batch_record(hella_ctx c)
     const BeamState &B = c->beams[(size_t)bt.beams[(size_t)i]];
     if (B.nlive == 0) fill_zero_rows(B.D.p, B.dstride, B.ntime_dd, B.ndm, s);
   }
+  int64_t min_ring = INT64_MAX;
+  for (int b : bt.beams) min_ring = std::min(min_ring, c->beams[(size_t)b].ntime);
   dd_launch(c->d_ddesc.as<DDBeamDev>(), bt.d_tiles.as<DDTileDesc>(), bt.ntiles, bt.tile, bt.spread,
-            s);
+            s, min_ring);
   int max_rows = 0;
   for (int b : bt.beams) max_rows = std::max(max_rows, c->beams[(size_t)b].rows);
   SearchRunner::run(c->d_sdesc.as<SearchBeamDev>(), nr, max_rows, c->common,
@@ -755,7 +761,7 @@ float ctx_bench_stage(hella_ctx c, int b, int which, int reps)
     } else if (which == 1) {
       if (B.nlive)
         dd_launch(c->d_ddesc.as<DDBeamDev>() + b, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile,
-                  B.spread, s);
+                  B.spread, s, B.ntime);
     } else {
       SearchRunner::run(c->d_sdesc.as<SearchBeamDev>() + b, 1, B.rows, c->common,
                         c->d_kern.as<float>(), c->kofs, s);
@@ -783,29 +789,35 @@ void ctx_autotune(hella_ctx c, const char *path)
   batch_alloc(c);
   // time the dedispersion of every beam's current window with each shape
   std::vector<DDBeamDev> dd;
+  int64_t min_ring = INT64_MAX;
   for (auto &B : c->beams)
-    if (B.nlive > 0) dd.push_back(make_ddesc(c, B, true));
+    if (B.nlive > 0) {
+      dd.push_back(make_ddesc(c, B, true));
+      min_ring = std::min(min_ring, B.ntime);
+    }
   if (dd.empty()) return;
   DevBuf d_beams;
   d_beams.ensure(dd.size() * sizeof(DDBeamDev));
   HAMD_CHECK(hipMemcpy(d_beams.p, dd.data(), dd.size() * sizeof(DDBeamDev), hipMemcpyHostToDevice));
-  struct Res { int G, cc, order; float ms; };
+  struct Res { int G, cc, order, dma; float ms; };
   std::vector<Res> res;
   hipEvent_t e0, e1;
   HAMD_CHECK(hipEventCreate(&e0));
   HAMD_CHECK(hipEventCreate(&e1));
   DevBuf d_tiles;
+  for (int dma : {2, 1, 0})
   for (int G : {8, 16, 32, 64})
     for (int cc : {8, 16, 32})
       for (int order : {0, 1}) {
-        DDTile t{G, cc};
+        if (dma == 2 && cc == 32) continue;
+        DDTile t{G, cc, dma};
         int spread = 0;
         bool fits = true;
         for (auto &B : c->beams) {
           if (!B.nlive) continue;
           const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
           spread = std::max(spread, sp);
-          if (dd_lds_bytes(t, sp) > 64 * 1024) fits = false;
+          if (dd_lds_bytes(t, sp) > DD_LDS_BUDGET) fits = false;
         }
         if (!fits) continue;
         std::vector<DDTileDesc> tiles;
@@ -813,25 +825,26 @@ void ctx_autotune(hella_ctx c, const char *path)
         d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
         HAMD_CHECK(hipMemcpy(d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
                              hipMemcpyHostToDevice));
+        if (dma == 2 && min_ring < (int64_t)dd_lds_bytes(DDTile{G, 1, 0}, spread)) continue;
         dd_launch(d_beams.as<DDBeamDev>(), d_tiles.as<DDTileDesc>(), (int)tiles.size(), t, spread,
-                  c->stream);
+                  c->stream, min_ring);
         HAMD_CHECK(hipEventRecord(e0, c->stream));
         const int reps = 3;
         for (int r = 0; r < reps; ++r)
           dd_launch(d_beams.as<DDBeamDev>(), d_tiles.as<DDTileDesc>(), (int)tiles.size(), t,
-                    spread, c->stream);
+                    spread, c->stream, min_ring);
         HAMD_CHECK(hipEventRecord(e1, c->stream));
         HAMD_CHECK(hipEventSynchronize(e1));
         float ms = 0.f;
         HAMD_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        res.push_back(Res{G, cc, order, ms / reps});
+        res.push_back(Res{G, cc, order, dma, ms / reps});
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (res.empty()) return;
   const Res best = *std::min_element(res.begin(), res.end(),
                                      [](const Res &a, const Res &b) { return a.ms < b.ms; });
-  c->tile = DDTile{best.G, best.cc};
+  c->tile = DDTile{best.G, best.cc, best.dma};
   c->tile_order = best.order;
   c->tile_forced = true;
   for (auto &B : c->beams) {
@@ -853,13 +866,13 @@ void ctx_autotune(hella_ctx c, const char *path)
   if (path && *path) {
     FILE *f = std::fopen(path, "w");
     if (f) {
-      std::fprintf(f, "{\"best\": {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"ms\": %.4f},\n",
-                   best.G, best.cc, best.order, best.ms);
+      std::fprintf(f, "{\"best\": {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"dma\": %d, \"ms\": %.4f},\n",
+                   best.G, best.cc, best.order, best.dma, best.ms);
       std::fprintf(f, " \"nbeam\": %zu, \"nchan_max\": %d, \"gulp\": %d,\n \"trials\": [", c->beams.size(),
                    c->max_nch, c->W);
       for (size_t i = 0; i < res.size(); ++i)
-        std::fprintf(f, "%s\n  {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"ms\": %.4f}",
-                     i ? "," : "", res[i].G, res[i].cc, res[i].order, res[i].ms);
+        std::fprintf(f, "%s\n  {\"lanes_per_dm\": %d, \"chunk\": %d, \"order\": %d, \"dma\": %d, \"ms\": %.4f}",
+                     i ? "," : "", res[i].G, res[i].cc, res[i].order, res[i].dma, res[i].ms);
       std::fprintf(f, "]}\n");
       std::fclose(f);
     }

@@ -53,9 +53,9 @@ struct BeamState {
   int64_t in_bytes = 0;
   DevBuf ring;
   // flagger state
-  DevBuf f_part, f_bp, f_scal, f_count, f_ts;
+  DevBuf f_part, f_part2, f_bp, f_scal, f_count, f_ts;
   // search state
-  DevBuf s_scal, s_part, s_counts, s_offs, s_tiles, s_cands, s_ncand;
+  DevBuf s_scal, s_part, s_part2, s_counts, s_offs, s_tiles, s_cands, s_ncand;
   // work planes owned by the beam (batched path)
   DevBuf X, mask, D, planes;
   int64_t dstride = 0;
@@ -71,7 +71,7 @@ struct Batch {
   std::vector<int> beams;  // searchable beams, in order
   DevBuf d_tiles;
   int ntiles = 0;
-  DDTile tile{16, 32};
+  DDTile tile{16, 32, 1};
   int spread = 0;
   hipGraphExec_t exec = nullptr;
   bool graph_ok = false;
@@ -101,7 +101,7 @@ struct hella_ctx_s {
   hamd::DevBuf d_kern;
   std::vector<int> kofs;
   hamd::SearchCommon common{};
-  hamd::DDTile tile{16, 32};
+  hamd::DDTile tile{16, 32, 1};
   int tile_order = 0;
   bool tile_forced = false;
   int cur_beam = -1;           // beam whose plane is in the shared D

@@ -17,6 +17,7 @@
 // (4096-sample blocks, 256 lanes, pairwise lane tree, ascending blocks).
 // Planes are never stored in BOXCAR mode: each pass recomputes the boxes
 // from D, which is cheaper than writing and re-reading nw planes.
+#include "reduce.hpp"
 #include "search.hpp"
 
 #include <cmath>
@@ -29,22 +30,7 @@ constexpr int NT = 256;
 constexpr int SB = SEARCH_BLOCK;
 constexpr int MAXBOX = 1024;
 
-enum Pass { P_MEAN = 0, P_VAR = 1, P_Y0MEAN = 2, P_Y0VAR = 3, P_COUNT = 4, P_WRITE = 5 };
-
-__device__ double tree256(double v, double *sh)
-{
-  const int l = threadIdx.x;
-  sh[l] = v;
-  __syncthreads();
-#pragma unroll
-  for (int s = NT / 2; s >= 1; s >>= 1) {
-    if (l < s) sh[l] += sh[l + s];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
-}
+enum Pass { P_STATS = 0, P_Y0 = 1, P_COUNT = 2, P_WRITE = 3 };
 
 // exclusive scan of one u32 per thread over the workgroup
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total)
@@ -79,15 +65,20 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
   const int b = blockIdx.x;
   const int base = b * SB;
   const int l = threadIdx.x;
-  __shared__ uint32_t P[SB + MAXBOX + 1];
-  __shared__ double shd[NT];
+  // P (prefix sums) and the reduction scratch share the LDS
+  constexpr int PN = SB + MAXBOX + 1;
+  constexpr int TREE_WORDS = 2 * SEARCH_MAXW * 128 * 2;
+  __shared__ __align__(16) uint32_t lds_u[(PN > TREE_WORDS ? PN : TREE_WORDS)];
   __shared__ uint32_t shu[NT];
+  __shared__ uint32_t shc[SEARCH_MAXW];
+  uint32_t *P = lds_u;
+  double *shd = reinterpret_cast<double *>(lds_u);
   const SScal &sc = *B.scal;
-  const int nwp = (pass == P_Y0MEAN || pass == P_Y0VAR) ? 1 : C.nw;
+  const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
+  const int c0 = C.maxW / 2;
 
   if (pass == P_COUNT && sc.gated) {
-    for (int k = l; k < C.nw; k += NT)
-      B.counts[((int64_t)k * B.rows + r) * C.nblk + b] = 0u;
+    for (int k = l; k < C.nw; k += NT) B.counts[((int64_t)k * B.rows + r) * C.nblk + b] = 0u;
     return;
   }
   if (pass == P_WRITE) {
@@ -95,10 +86,10 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
     for (int k = 0; k < C.nw; ++k) any |= B.counts[((int64_t)k * B.rows + r) * C.nblk + b];
     if (!any) return;
   }
-
-  if (!HELLA) {
+  // width 1 needs no prefix sums: its box is the dedispersed sample itself
+  const bool direct = !HELLA && pass == P_Y0 && C.minW == 1;
+  if (!HELLA && !direct) {
     // prefix sums of D[row][base .. base + L) in LDS (u32, wrap-around)
-    const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
     const int L = SB + C.maxW;
     for (int i = l; i < L; i += NT) {
       const int col = base + i;
@@ -120,47 +111,58 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
     __syncthreads();
   }
 
-  const int c0 = C.maxW / 2;
   auto raw = [&](int k, int j) -> double {
     if (HELLA) return (double)B.planes[((int64_t)k * B.rows + r) * B.ps + base + j];
+    if (direct) return (double)drow[base + j + c0];
     const int w = C.minW << k;
     const int a = j + c0 - w / 2;
     return (double)(uint32_t)(P[a + w] - P[a]);
   };
 
-  if (pass <= P_Y0VAR) {
-    double acc[SEARCH_MAXW];
+  if (pass == P_STATS || pass == P_Y0) {
+    const int nwp = pass == P_Y0 ? 1 : C.nw;
+    double acc[2 * SEARCH_MAXW];
 #pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k) acc[k] = 0.0;
+    for (int k = 0; k < 2 * SEARCH_MAXW; ++k) acc[k] = 0.0;
     for (int kk = 0; kk < SB / NT; ++kk) {
       const int j = l + NT * kk;
       if (base + j >= C.wp) break;
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k) {
-        if (k >= nwp) break;
-        const double v = raw(k, j);
-        if (pass == P_MEAN) {
+        if (k < nwp) {
+          double v = raw(k, j);
+          if (pass == P_Y0) v = (double)normed(v, sc, 0);
           acc[k] += v;
-        } else if (pass == P_VAR) {
-          const double d = v - sc.mu[k];
-          acc[k] += d * d;
-        } else if (pass == P_Y0MEAN) {
-          acc[k] += (double)normed(v, sc, 0);
-        } else {
-          const double d = (double)normed(v, sc, 0) - sc.mu0;
-          acc[k] += d * d;
+          acc[SEARCH_MAXW + k] += v * v;
         }
       }
     }
-    for (int k = 0; k < nwp; ++k) {
-      const double s = tree256(acc[k], shd);
-      if (l == 0) B.part[((int64_t)k * B.rows + r) * C.nblk + b] = s;
+    __syncthreads();  // P is dead; its LDS becomes the tree scratch
+    if (pass == P_Y0) {
+      double two[2] = {acc[0], acc[SEARCH_MAXW]};
+      tree256_multi(two, shd);
+      acc[0] = two[0];
+      acc[SEARCH_MAXW] = two[1];
+    } else {
+      tree256_multi(acc, shd);
+    }
+    if (l == 0) {
+#pragma unroll
+      for (int k = 0; k < SEARCH_MAXW; ++k) {
+        if (k < nwp) {
+          const int64_t slot = ((int64_t)k * B.rows + r) * C.nblk + b;
+          B.part[slot] = acc[k];
+          B.part2[slot] = acc[SEARCH_MAXW + k];
+        }
+      }
     }
     return;
   }
 
   const float thr = sc.thr;
   if (pass == P_COUNT) {
+    if (l < SEARCH_MAXW) shc[l] = 0u;
+    __syncthreads();
     uint32_t cnt[SEARCH_MAXW];
 #pragma unroll
     for (int k = 0; k < SEARCH_MAXW; ++k) cnt[k] = 0;
@@ -168,16 +170,26 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
       const int j = l + NT * kk;
       if (base + j >= C.nout) break;
 #pragma unroll
-      for (int k = 0; k < SEARCH_MAXW; ++k) {
-        if (k >= C.nw) break;
-        cnt[k] += normed(raw(k, j), sc, k) > thr ? 1u : 0u;
+      for (int k = 0; k < SEARCH_MAXW; ++k)
+        if (k < C.nw) {
+          if (HELLA)
+            cnt[k] += normed(raw(k, j), sc, k) > thr ? 1u : 0u;
+          else
+            cnt[k] += (uint64_t)raw(k, j) >= sc.rthr[k] ? 1u : 0u;
+        }
+    }
+    // integer totals are order free: wave sums, then LDS atomics
+#pragma unroll
+    for (int k = 0; k < SEARCH_MAXW; ++k) {
+      if (k < C.nw) {
+        uint32_t x = cnt[k];
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) x += __shfl_down(x, s, 64);
+        if ((l & 63) == 0 && x) atomicAdd(&shc[k], x);
       }
     }
-    for (int k = 0; k < C.nw; ++k) {
-      uint32_t tot;
-      (void)block_excl_scan(cnt[k], shu, &tot);
-      if (l == 0) B.counts[((int64_t)k * B.rows + r) * C.nblk + b] = tot;
-    }
+    __syncthreads();
+    if (l < C.nw) B.counts[((int64_t)l * B.rows + r) * C.nblk + b] = shc[l];
     return;
   }
 
@@ -192,8 +204,9 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
       float y = 0.f;
       bool hit = false;
       if (in) {
-        y = normed(raw(k, j), sc, k);
-        hit = y > thr;
+        const double v = raw(k, j);
+        hit = HELLA ? normed(v, sc, k) > thr : (uint64_t)v >= sc.rthr[k];
+        if (hit) y = normed(v, sc, k);
       }
       uint32_t tot;
       const uint32_t rank = block_excl_scan(hit ? 1u : 0u, shu, &tot);
@@ -210,44 +223,76 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
   }
 }
 
-// row values (sum_b part / wp) combined over rows; which = pass of the partials
+// Statistics from the block sums (calculate_stddev, normalization.cu:125-162):
+// S1_r, S2_r = ascending sums of the row's block partials; mu = canonical
+// sum over rows of S1_r / n, divided by the row count; the row variance about
+// mu is ((S2_r - 2 mu S1_r) + n mu mu) / n, combined the same way.
+__device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k, double mu,
+                            bool var, double *sh)
+{
+  const int l = threadIdx.x;
+  const double n = (double)C.wp;
+  double total = 0.0;
+  for (int r0 = 0; r0 < B.rows; r0 += SB) {
+    double acc[1] = {0.0};
+    for (int kk = 0; kk < SB / NT; ++kk) {
+      const int r = r0 + l + NT * kk;
+      if (r >= B.rows) break;
+      const int64_t base = ((int64_t)k * B.rows + r) * C.nblk;
+      double s1 = 0.0;
+      for (int b = 0; b < C.nblk; ++b) s1 += B.part[base + b];
+      if (!var) {
+        acc[0] += s1 / n;
+      } else {
+        double s2 = 0.0;
+        for (int b = 0; b < C.nblk; ++b) s2 += B.part2[base + b];
+        acc[0] += ((s2 - (2.0 * mu) * s1) + (n * mu) * mu) / n;
+      }
+    }
+    tree256_multi(acc, sh);
+    if (l == 0) sh[256] = acc[0];
+    __syncthreads();
+    total += sh[256];
+    __syncthreads();
+  }
+  return total / (double)B.rows;
+}
+
 __global__ __launch_bounds__(NT) void k_finish(const SearchBeamDev *__restrict__ beams,
                                                SearchCommon C, int which)
 {
   const SearchBeamDev &B = beams[blockIdx.y];
   const int k = blockIdx.x;
-  const int l = threadIdx.x;
-  __shared__ double sh[NT];
-  double total = 0.0;
-  for (int r0 = 0; r0 < B.rows; r0 += SB) {
-    double acc = 0.0;
-    for (int kk = 0; kk < SB / NT; ++kk) {
-      const int r = r0 + l + NT * kk;
-      if (r >= B.rows) break;
-      double s = 0.0;
-      for (int b = 0; b < C.nblk; ++b) s += B.part[((int64_t)k * B.rows + r) * C.nblk + b];
-      acc += s / (double)C.wp;
-    }
-    total += tree256(acc, sh);
-  }
-  if (l != 0) return;
-  const double v = total / (double)B.rows;
+  __shared__ double sh[260];  // 128 tree slots + the broadcast slot 256
+  const double mu = rows_csum(B, C, k, 0.0, false, sh);
+  const double var = rows_csum(B, C, k, mu, true, sh);
   SScal &sc = *B.scal;
-  if (which == P_MEAN) {
-    sc.mu[k] = v;
-  } else if (which == P_VAR) {
-    const float sf = (float)sqrt(v);
+  const float sf = (float)sqrt(var);
+  if (which == P_STATS) {
+    if (threadIdx.x != 0) return;
+    sc.mu[k] = mu;
     sc.sigf[k] = sf;
     sc.inv[k] = 1.0 / (double)sf;
-  } else if (which == P_Y0MEAN) {
-    sc.mu0 = v;
-  } else {
-    const float s0 = (float)sqrt(v);
-    sc.sig0 = s0;
-    // peaks.cu:278: search only when 0.1 < sigma < 5
-    sc.gated = (s0 < 5.f && s0 > 0.1f) ? 0 : 1;
-    sc.thr = C.snr * s0;
+    return;
   }
+  const float thr = C.snr * sf;
+  const int kw = threadIdx.x;
+  if (kw < C.nw && C.mode == SEARCH_BOXCAR) {
+    // binary search for the first integer sum that crosses the threshold
+    uint64_t lo = 0, hi = 1ull << 32;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (normed((double)mid, sc, kw) > thr) hi = mid;
+      else lo = mid + 1;
+    }
+    sc.rthr[kw] = lo;
+  }
+  if (threadIdx.x != 0) return;
+  sc.mu0 = mu;
+  sc.sig0 = sf;
+  // peaks.cu:278: search only when 0.1 < sigma < 5
+  sc.gated = (sf < 5.f && sf > 0.1f) ? 0 : 1;
+  sc.thr = thr;
 }
 
 // ---- ordered compaction offsets: exclusive scan of counts per beam
@@ -400,14 +445,12 @@ void SearchRunner::smooth(const SearchBeamDev *d_beams, int nbeam, int max_rows,
                          dim3(NT), 0, s, d_beams, C, k, sm, d_kern + kofs[(size_t)k]);
     }
   }
-  for (int p : {(int)P_MEAN, (int)P_VAR}) {
-    if (hella)
-      hipLaunchKernelGGL(k_pass<true>, gp, dim3(NT), 0, s, d_beams, C, p);
-    else
-      hipLaunchKernelGGL(k_pass<false>, gp, dim3(NT), 0, s, d_beams, C, p);
-    hipLaunchKernelGGL(k_finish, dim3((unsigned)C.nw, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C,
-                       p);
-  }
+  if (hella)
+    hipLaunchKernelGGL(k_pass<true>, gp, dim3(NT), 0, s, d_beams, C, (int)P_STATS);
+  else
+    hipLaunchKernelGGL(k_pass<false>, gp, dim3(NT), 0, s, d_beams, C, (int)P_STATS);
+  hipLaunchKernelGGL(k_finish, dim3((unsigned)C.nw, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C,
+                     (int)P_STATS);
   HAMD_CHECK(hipGetLastError());
 }
 
@@ -426,10 +469,8 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
   auto fin = [&](int p) {
     hipLaunchKernelGGL(k_finish, dim3(1u, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C, p);
   };
-  pass(P_Y0MEAN);
-  fin(P_Y0MEAN);
-  pass(P_Y0VAR);
-  fin(P_Y0VAR);
+  pass(P_Y0);
+  fin(P_Y0);
   pass(P_COUNT);
   const int64_t n = (int64_t)C.nw * max_rows * C.nblk;
   const unsigned ntiles = (unsigned)((n + SB - 1) / SB);

@@ -7,7 +7,7 @@
 namespace hamd {
 
 constexpr int SEARCH_BLOCK = 4096;  // canonical reduction block (samples)
-constexpr int SEARCH_MAXW = 16;     // maximum number of width trials
+constexpr int SEARCH_MAXW = 12;     // maximum number of width trials
 
 enum SearchMode { SEARCH_BOXCAR = 0, SEARCH_HELLA = 1 };
 
@@ -27,6 +27,9 @@ struct SScal {
   float thr;       // snr * sig0
   int gated;       // 1 when sig0 is outside (0.1, 5): no candidates
   int pad;
+  // BOXCAR: smallest integer box sum whose normalised value exceeds thr
+  // (normalisation is monotone in the sum); 2^32 when none does
+  uint64_t rthr[SEARCH_MAXW];
 };
 
 struct SearchBeamDev {
@@ -36,7 +39,8 @@ struct SearchBeamDev {
   int row0;           // first D row searched
   float *planes;      // hella mode: raw smoothed planes [nw][rows][ps]
   int64_t ps;
-  double *part;       // [nw][rows][nblk]
+  double *part;       // [nw][rows][nblk] block sums of x
+  double *part2;      // [nw][rows][nblk] block sums of x*x
   SScal *scal;
   uint32_t *counts;   // [nw][rows][nblk]
   uint32_t *offs;     // exclusive scan of counts

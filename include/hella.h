@@ -159,7 +159,9 @@ hella_status hella_bench_stage(hella_ctx ctx, int beam, int which, int reps, flo
  * the choice and its timings are written as JSON to path (may be NULL) */
 hella_status hella_autotune(hella_ctx ctx, const char *path);
 /* force a tile shape: lanes per DM trial (8/16/32/64), channels per LDS chunk
- * (8/16/32), tile order (0 DM-block-major, 1 time-major) */
+ * (8/16/32); order bit 0: tile order (0 DM-block-major, 1 time-major); order
+ * bits 1-2: window loader (0 LDS DMA, 1 through registers, 2 next chunk
+ * prefetched in registers, chunk 8 or 16) */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
 
 /* ---- device helpers (test / bench plumbing) ------------------------------ */

@@ -132,29 +132,44 @@ static size_t stats_width(size_t w)
   return wp ? wp : w;
 }
 
+/* Per row r: S1_r = csum(x), S2_r = csum(x*x) over the statistics columns;
+ * mu = csum_r(S1_r / n) / h;  v_r = ((S2_r - (2 mu) S1_r) + (n mu) mu) / n;
+ * var = csum_r(v_r) / h.  This is the reference's mean of row means and mean
+ * of row variances about the global mean (normalization.cu:135-151) with the
+ * row sums of squared deviations expanded, so one pass over the data gives
+ * both moments. */
+void orc_stats_rows(const double *S1, const double *S2, size_t h, double n,
+                    double *mu, double *var)
+{
+  double *rr = (double *)calloc(h ? h : 1, sizeof(double));
+  for (size_t r = 0; r < h; ++r) rr[r] = S1[r] / n;
+  double m = orc_csum(rr, h) / (double)h;
+  for (size_t r = 0; r < h; ++r) rr[r] = ((S2[r] - (2.0 * m) * S1[r]) + (n * m) * m) / n;
+  double v = orc_csum(rr, h) / (double)h;
+  free(rr);
+  *mu = m;
+  *var = v;
+}
+
 void orc_stats(const float *x, size_t h, size_t w, size_t stride,
                double *mu, double *var)
 {
   size_t wp = stats_width(w);
-  double *tmp = (double *)malloc((wp ? wp : 1) * sizeof(double));
-  double *rr = (double *)calloc(h ? h : 1, sizeof(double));
-  for (size_t r = 0; r < h; ++r) {
-    for (size_t j = 0; j < wp; ++j) tmp[j] = (double)x[r * stride + j];
-    rr[r] = orc_csum(tmp, wp) / (double)wp;
-  }
-  double m = orc_csum(rr, h) / (double)h;
+  double *t1 = (double *)malloc((wp ? wp : 1) * sizeof(double));
+  double *t2 = (double *)malloc((wp ? wp : 1) * sizeof(double));
+  double *S1 = (double *)calloc(h ? h : 1, sizeof(double));
+  double *S2 = (double *)calloc(h ? h : 1, sizeof(double));
   for (size_t r = 0; r < h; ++r) {
     for (size_t j = 0; j < wp; ++j) {
-      double d = (double)x[r * stride + j] - m;
-      tmp[j] = d * d;
+      double v = (double)x[r * stride + j];
+      t1[j] = v;
+      t2[j] = v * v;
     }
-    rr[r] = orc_csum(tmp, wp) / (double)wp;
+    S1[r] = orc_csum(t1, wp);
+    S2[r] = orc_csum(t2, wp);
   }
-  double v = orc_csum(rr, h) / (double)h;
-  free(tmp);
-  free(rr);
-  *mu = m;
-  *var = v;
+  orc_stats_rows(S1, S2, h, (double)wp, mu, var);
+  free(t1); free(t2); free(S1); free(S2);
 }
 
 void orc_stats_f(const float *x, size_t h, size_t w, size_t stride,
@@ -465,16 +480,20 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
       for (int r = 0; r < rows; ++r)
         for (int s = 0; s < nout; ++s) rawplane[(size_t)r * nout + s] = plane_raw(&p, r, s);
     }
-    for (int r = 0; r < rows; ++r) rr[r] = orc_csum(rawplane + (size_t)r * nout, wp) / (double)wp;
-    double m = orc_csum(rr, (size_t)rows) / (double)rows;
-    for (int r = 0; r < rows; ++r) {
-      for (size_t j = 0; j < wp; ++j) {
-        double d = rawplane[(size_t)r * nout + j] - m;
-        tmp[j] = d * d;
+    double m, v;
+    {
+      double *S1 = (double *)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+      double *S2 = (double *)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+      for (int r = 0; r < rows; ++r) {
+        const double *row = rawplane + (size_t)r * nout;
+        for (size_t j = 0; j < wp; ++j) tmp[j] = row[j] * row[j];
+        S1[r] = orc_csum(row, wp);
+        S2[r] = orc_csum(tmp, wp);
       }
-      rr[r] = orc_csum(tmp, wp) / (double)wp;
+      orc_stats_rows(S1, S2, (size_t)rows, (double)wp, &m, &v);
+      free(S1);
+      free(S2);
     }
-    double v = orc_csum(rr, (size_t)rows) / (double)rows;
     float sf = (float)sqrt(v);
     mu_w[k] = (float)m;
     sig_w[k] = sf;

@@ -56,6 +56,10 @@ void orc_dedisperse(const uint8_t *in, size_t in_tstride, size_t in_cstride,
                     uint32_t *out, size_t out_stride);
 
 /* ---- A4 statistics (src/normalization.cu:125-162) ----------------------- */
+/* combine per-row sums S1 (of x) and S2 (of x^2) over n columns into the
+ * global mean and the mean of row variances about it */
+void orc_stats_rows(const double *S1, const double *S2, size_t h, double n,
+                    double *mu, double *var);
 /* mean of row means / mean of row variances over the first 256*floor(w/256)
  * columns (all w columns when w < 256). */
 void orc_stats(const float *x, size_t h, size_t w, size_t stride,

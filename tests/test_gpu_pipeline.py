@@ -192,3 +192,29 @@ def test_injection_recovery_c1(hamd, orc):
     assert best["snr"] > cfg["SNR"]
     # same answer from the oracle on the same plane
     _check_search(orc, h, cfg, h.dedisp_plane(0))
+
+
+@pytest.mark.parametrize("lanes,chunk,order", [(16, 8, 0), (32, 8, 1), (64, 16, 1), (8, 32, 0)])
+@pytest.mark.parametrize("loader", [0, 1, 2])
+def test_tile_shapes_and_loaders(hamd, orc, lanes, chunk, order, loader):
+    """Every dedispersion tile shape / window loader gives the same exact sums
+    (loader 0: LDS DMA double buffer, 1: through registers, 2: register
+    prefetch; 2 needs chunk <= 16)."""
+    if loader == 2 and chunk > 16:
+        pytest.skip("the register-prefetch loader takes chunks of 8 or 16")
+    W = 4096
+    cfg = _cfg(INPUT="FILTERBANK", GULP=W, DM_MAX=150, WIDTH_MAX=8)
+    h = hamd.Hella(cfg)
+    h.set_tile(lanes, chunk, order | (loader << 1))
+    i = h.info(0)
+    op = _oracle_plan(orc, cfg)
+    flagged = []
+    for g in range(i["min_gulp"] + 2):
+        gulp = _synth(hamd, cfg, W, seed=5, t_offset=g * W, pulses=[(1.5, 90.0, 1.0, 30.0)])
+        h.process_gulp(gulp)
+        flagged.append(_oracle_flag(orc, cfg, gulp))
+    F = np.concatenate(flagged, axis=1)
+    a0 = (i["min_gulp"] + 2) * W - i["ntime"]
+    D = h.dedisp_plane(0)
+    np.testing.assert_array_equal(D, orc.dedisperse_ft(F[:, a0:a0 + i["ntime"]], op.delta,
+                                                       i["ntime_dd"]))
