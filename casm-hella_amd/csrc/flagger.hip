@@ -22,7 +22,7 @@ constexpr int NT = 256;
 
 __device__ __forceinline__ uint8_t quant(float x)
 {
-  // transpose_output (src/hella/src/transpose.cu:428-438); NaN -> 0
+  // transpose_output (src/hella/src/transpose.cu:65-75); NaN -> 0
   float v = x + 4.f;
   v = v * (255.f / 14.f);
   if (v < 0.f) v = 0.f;

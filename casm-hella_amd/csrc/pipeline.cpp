@@ -443,7 +443,7 @@ void ctx_smooth(hella_ctx c, int b)
 }
 
 // append beam b's first candidates to the gulp list under the caps of
-// peaks.cu:295-296 (per beam) and peaks.cu:319-334 (per gulp)
+// peaks.cu:77-78 (per beam) and peaks.cu:101-104 (per gulp)
 static void append_peaks(hella_ctx c, int b, uint32_t ncand, const std::vector<Cand> &cs)
 {
   BeamState &B = c->beams[(size_t)b];

@@ -11,7 +11,7 @@
 //    row segment in LDS, so every width costs two LDS reads per sample and
 //    the sums stay exact integers (wrap-around u32 differences).
 //  * HELLA (reference kernel): the 3 x (2sm+1) polynomial-Gaussian kernel
-//    of smooth.cpp:136-149 applied as a direct FIR with replicate border.
+//    of smooth.cpp:45-58 applied as a direct FIR with replicate border.
 // Statistics follow calculate_stddev (normalization.cu:125-162) over the
 // first 256*floor(nout/256) samples with the canonical blocked reduction
 // (4096-sample blocks, 256 lanes, pairwise lane tree, ascending blocks).
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(NT) void k_finish(const SearchBeamDev *__restrict__
   if (threadIdx.x != 0) return;
   sc.mu0 = mu;
   sc.sig0 = sf;
-  // peaks.cu:278: search only when 0.1 < sigma < 5
+  // peaks.cu:61, definitions.h:69-72: search only when 0.1 < sigma < 5
   sc.gated = (sf < 5.f && sf > 0.1f) ? 0 : 1;
   sc.thr = thr;
 }
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(NT) void k_scan_down(const SearchBeamDev *__restric
   }
 }
 
-// hella smoothing (smooth.cpp:136-175): plane[k][r][s] =
+// hella smoothing (smooth.cpp:42-87): plane[k][r][s] =
 //   sum_{i<3} sum_{j<=2sm} K[i][j] * D[r+row0+i-1][min(s + maxW/2 + j, ntime_dd-1)]
 __global__ __launch_bounds__(NT) void k_hella_fir(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C, int k, int sm,
@@ -402,7 +402,7 @@ int search_nwidth(int mode, int minW, int maxW)
   return n;
 }
 
-// smooth.cpp:136-149 (the polynomial approximation of a squared Gaussian,
+// smooth.cpp:45-58 (the polynomial approximation of a squared Gaussian,
 // DM weights 0.3 / 1 / 0.3, normalised to unit sum)
 void hella_kernel_host(int sm, std::vector<float> &k)
 {

@@ -16,13 +16,13 @@
  *    every reduction is a double-precision canonical blocked sum (orc_csum),
  *    channel sums per sample (timeseries.cu:19-36) are sequential in channel
  *    order, in double.  mu_f = (float)mu, sigma_f = (float)sqrt(var).
- *  - Normalisation (flagger.cu:198-214, smooth.cpp:186-216) is
+ *  - Normalisation (flagger.cu:198-214, smooth.cpp:95-124) is
  *    y = (float)(((double)x - mu) * (1.0 / (double)sigma_f)).
  *  - NPP filters (convolution.cu:40, smooth.cpp:61) use the documented anchor
  *    convention: output (x,y) covers source columns x-ax .. x-ax+kw-1 and rows
  *    y-ay .. y-ay+kh-1, border replicate; taps accumulate row-major (kernel
  *    row outer, column inner) into a float starting at 0.
- *  - Quantise (transpose.cu:428-438) maps NaN to 0.
+ *  - Quantise (transpose.cu:65-75) maps NaN to 0.
  *  - Statistics use 256*floor(w/256) columns (normalization.cu:129); when
  *    w < 256 (where the reference divides by zero) all w columns are used.
  */
@@ -301,7 +301,7 @@ void orc_ts_correct(float *x, int nch, int W, float *ts)
   }
 }
 
-/* transpose_output quantiser (src/transpose.cu:428-438) */
+/* transpose_output quantiser (src/transpose.cu:65-75) */
 static inline uint8_t quant(float x)
 {
   float v = x + 4.f;
@@ -377,7 +377,7 @@ int orc_nwidth(const orc_search_cfg *cfg)
 
 int orc_width_of(const orc_search_cfg *cfg, int k) { return cfg->minW << k; }
 
-/* src/smooth.cpp:136-149 */
+/* src/smooth.cpp:45-58 */
 void orc_hella_kernel(int sm, float *k)
 {
   const float w[3] = {0.3f, 1.f, 0.3f};
@@ -458,7 +458,7 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
     } else {
       p.w = orc_width_of(cfg, k);
     }
-    /* per-width mean / variance (smooth.cpp:186-194 -> calculate_stddev) */
+    /* per-width mean / variance (smooth.cpp:95-124 -> calculate_stddev) */
     double *rawplane = (double *)malloc(sizeof(double) * plane);
     if (cfg->mode == ORC_MODE_BOXCAR) {
       /* exact integer box sums from a u64 prefix sum of each row */

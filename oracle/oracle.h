@@ -127,7 +127,7 @@ typedef struct {
 int orc_nwidth(const orc_search_cfg *cfg);
 /* width (boxcar) or smoothing scale sm (hella) of trial k */
 int orc_width_of(const orc_search_cfg *cfg, int k);
-/* hella smoothing kernel for scale sm: 3 x (2sm+1) floats (src/smooth.cpp:136-149) */
+/* hella smoothing kernel for scale sm: 3 x (2sm+1) floats (src/smooth.cpp:45-58) */
 void orc_hella_kernel(int sm, float *k);
 
 /* Full search of one beam's dedispersed plane D[ndm][ntime_dd] (row stride
