@@ -186,8 +186,7 @@ hella_status hella_dedisperse(hella_ctx c, int beam)
   if (!valid_beam(c, beam)) return fail(HELLA_ERR_INVALID, "bad beam");
   if (!c->built) return fail(HELLA_ERR_STATE, "no gulp flagged yet");
   const BeamState &B = c->beams[(size_t)beam];
-  if (!c->cfg.oneshot && c->gulp_index < B.min_gulp)
-    return fail(HELLA_ERR_STATE, "the beam's window is not full yet");
+  if (!beam_ready(c->cfg, B)) return fail(HELLA_ERR_STATE, "the beam's window is not full yet");
   return guard([&] { ctx_dedisperse(c, beam); });
 }
 
@@ -294,7 +293,8 @@ hella_status hella_copy_dedisp(hella_ctx c, int beam, void *host, size_t bytes)
   if (!valid_beam(c, beam) || !host) return fail(HELLA_ERR_INVALID, "bad argument");
   return guard([&] {
     const BeamState &B = c->beams[(size_t)beam];
-    const void *src = c->batch_mode ? B.D.p : (c->cur_beam == beam ? c->D.p : nullptr);
+    const void *src = c->batch_mode ? (c->ws.size() > (size_t)beam ? c->ws[(size_t)beam].D.p : nullptr)
+                                    : (c->cur_beam == beam ? c->ws[0].D.p : nullptr);
     if (!src) throw std::invalid_argument("no dedispersed plane for this beam");
     const size_t need = (size_t)B.ndm * B.ntime_dd * 4;
     if (bytes < need) throw std::invalid_argument("buffer too small");

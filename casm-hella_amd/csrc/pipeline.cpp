@@ -108,6 +108,202 @@ static void alloc_zero(DevBuf &b, size_t bytes, hipStream_t s)
   zero_u32(b.as<uint32_t>(), (int)(round_up(b.bytes, 4) / 4), s);
 }
 
+static int input_esize(const Config &cfg)
+{
+  return cfg.input_fmt == FLAG_IN_U8_TF ? 1 : (cfg.input_fmt == FLAG_IN_F32_TF ? 4 : 2);
+}
+
+void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s)
+{
+  const int maxW = cfg.width_max;
+  if (B.nch <= 0) throw std::runtime_error("NCHAN must be positive");
+  B.plan = PlanHost{};
+  B.plan.nchans = (size_t)B.nch;
+  B.plan.dt = (float)B.dt;
+  B.plan.f0 = (float)B.f0;
+  B.plan.df = (float)B.df;
+  if (!(B.plan.df < 0.f) || !(B.plan.dt > 0.f)) throw std::runtime_error("FOFF must be < 0, TSAMP > 0");
+  B.plan.build_delay_table();
+  B.plan.generate_dm_list(B.dm_lo, B.dm_hi, (double)cfg.pulse_width_us, (double)B.dm_tol);
+  if (B.killmask.empty()) B.killmask.assign((size_t)B.nch, 1);
+  B.plan.killmask = B.killmask;
+  B.plan.delays_live(0, B.plan.dm_list.size(), B.delta_host, B.live_host);
+  B.nlive = (int)B.live_host.size();
+  B.ndm = (int)B.plan.dm_list.size();
+  B.md = (int64_t)B.plan.max_delay;
+  if (cfg.oneshot) {
+    B.ntime = W;
+    B.ntime_dd = (int)(W - B.md);
+    B.nout = B.ntime_dd - maxW;
+    B.min_gulp = 0;
+  } else {
+    B.ntime = 32 * ((W + B.md + maxW) / 32 + 1);
+    B.ntime_dd = W + maxW;
+    B.nout = W;
+    B.min_gulp = (int)(B.ntime / W);
+  }
+  if (B.nout <= 0) throw std::runtime_error("gulp too short for the maximum DM delay");
+  B.rows = cfg.search_mode == SEARCH_HELLA ? B.ndm - 2 : B.ndm;
+  if (B.rows <= 0) throw std::runtime_error("too few DM trials for the search mode");
+  const int64_t ring_len = cfg.oneshot ? (int64_t)round_up((size_t)W, 16) : B.ntime;
+  B.ring_stride = (int64_t)round_up((size_t)ring_len, 64);
+  B.ntime = cfg.oneshot ? ring_len : B.ntime;
+  B.ring.ensure((size_t)B.nch * B.ring_stride);
+  alloc_zero(B.ring, (size_t)B.nch * B.ring_stride, s);
+  B.dstride = (int64_t)round_up((size_t)B.ntime_dd, 64);
+  B.in_bytes = (int64_t)W * B.nch * input_esize(cfg);
+  if (B.nlive > 0 && !B.delta) {
+    B.d_delta.ensure(B.delta_host.size() * 4);
+    B.d_live.ensure(B.live_host.size() * 4);
+    HAMD_CHECK(hipMemcpy(B.d_delta.p, B.delta_host.data(), B.delta_host.size() * 4,
+                         hipMemcpyHostToDevice));
+    HAMD_CHECK(hipMemcpy(B.d_live.p, B.live_host.data(), B.live_host.size() * 4,
+                         hipMemcpyHostToDevice));
+    B.delta = B.d_delta.as<int32_t>();
+    B.live = B.d_live.as<int32_t>();
+  }
+  B.f_bp.ensure((size_t)B.nch * 4);
+  alloc_zero(B.f_scal, sizeof(FScal), s);
+  alloc_zero(B.f_count, (size_t)B.nch * 4, s);
+  alloc_zero(B.s_scal, sizeof(SScal), s);
+  B.gidx = 0;
+  B.crossings = 0;
+}
+
+WorkDims beam_dims(const BeamState &B, const Config &cfg, int W, int nw)
+{
+  WorkDims d;
+  d.nch = B.nch;
+  d.W = W;
+  d.xs = (int64_t)round_up((size_t)W, 64);
+  d.ndm = B.ndm;
+  d.dstride = B.dstride;
+  d.rows = B.rows;
+  d.nout = B.nout;
+  d.nw = nw;
+  d.cand_cap = cfg.cand_cap;
+  d.mask = cfg.flag.nscrunch > 0;
+  d.planes = cfg.search_mode == SEARCH_HELLA;
+  return d;
+}
+
+void WorkDims::merge(const WorkDims &o)
+{
+  nch = std::max(nch, o.nch);
+  W = std::max(W, o.W);
+  ndm = std::max(ndm, o.ndm);
+  rows = std::max(rows, o.rows);
+  nout = std::max(nout, o.nout);
+  nw = std::max(nw, o.nw);
+  cand_cap = std::max(cand_cap, o.cand_cap);
+  xs = std::max(xs, o.xs);
+  dstride = std::max(dstride, o.dstride);
+  mask = mask || o.mask;
+  planes = planes || o.planes;
+}
+
+struct WorkSizes {
+  size_t X, mask, D, planes, fpart, fts, spart, scnt, stiles, scands;
+};
+
+static WorkSizes work_sizes(const WorkDims &d)
+{
+  WorkSizes z{};
+  const size_t nblk = (size_t)(d.W + FLAG_BLOCK - 1) / FLAG_BLOCK;
+  const size_t sblk = (size_t)(d.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
+  const size_t ncnt = (size_t)d.nw * d.rows * sblk;
+  z.X = (size_t)d.nch * d.xs * 4;
+  z.mask = d.mask ? (size_t)d.nch * d.xs : 0;
+  z.D = (size_t)d.ndm * d.dstride * 4;
+  z.planes = d.planes ? (size_t)d.nw * d.rows * round_up((size_t)d.nout, 64) * 4 : 0;
+  z.fpart = (size_t)d.nch * nblk * 8;
+  z.fts = (size_t)d.W * 4;
+  z.spart = ncnt * 8;
+  z.scnt = ncnt * 4;
+  z.stiles = ((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4;
+  z.scands = (size_t)d.cand_cap * sizeof(Cand);
+  return z;
+}
+
+size_t WorkSet::bytes_for(const WorkDims &d)
+{
+  const WorkSizes z = work_sizes(d);
+  return z.X + z.mask + z.D + z.planes + 2 * z.fpart + z.fts + 2 * z.spart + 2 * z.scnt +
+         z.stiles + z.scands + 16;
+}
+
+void WorkSet::ensure(const WorkDims &d)
+{
+  const WorkSizes z = work_sizes(d);
+  X.ensure(z.X);
+  if (z.mask) mask.ensure(z.mask);
+  D.ensure(z.D);
+  if (z.planes) planes.ensure(z.planes);
+  f_part.ensure(z.fpart);
+  f_part2.ensure(z.fpart);
+  f_ts.ensure(z.fts);
+  s_part.ensure(z.spart);
+  s_part2.ensure(z.spart);
+  s_counts.ensure(z.scnt);
+  s_offs.ensure(z.scnt);
+  s_tiles.ensure(z.stiles);
+  s_cands.ensure(z.scands);
+  s_ncand.ensure(16);
+}
+
+void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
+{
+  // measured on MI355X (1024 ch, DM 0-2000): 32 lanes per DM trial, 8-channel
+  // chunks staged through registers, time-major tile order; wider lane
+  // groups take over when the delay spread of 8 DM trials does not fit
+  DDTile best{64, 8, 0};
+  for (int G : {32, 64}) {
+    bool fits = true;
+    for (const BeamState *B : beams) {
+      if (B->nlive == 0) continue;
+      const int sp = dd_max_spread(B->delta_host.data(), B->ndm, B->nlive, 256 / G);
+      if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
+    }
+    if (fits) { best = DDTile{G, 8, 0}; break; }
+  }
+  tile = best;
+  order = 1;
+}
+
+void build_plain_tiles(BeamState &B, const DDTile &t, int order)
+{
+  B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(t)) : 0;
+  B.ntiles = 0;
+  if (B.nlive == 0) return;
+  DDBeamDev dd{};
+  dd.ndm = B.ndm;
+  dd.nout = B.ntime_dd;
+  dd.nlive = B.nlive;
+  std::vector<DDTileDesc> tiles;
+  dd_build_tiles(std::vector<DDBeamDev>{dd}, t, tiles, order);
+  B.d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
+  HAMD_CHECK(hipMemcpy(B.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
+                       hipMemcpyHostToDevice));
+  B.ntiles = (int)tiles.size();
+}
+
+SearchCommon search_common(const Config &cfg, const BeamState &B0, int nw)
+{
+  SearchCommon C{};
+  C.nout = B0.nout;
+  C.ntime_dd = B0.ntime_dd;
+  C.maxW = cfg.width_max;
+  C.minW = cfg.width_min;
+  C.nw = nw;
+  C.mode = cfg.search_mode;
+  C.snr = cfg.snr;
+  C.cap = cfg.cand_cap;
+  C.nblk = (C.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
+  C.wp = (C.nout / 256) * 256;
+  if (C.wp == 0) C.wp = C.nout;
+  return C;
+}
+
 void ctx_build(hella_ctx c)
 {
   if (c->built) return;
@@ -121,117 +317,33 @@ void ctx_build(hella_ctx c)
     throw std::runtime_error("WIDTH_MIN/WIDTH_MAX must be powers of two, maxW <= 1024");
   hipStream_t s = c->stream;
   int64_t in_off = 0;
-  const int esz = cfg.input_fmt == FLAG_IN_U8_TF ? 1 : (cfg.input_fmt == FLAG_IN_F32_TF ? 4 : 2);
   c->max_nch = 0;
   c->max_rows = 0;
+  c->wdims = WorkDims{};
   int ntdd = -1, nout = -1;
   for (auto &B : c->beams) {
-    if (B.nch <= 0) throw std::runtime_error("NCHAN must be positive");
-    B.plan = PlanHost{};
-    B.plan.nchans = (size_t)B.nch;
-    B.plan.dt = (float)B.dt;
-    B.plan.f0 = (float)B.f0;
-    B.plan.df = (float)B.df;
-    if (!(B.plan.df < 0.f) || !(B.plan.dt > 0.f)) throw std::runtime_error("FOFF must be < 0, TSAMP > 0");
-    B.plan.build_delay_table();
-    B.plan.generate_dm_list(B.dm_lo, B.dm_hi, (double)cfg.pulse_width_us, (double)B.dm_tol);
-    if (B.killmask.empty()) B.killmask.assign((size_t)B.nch, 1);
-    B.plan.killmask = B.killmask;
-    B.plan.delays_live(0, B.plan.dm_list.size(), B.delta_host, B.live_host);
-    B.nlive = (int)B.live_host.size();
-    B.ndm = (int)B.plan.dm_list.size();
-    B.md = (int64_t)B.plan.max_delay;
-    if (cfg.oneshot) {
-      B.ntime = W;
-      B.ntime_dd = (int)(W - B.md);
-      B.nout = B.ntime_dd - maxW;
-      B.min_gulp = 0;
-    } else {
-      B.ntime = 32 * ((W + B.md + maxW) / 32 + 1);
-      B.ntime_dd = W + maxW;
-      B.nout = W;
-      B.min_gulp = (int)(B.ntime / W);
-    }
-    if (B.nout <= 0) throw std::runtime_error("gulp too short for the maximum DM delay");
+    beam_setup(B, cfg, W, s);
     if (ntdd >= 0 && (ntdd != B.ntime_dd || nout != B.nout))
       throw std::runtime_error("one-shot sizing needs the same max delay in every beam");
     ntdd = B.ntime_dd;
     nout = B.nout;
-    B.rows = cfg.search_mode == SEARCH_HELLA ? B.ndm - 2 : B.ndm;
-    if (B.rows <= 0) throw std::runtime_error("too few DM trials for the search mode");
-    const int64_t ring_len = cfg.oneshot ? (int64_t)round_up((size_t)W, 16) : B.ntime;
-    B.ring_stride = (int64_t)round_up((size_t)ring_len, 64);
-    B.ntime = cfg.oneshot ? ring_len : B.ntime;
-    B.ring.ensure((size_t)B.nch * B.ring_stride);
-    alloc_zero(B.ring, (size_t)B.nch * B.ring_stride, s);
-    B.dstride = (int64_t)round_up((size_t)B.ntime_dd, 64);
     B.in_off = in_off;
-    B.in_bytes = (int64_t)W * B.nch * esz;
     in_off += B.in_bytes;
-    if (B.nlive > 0) {
-      B.d_delta.ensure(B.delta_host.size() * 4);
-      B.d_live.ensure(B.live_host.size() * 4);
-      HAMD_CHECK(hipMemcpy(B.d_delta.p, B.delta_host.data(), B.delta_host.size() * 4,
-                           hipMemcpyHostToDevice));
-      HAMD_CHECK(hipMemcpy(B.d_live.p, B.live_host.data(), B.live_host.size() * 4,
-                           hipMemcpyHostToDevice));
-    }
-    // flagger state
-    const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
-    B.f_part.ensure((size_t)B.nch * nblk * 8);
-    B.f_part2.ensure((size_t)B.nch * nblk * 8);
-    B.f_bp.ensure((size_t)B.nch * 4);
-    alloc_zero(B.f_scal, sizeof(FScal), s);
-    alloc_zero(B.f_count, (size_t)B.nch * 4, s);
-    B.f_ts.ensure((size_t)W * 4);
-    // search state
-    const int sblk = (B.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
-    const size_t ncnt = (size_t)c->nw * B.rows * sblk;
-    B.s_part.ensure(ncnt * 8);
-    B.s_part2.ensure(ncnt * 8);
-    alloc_zero(B.s_scal, sizeof(SScal), s);
-    B.s_counts.ensure(ncnt * 4);
-    B.s_offs.ensure(ncnt * 4);
-    B.s_tiles.ensure(((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4);
-    B.s_cands.ensure((size_t)cfg.cand_cap * sizeof(Cand));
-    alloc_zero(B.s_ncand, 4, s);
     c->max_nch = std::max(c->max_nch, B.nch);
     c->max_rows = std::max(c->max_rows, B.rows);
+    c->wdims.merge(beam_dims(B, cfg, W, c->nw));
   }
   c->gulp_bytes = in_off;
-  // tile shape shared by all beams: the preferred lane count whose LDS
-  // footprint fits every beam with the largest channel chunk
+  // tile shape shared by all beams
   if (!c->tile_forced) {
-    // measured on MI355X (1024 ch, DM 0-2000): 32 lanes per DM trial, 8-channel
-    // chunks staged through registers, time-major tile order; wider lane
-    // groups take over when the delay spread of 8 DM trials does not fit
-    DDTile best{64, 8, 0};
-    for (int G : {32, 64}) {
-      bool fits = true;
-      for (auto &B : c->beams) {
-        if (B.nlive == 0) continue;
-        const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
-        if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
-      }
-      if (fits) { best = DDTile{G, 8, 0}; break; }
-    }
-    c->tile = best;
-    c->tile_order = 1;
+    std::vector<BeamState *> bp;
+    for (auto &B : c->beams) bp.push_back(&B);
+    choose_default_tile(bp, c->tile, c->tile_order);
   }
-  for (auto &B : c->beams)
-    B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(c->tile)) : 0;
-  // shared work buffers (plain path)
-  c->xs = (int64_t)round_up((size_t)W, 64);
-  c->X.ensure((size_t)c->max_nch * c->xs * 4);
-  if (cfg.flag.nscrunch > 0) c->mask.ensure((size_t)c->max_nch * c->xs);
-  size_t dmax = 0, pmax = 0;
-  for (auto &B : c->beams) {
-    dmax = std::max(dmax, (size_t)B.ndm * B.dstride * 4);
-    pmax = std::max(pmax, (size_t)c->nw * B.rows * round_up((size_t)B.nout, 64) * 4);
-  }
-  c->D.ensure(dmax);
+  c->xs = c->wdims.xs;
+  c->ws.resize(1);
+  c->ws[0].ensure(c->wdims);
   if (cfg.search_mode == SEARCH_HELLA) {
-    c->planes.ensure(pmax);
     std::vector<float> all, k;
     c->kofs.clear();
     for (int i = 0; i < c->nw; ++i) {
@@ -252,122 +364,99 @@ void ctx_build(hella_ctx c)
   c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
   c->d_in.ensure((size_t)c->gulp_bytes);
-  // search constants
-  SearchCommon &C = c->common;
-  C.nout = c->beams[0].nout;
-  C.ntime_dd = c->beams[0].ntime_dd;
-  C.maxW = maxW;
-  C.minW = minW;
-  C.nw = c->nw;
-  C.mode = cfg.search_mode;
-  C.snr = cfg.snr;
-  C.cap = cfg.cand_cap;
-  C.nblk = (C.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
-  C.wp = (C.nout / 256) * 256;
-  if (C.wp == 0) C.wp = C.nout;
-  // per-beam tile tables of the plain path (beam index 0 of a one-entry array)
-  for (auto &B : c->beams) {
-    if (B.nlive == 0) continue;
-    DDBeamDev dd{};
-    dd.ndm = B.ndm;
-    dd.nout = B.ntime_dd;
-    dd.nlive = B.nlive;
-    std::vector<DDTileDesc> tiles;
-    dd_build_tiles(std::vector<DDBeamDev>{dd}, c->tile, tiles, c->tile_order);
-    B.d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
-    HAMD_CHECK(hipMemcpy(B.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
-                         hipMemcpyHostToDevice));
-    B.ntiles = (int)tiles.size();
-  }
+  c->common = search_common(cfg, c->beams[0], c->nw);
+  for (auto &B : c->beams) build_plain_tiles(B, c->tile, c->tile_order);
   HAMD_CHECK(hipStreamSynchronize(s));
   c->built = true;
 }
 
 // ------------------------------------------------------------------ descriptors
-static bool beam_ready(hella_ctx c, const BeamState &B)
+bool beam_ready(const Config &cfg, const BeamState &B)
 {
-  return c->cfg.oneshot || c->gulp_index >= B.min_gulp;
+  return cfg.oneshot || B.gidx >= B.min_gulp;
 }
 
-static int64_t write_col(hella_ctx c, const BeamState &B)
+// ring column of the beam's next ingested gulp
+static int64_t write_col(const Config &cfg, int W, const BeamState &B)
 {
-  if (c->cfg.oneshot) return 0;
-  return (int64_t)((c->gulp_index * (long long)c->W) % B.ntime);
+  if (cfg.oneshot) return 0;
+  return (int64_t)((B.gidx * (long long)W) % B.ntime);
 }
 
-static int64_t window_start(hella_ctx c, const BeamState &B)
+// ring column of the oldest sample of the window that ends with that gulp
+static int64_t window_start(const Config &cfg, int W, const BeamState &B)
 {
-  if (c->cfg.oneshot) return 0;
-  return (int64_t)(((c->gulp_index + 1) * (long long)c->W) % B.ntime);
+  if (cfg.oneshot) return 0;
+  return (int64_t)(((B.gidx + 1) * (long long)W) % B.ntime);
 }
 
-static long long window_abs0(hella_ctx c, const BeamState &B)
+long long window_abs0(const Config &cfg, int W, const BeamState &B)
 {
-  if (c->cfg.oneshot) return 0;
-  return (c->gulp_index + 1) * (long long)c->W - B.ntime;
+  if (cfg.oneshot) return 0;
+  return (B.gidx + 1) * (long long)W - B.ntime;
 }
 
-static FlagBeamDev make_fdesc(hella_ctx c, BeamState &B, bool own, const void *d_gulp)
+FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const void *d_in)
 {
   FlagBeamDev f{};
-  f.X = own ? B.X.as<float>() : c->X.as<float>();
-  f.xs = c->xs;
+  f.X = w.X.as<float>();
+  f.xs = (int64_t)round_up((size_t)W, 64);
   f.nch = B.nch;
-  f.nblk = (c->W + FLAG_BLOCK - 1) / FLAG_BLOCK;
-  f.in = static_cast<const uint8_t *>(d_gulp) + B.in_off;
-  f.in_fmt = c->cfg.input_fmt;
-  f.flag1 = c->cfg.flag1;
-  f.flag2 = c->cfg.flag2;
-  f.in_stride = c->cfg.input_fmt == FLAG_IN_F16_FT ? c->W : B.nch;
+  f.nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
+  f.in = d_in;
+  f.in_fmt = cfg.input_fmt;
+  f.flag1 = cfg.flag1;
+  f.flag2 = cfg.flag2;
+  f.in_stride = cfg.input_fmt == FLAG_IN_F16_FT ? W : B.nch;
   f.ring = B.ring.as<uint8_t>();
   f.ring_stride = B.ring_stride;
   f.ring_len = B.ntime;
-  f.col0 = write_col(c, B);
-  f.part = B.f_part.as<double>();
-  f.part2 = B.f_part2.as<double>();
+  f.col0 = write_col(cfg, W, B);
+  f.part = w.f_part.as<double>();
+  f.part2 = w.f_part2.as<double>();
   f.bp = B.f_bp.as<float>();
   f.scal = B.f_scal.as<FScal>();
   f.flagcount = B.f_count.as<uint32_t>();
-  f.ts = B.f_ts.as<float>();
-  f.mask = own ? B.mask.as<uint8_t>() : c->mask.as<uint8_t>();
+  f.ts = w.f_ts.as<float>();
+  f.mask = w.mask.as<uint8_t>();
   return f;
 }
 
-static DDBeamDev make_ddesc(hella_ctx c, BeamState &B, bool own)
+DDBeamDev make_ddesc(const Config &cfg, int W, BeamState &B, WorkSet &w)
 {
   DDBeamDev d{};
   d.base = B.ring.as<uint8_t>();
   d.row_stride = B.ring_stride;
   d.ring = B.ntime;
-  d.start = window_start(c, B);
-  d.delta = B.d_delta.as<int32_t>();
-  d.live = B.d_live.as<int32_t>();
+  d.start = window_start(cfg, W, B);
+  d.delta = B.delta;
+  d.live = B.live;
   d.nlive = B.nlive;
   d.ndm = B.ndm;
   d.nout = B.ntime_dd;
   d.out_f32 = 0;
-  d.out = own ? B.D.p : c->D.p;
+  d.out = w.D.p;
   d.out_stride = B.dstride;
   return d;
 }
 
-static SearchBeamDev make_sdesc(hella_ctx c, BeamState &B, bool own)
+SearchBeamDev make_sdesc(const Config &cfg, BeamState &B, WorkSet &w)
 {
   SearchBeamDev s{};
-  s.D = own ? B.D.as<uint32_t>() : c->D.as<uint32_t>();
+  s.D = w.D.as<uint32_t>();
   s.ds = B.dstride;
   s.rows = B.rows;
-  s.row0 = c->cfg.search_mode == SEARCH_HELLA ? 1 : 0;
-  s.planes = own ? B.planes.as<float>() : c->planes.as<float>();
+  s.row0 = cfg.search_mode == SEARCH_HELLA ? 1 : 0;
+  s.planes = w.planes.as<float>();
   s.ps = (int64_t)round_up((size_t)B.nout, 64);
-  s.part = B.s_part.as<double>();
-  s.part2 = B.s_part2.as<double>();
+  s.part = w.s_part.as<double>();
+  s.part2 = w.s_part2.as<double>();
   s.scal = B.s_scal.as<SScal>();
-  s.counts = B.s_counts.as<uint32_t>();
-  s.offs = B.s_offs.as<uint32_t>();
-  s.tiles = B.s_tiles.as<uint32_t>();
-  s.cands = B.s_cands.as<Cand>();
-  s.ncand = B.s_ncand.as<uint32_t>();
+  s.counts = w.s_counts.as<uint32_t>();
+  s.offs = w.s_offs.as<uint32_t>();
+  s.tiles = w.s_tiles.as<uint32_t>();
+  s.cands = w.s_cands.as<Cand>();
+  s.ncand = w.s_ncand.as<uint32_t>();
   return s;
 }
 
@@ -408,7 +497,8 @@ void ctx_flag_beam(hella_ctx c, int b, const void *d_gulp)
 {
   BeamState &B = c->beams[(size_t)b];
   StageTimer t(c, 0);
-  c->h_fdesc[b] = make_fdesc(c, B, false, d_gulp);
+  c->h_fdesc[b] = make_fdesc(c->cfg, c->W, B, c->ws[0],
+                             static_cast<const uint8_t *>(d_gulp) + B.in_off);
   FlagBeamDev *dd = c->d_fdesc.as<FlagBeamDev>() + b;
   HAMD_CHECK(hipMemcpyAsync(dd, &c->h_fdesc[b], sizeof(FlagBeamDev), hipMemcpyHostToDevice,
                             c->stream));
@@ -419,9 +509,9 @@ void ctx_dedisperse(hella_ctx c, int b)
 {
   BeamState &B = c->beams[(size_t)b];
   StageTimer t(c, 1);
-  c->h_ddesc[b] = make_ddesc(c, B, false);
+  c->h_ddesc[b] = make_ddesc(c->cfg, c->W, B, c->ws[0]);
   if (B.nlive == 0) {
-    fill_zero_rows(c->D.p, B.dstride, B.ntime_dd, B.ndm, c->stream);
+    fill_zero_rows(c->ws[0].D.p, B.dstride, B.ntime_dd, B.ndm, c->stream);
   } else {
     DDBeamDev *dd = c->d_ddesc.as<DDBeamDev>() + b;
     HAMD_CHECK(hipMemcpyAsync(dd, &c->h_ddesc[b], sizeof(DDBeamDev), hipMemcpyHostToDevice,
@@ -435,64 +525,70 @@ void ctx_smooth(hella_ctx c, int b)
 {
   BeamState &B = c->beams[(size_t)b];
   StageTimer t(c, 2);
-  c->h_sdesc[b] = make_sdesc(c, B, false);
+  c->h_sdesc[b] = make_sdesc(c->cfg, B, c->ws[0]);
   SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
   HAMD_CHECK(hipMemcpyAsync(sd, &c->h_sdesc[b], sizeof(SearchBeamDev), hipMemcpyHostToDevice,
                             c->stream));
   SearchRunner::smooth(sd, 1, B.rows, c->common, c->d_kern.as<float>(), c->kofs, c->stream);
 }
 
-// append beam b's first candidates to the gulp list under the caps of
-// peaks.cu:77-78 (per beam) and peaks.cu:101-104 (per gulp)
-static void append_peaks(hella_ctx c, int b, uint32_t ncand, const std::vector<Cand> &cs)
+// the first candidates of a beam's search under the caps of peaks.cu:77-78
+// (per beam) and peaks.cu:101-104 (per gulp, applied by the caller)
+static size_t beam_take(const Config &cfg, uint32_t ncand)
 {
-  BeamState &B = c->beams[(size_t)b];
-  B.crossings = ncand;
-  const long long a0 = window_abs0(c, B) + c->cfg.width_max / 2;
+  size_t n = std::min<size_t>(ncand, (size_t)cfg.cand_cap);
+  return std::min<size_t>(n, (size_t)cfg.max_giants);
+}
+
+// append beam b's candidates to a gulp list holding at most cap entries
+void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id,
+                       const std::vector<Cand> &cs, size_t cap, std::vector<hella_candidate> &out)
+{
+  const long long a0 = window_abs0(cfg, W, B) + cfg.width_max / 2;
   for (const Cand &x : cs) {
-    if ((int)c->peaks.size() >= c->cfg.max_giants) break;
+    if (out.size() >= cap) break;
     hella_candidate h;
     h.snr = x.snr;
-    h.beam = b;
+    h.beam = beam_id;
     h.samp = x.samp;
     h.width = x.width;
     h.dm_idx = x.dm_idx;
     h.dm = B.plan.dm_list[(size_t)x.dm_idx];
     h.abs_samp = a0 + x.samp;
-    c->peaks.push_back(h);
+    out.push_back(h);
   }
-}
-
-static size_t beam_take(hella_ctx c, uint32_t ncand)
-{
-  size_t n = std::min<size_t>(ncand, (size_t)c->cfg.cand_cap);
-  return std::min<size_t>(n, (size_t)c->cfg.max_giants);
 }
 
 int ctx_find_peaks(hella_ctx c, int b)
 {
   BeamState &B = c->beams[(size_t)b];
+  WorkSet &w = c->ws[0];
   uint32_t nc = 0;
   std::vector<Cand> cs;
   {
     StageTimer t(c, 3);
     SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
     SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
-    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[b], B.s_ncand.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[b], w.s_ncand.p, 4, hipMemcpyDeviceToHost, c->stream));
     HAMD_CHECK(hipStreamSynchronize(c->stream));
     nc = c->h_ncand[b];
-    cs.resize(beam_take(c, nc));
+    cs.resize(beam_take(c->cfg, nc));
     if (!cs.empty()) {
-      HAMD_CHECK(hipMemcpyAsync(cs.data(), B.s_cands.p, cs.size() * sizeof(Cand),
+      HAMD_CHECK(hipMemcpyAsync(cs.data(), w.s_cands.p, cs.size() * sizeof(Cand),
                                 hipMemcpyDeviceToHost, c->stream));
       HAMD_CHECK(hipStreamSynchronize(c->stream));
     }
   }
-  append_peaks(c, b, nc, cs);
+  B.crossings = nc;
+  append_candidates(c->cfg, c->W, B, b, cs, (size_t)c->cfg.max_giants, c->peaks);
   return (int)cs.size();
 }
 
-void ctx_end_gulp(hella_ctx c) { c->gulp_index += 1; }
+void ctx_end_gulp(hella_ctx c)
+{
+  c->gulp_index += 1;
+  for (auto &B : c->beams) B.gidx += 1;
+}
 
 void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr)
 {
@@ -505,7 +601,7 @@ void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr)
   for (size_t b = 0; b < c->beams.size(); ++b) ctx_flag_beam(c, (int)b, d_gulp);
   for (size_t b = 0; b < c->beams.size(); ++b) {
     BeamState &B = c->beams[b];
-    if (!beam_ready(c, B)) continue;
+    if (!beam_ready(c->cfg, B)) continue;
     if ((int)c->peaks.size() >= c->cfg.max_giants) break;
     const double el =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -520,15 +616,12 @@ void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr)
 }
 
 // ------------------------------------------------------------------ batched path
+// work set b belongs to beam b
 static void batch_alloc(hella_ctx c)
 {
-  for (auto &B : c->beams) {
-    B.X.ensure((size_t)B.nch * c->xs * 4);
-    if (c->cfg.flag.nscrunch > 0) B.mask.ensure((size_t)B.nch * c->xs);
-    B.D.ensure((size_t)B.ndm * B.dstride * 4);
-    if (c->cfg.search_mode == SEARCH_HELLA)
-      B.planes.ensure((size_t)c->nw * B.rows * round_up((size_t)B.nout, 64) * 4);
-  }
+  if (c->ws.size() < c->beams.size()) c->ws.resize(c->beams.size());
+  for (size_t b = 0; b < c->beams.size(); ++b)
+    c->ws[b].ensure(beam_dims(c->beams[b], c->cfg, c->W, c->nw));
 }
 
 static void batch_destroy_graph(Batch &bt)
@@ -543,7 +636,7 @@ static void batch_prepare(hella_ctx c)
 {
   std::vector<int> ready;
   for (size_t b = 0; b < c->beams.size(); ++b)
-    if (beam_ready(c, c->beams[b])) ready.push_back((int)b);
+    if (beam_ready(c->cfg, c->beams[b])) ready.push_back((int)b);
   Batch &bt = c->batch;
   if (bt.valid && ready == bt.beams) return;
   batch_destroy_graph(bt);
@@ -551,7 +644,7 @@ static void batch_prepare(hella_ctx c)
   std::vector<DDBeamDev> dd;
   int spread = 0;
   for (int b : ready) {
-    dd.push_back(make_ddesc(c, c->beams[(size_t)b], true));
+    dd.push_back(make_ddesc(c->cfg, c->W, c->beams[(size_t)b], c->ws[(size_t)b]));
     spread = std::max(spread, c->beams[(size_t)b].spread);
   }
   std::vector<DDTileDesc> tiles;
@@ -582,8 +675,9 @@ static void batch_record(hella_ctx c)
   HAMD_CHECK(hipMemcpyAsync(c->d_sdesc.p, c->h_sdesc, (size_t)nr * sizeof(SearchBeamDev),
                             hipMemcpyHostToDevice, s));
   for (int i = 0; i < nr; ++i) {
-    const BeamState &B = c->beams[(size_t)bt.beams[(size_t)i]];
-    if (B.nlive == 0) fill_zero_rows(B.D.p, B.dstride, B.ntime_dd, B.ndm, s);
+    const int b = bt.beams[(size_t)i];
+    const BeamState &B = c->beams[(size_t)b];
+    if (B.nlive == 0) fill_zero_rows(c->ws[(size_t)b].D.p, B.dstride, B.ntime_dd, B.ndm, s);
   }
   int64_t min_ring = INT64_MAX;
   for (int b : bt.beams) min_ring = std::min(min_ring, c->beams[(size_t)b].ntime);
@@ -594,8 +688,9 @@ static void batch_record(hella_ctx c)
   SearchRunner::run(c->d_sdesc.as<SearchBeamDev>(), nr, max_rows, c->common,
                     c->d_kern.as<float>(), c->kofs, s);
   for (int i = 0; i < nr; ++i) {
-    const BeamState &B = c->beams[(size_t)bt.beams[(size_t)i]];
-    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[i], B.s_ncand.p, 4, hipMemcpyDeviceToHost, s));
+    const int b = bt.beams[(size_t)i];
+    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[i], c->ws[(size_t)b].s_ncand.p, 4,
+                              hipMemcpyDeviceToHost, s));
   }
 }
 
@@ -611,11 +706,12 @@ void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
   const void *d_gulp = ctx_stage_input(c, gulp, host_ptr);
   // refresh the per-gulp descriptor values (read by the graph's copy nodes)
   for (size_t b = 0; b < c->beams.size(); ++b)
-    c->h_fdesc[b] = make_fdesc(c, c->beams[b], true, d_gulp);
+    c->h_fdesc[b] = make_fdesc(c->cfg, c->W, c->beams[b], c->ws[b],
+                               static_cast<const uint8_t *>(d_gulp) + c->beams[b].in_off);
   for (size_t i = 0; i < bt.beams.size(); ++i) {
-    BeamState &B = c->beams[(size_t)bt.beams[i]];
-    c->h_ddesc[i] = make_ddesc(c, B, true);
-    c->h_sdesc[i] = make_sdesc(c, B, true);
+    const size_t b = (size_t)bt.beams[i];
+    c->h_ddesc[i] = make_ddesc(c->cfg, c->W, c->beams[b], c->ws[b]);
+    c->h_sdesc[i] = make_sdesc(c->cfg, c->beams[b], c->ws[b]);
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
@@ -651,16 +747,18 @@ void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
   // candidates: only beams with crossings are copied back
   std::vector<std::vector<Cand>> per(bt.beams.size());
   for (size_t i = 0; i < bt.beams.size(); ++i) {
-    const size_t n = beam_take(c, c->h_ncand[i]);
+    const size_t n = beam_take(c->cfg, c->h_ncand[i]);
     if (!n) continue;
     per[i].resize(n);
-    HAMD_CHECK(hipMemcpyAsync(per[i].data(), c->beams[(size_t)bt.beams[i]].s_cands.p,
+    HAMD_CHECK(hipMemcpyAsync(per[i].data(), c->ws[(size_t)bt.beams[i]].s_cands.p,
                               n * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
   }
   HAMD_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < bt.beams.size(); ++i) {
     if ((int)c->peaks.size() >= c->cfg.max_giants) break;
-    append_peaks(c, bt.beams[i], c->h_ncand[i], per[i]);
+    BeamState &B = c->beams[(size_t)bt.beams[i]];
+    B.crossings = c->h_ncand[i];
+    append_candidates(c->cfg, c->W, B, bt.beams[i], per[i], (size_t)c->cfg.max_giants, c->peaks);
   }
   ctx_cluster(c);
   ctx_end_gulp(c);
@@ -670,17 +768,18 @@ void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
 // Cells of CLUSTER_DT samples x CLUSTER_DDM DM trials per beam; 8-connected
 // occupied cells form one cluster, represented by its highest-S/N member
 // (lowest list index on ties); clusters ordered by representative index.
-void ctx_cluster(hella_ctx c)
+void cluster_candidates(const std::vector<hella_candidate> &peaks, int ct_, int cd_,
+                        std::vector<hella_cluster> &clusters)
 {
-  c->clusters.clear();
-  const int n = (int)c->peaks.size();
+  clusters.clear();
+  const int n = (int)peaks.size();
   if (!n) return;
-  const int ct = std::max(1, c->cfg.cluster_dt), cd = std::max(1, c->cfg.cluster_ddm);
+  const int ct = std::max(1, ct_), cd = std::max(1, cd_);
   struct KI { int64_t beam; int64_t key; int idx; };
   auto key = [&](int64_t cdm, int64_t cts) { return (cdm << 32) | (int64_t)(uint32_t)cts; };
   std::vector<KI> v((size_t)n);
   for (int i = 0; i < n; ++i) {
-    const hella_candidate &h = c->peaks[(size_t)i];
+    const hella_candidate &h = peaks[(size_t)i];
     v[(size_t)i] = KI{h.beam, key(h.dm_idx / cd, h.samp / ct), i};
   }
   std::sort(v.begin(), v.end(), [](const KI &a, const KI &b) {
@@ -723,7 +822,7 @@ void ctx_cluster(hella_ctx c)
   for (int i = 0; i < n; ++i) {
     const int r = find(cell_of[(size_t)i]);
     const int cur = rep[(size_t)r];
-    if (cur < 0 || c->peaks[(size_t)i].snr > c->peaks[(size_t)cur].snr) rep[(size_t)r] = i;
+    if (cur < 0 || peaks[(size_t)i].snr > peaks[(size_t)cur].snr) rep[(size_t)r] = i;
   }
   std::vector<int> reps;
   for (int i = 0; i < nc; ++i)
@@ -736,18 +835,23 @@ void ctx_cluster(hella_ctx c)
     h.rep = reps[q];
     h.samp_min = h.dm_min = 0x7fffffff;
     h.samp_max = h.dm_max = -1;
-    h.beam = c->peaks[(size_t)reps[q]].beam;
-    c->clusters.push_back(h);
+    h.beam = peaks[(size_t)reps[q]].beam;
+    clusters.push_back(h);
   }
   for (int i = 0; i < n; ++i) {
-    hella_cluster &h = c->clusters[(size_t)slot[(size_t)find(cell_of[(size_t)i])]];
-    const hella_candidate &p = c->peaks[(size_t)i];
+    hella_cluster &h = clusters[(size_t)slot[(size_t)find(cell_of[(size_t)i])]];
+    const hella_candidate &p = peaks[(size_t)i];
     h.count++;
     h.samp_min = std::min(h.samp_min, p.samp);
     h.samp_max = std::max(h.samp_max, p.samp);
     h.dm_min = std::min(h.dm_min, p.dm_idx);
     h.dm_max = std::max(h.dm_max, p.dm_idx);
   }
+}
+
+void ctx_cluster(hella_ctx c)
+{
+  cluster_candidates(c->peaks, c->cfg.cluster_dt, c->cfg.cluster_ddm, c->clusters);
 }
 
 // ------------------------------------------------------------------ stage timing
@@ -790,11 +894,13 @@ void ctx_autotune(hella_ctx c, const char *path)
   // time the dedispersion of every beam's current window with each shape
   std::vector<DDBeamDev> dd;
   int64_t min_ring = INT64_MAX;
-  for (auto &B : c->beams)
+  for (size_t b = 0; b < c->beams.size(); ++b) {
+    BeamState &B = c->beams[b];
     if (B.nlive > 0) {
-      dd.push_back(make_ddesc(c, B, true));
+      dd.push_back(make_ddesc(c->cfg, c->W, B, c->ws[b]));
       min_ring = std::min(min_ring, B.ntime);
     }
+  }
   if (dd.empty()) return;
   DevBuf d_beams;
   d_beams.ensure(dd.size() * sizeof(DDBeamDev));
@@ -847,20 +953,7 @@ void ctx_autotune(hella_ctx c, const char *path)
   c->tile = DDTile{best.G, best.cc, best.dma};
   c->tile_order = best.order;
   c->tile_forced = true;
-  for (auto &B : c->beams) {
-    B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(c->tile)) : 0;
-    if (!B.nlive) continue;
-    DDBeamDev one{};
-    one.ndm = B.ndm;
-    one.nout = B.ntime_dd;
-    one.nlive = B.nlive;
-    std::vector<DDTileDesc> tiles;
-    dd_build_tiles(std::vector<DDBeamDev>{one}, c->tile, tiles, c->tile_order);
-    B.d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
-    HAMD_CHECK(hipMemcpy(B.d_tiles.p, tiles.data(), tiles.size() * sizeof(DDTileDesc),
-                         hipMemcpyHostToDevice));
-    B.ntiles = (int)tiles.size();
-  }
+  for (auto &B : c->beams) build_plain_tiles(B, c->tile, c->tile_order);
   c->batch.valid = false;
   batch_destroy_graph(c->batch);
   if (path && *path) {

@@ -34,6 +34,9 @@ struct Config {
   bool parse(const std::string &text, std::string &err, bool beam_only = false);
 };
 
+// Resident state of one beam: plan, delays, circular window and the
+// flagger's running per-channel state.  Everything a single pass through the
+// chain needs beyond this lives in a WorkSet.
 struct BeamState {
   int id = 0;
   int nch = 0;
@@ -43,27 +46,44 @@ struct BeamState {
   std::vector<int> killmask;
   std::vector<int32_t> delta_host, live_host;
   DevBuf d_delta, d_live;
+  const int32_t *delta = nullptr, *live = nullptr;  // device tables (own or shared)
   int nlive = 0, ndm = 0;
   int64_t md = 0;
   int64_t ntime = 0;        // ring columns (NTIME)
   int64_t ring_stride = 0;  // bytes between channel rows
   int ntime_dd = 0, nout = 0, rows = 0;
   int min_gulp = 0;
+  int64_t dstride = 0;      // row stride of the dedispersed plane (u32)
   int64_t in_off = 0;       // byte offset of this beam in a gulp buffer
   int64_t in_bytes = 0;
+  long long gidx = 0;       // gulps ingested by this beam
   DevBuf ring;
-  // flagger state
-  DevBuf f_part, f_part2, f_bp, f_scal, f_count, f_ts;
-  // search state
-  DevBuf s_scal, s_part, s_part2, s_counts, s_offs, s_tiles, s_cands, s_ncand;
-  // work planes owned by the beam (batched path)
-  DevBuf X, mask, D, planes;
-  int64_t dstride = 0;
-  // per-beam tile table for the plain path
+  DevBuf f_bp, f_scal, f_count;  // bandpass, flagger scalars, grown-mask counts
+  DevBuf s_scal;                 // statistics of the beam's last search
+  // per-beam tile table of the plain path
   DevBuf d_tiles;
   int ntiles = 0;
   int spread = 0;
   long long crossings = 0;
+};
+
+// Sizes a WorkSet must cover (maximum over the beams that use it).
+struct WorkDims {
+  int nch = 0, W = 0, ndm = 0, rows = 0, nout = 0, nw = 0, cand_cap = 0;
+  int64_t xs = 0, dstride = 0;
+  bool mask = false, planes = false;
+  void merge(const WorkDims &o);
+};
+
+// Scratch of one beam's pass through the chain: flagger plane and mask,
+// dedispersed plane, width planes (hella mode), block partial sums, counts
+// and the candidate list.
+struct WorkSet {
+  DevBuf X, mask, D, planes;
+  DevBuf f_part, f_part2, f_ts;
+  DevBuf s_part, s_part2, s_counts, s_offs, s_tiles, s_cands, s_ncand;
+  void ensure(const WorkDims &d);
+  static size_t bytes_for(const WorkDims &d);
 };
 
 struct Batch {
@@ -87,8 +107,9 @@ struct hella_ctx_s {
   long long gulp_index = 0;
   int W = 0, nw = 0, max_nch = 0, max_rows = 0;
   int64_t gulp_bytes = 0;
-  // shared work buffers (plain path)
-  hamd::DevBuf X, mask, D, planes;
+  // work sets: [0] serves the plain path; the batched path gives beam b set b
+  std::vector<hamd::WorkSet> ws;
+  hamd::WorkDims wdims;
   int64_t xs = 0;
   // descriptor arrays: pinned host copies, device copies
   hamd::FlagBeamDev *h_fdesc = nullptr;
@@ -104,7 +125,7 @@ struct hella_ctx_s {
   hamd::DDTile tile{16, 32, 1};
   int tile_order = 0;
   bool tile_forced = false;
-  int cur_beam = -1;           // beam whose plane is in the shared D
+  int cur_beam = -1;           // beam whose plane is in work set 0
   std::vector<hella_candidate> peaks;
   std::vector<hella_cluster> clusters;
   bool timing = false;
@@ -114,6 +135,28 @@ struct hella_ctx_s {
 };
 
 namespace hamd {
+// plan, delays, window sizing and resident buffers of one beam
+void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s);
+WorkDims beam_dims(const BeamState &B, const Config &cfg, int W, int nw);
+// device descriptors of one beam's pass: d_in is the beam's gulp on the device
+FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const void *d_in);
+DDBeamDev make_ddesc(const Config &cfg, int W, BeamState &B, WorkSet &w);
+SearchBeamDev make_sdesc(const Config &cfg, BeamState &B, WorkSet &w);
+// absolute sample of output sample 0 of the beam's current window
+long long window_abs0(const Config &cfg, int W, const BeamState &B);
+bool beam_ready(const Config &cfg, const BeamState &B);
+SearchCommon search_common(const Config &cfg, const BeamState &B0, int nw);
+// append a beam's search results (first entries of its ordered list) to a
+// candidate list holding at most cap entries
+void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id,
+                       const std::vector<Cand> &cs, size_t cap, std::vector<hella_candidate> &out);
+// clusters of a candidate list (8-connected cells of ct samples x cd trials,
+// per beam), ordered by representative index
+void cluster_candidates(const std::vector<hella_candidate> &peaks, int ct, int cd,
+                        std::vector<hella_cluster> &out);
+// default dedispersion tile shape for a set of beams
+void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order);
+void build_plain_tiles(BeamState &B, const DDTile &t, int order);
 void ctx_build(hella_ctx c);
 void ctx_flag_beam(hella_ctx c, int b, const void *d_gulp_base);
 void ctx_dedisperse(hella_ctx c, int b);

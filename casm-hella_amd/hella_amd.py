@@ -186,6 +186,35 @@ class SynthParams(C.Structure):
                 ("burst_amp", _F), ("bp_slope", _F)]
 
 
+class _RingBlock(C.Structure):
+    _fields_ = [("index", _I), ("beam", _I), ("seq", C.c_longlong), ("flags", _I),
+                ("bytes", C.c_size_t), ("t_arrival_ns", C.c_longlong), ("data", _VP)]
+
+
+class _ReplayEvent(C.Structure):
+    _fields_ = [("t", C.c_double), ("beam", _I), ("pool", _I), ("seq", C.c_longlong),
+                ("flags", _I)]
+
+
+class _ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_ulonglong), ("max_batch", _I), ("lanes", _I), ("plain", _I)]
+
+
+class _ChunkResult(C.Structure):
+    _fields_ = [("beam", _I), ("seq", C.c_longlong), ("searched", _I), ("ncand", _I),
+                ("crossings", C.c_longlong), ("nclusters", _I), ("t_arrival_ns", C.c_longlong),
+                ("t_done_ns", C.c_longlong), ("batch", _I)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("chunks", C.c_longlong), ("batches", C.c_longlong), ("beams_admitted", _I),
+                ("beams_waited", _I), ("max_resident", _I), ("work_bytes", C.c_ulonglong),
+                ("peak_resident_bytes", C.c_ulonglong), ("busy_s", C.c_double),
+                ("wall_s", C.c_double)]
+
+
+RING_LAST, RING_END = 1, 2
+
 CAND_DTYPE = np.dtype([("snr", "f4"), ("beam", "i4"), ("samp", "i4"), ("width", "i4"),
                        ("dm_idx", "i4"), ("dm", "f4"), ("abs_samp", "i8")])
 CLUSTER_DTYPE = np.dtype([("rep", "i4"), ("count", "i4"), ("samp_min", "i4"), ("samp_max", "i4"),
@@ -233,6 +262,27 @@ def _declare_hella(L):
     d("hella_memcpy", _I, P, P, C.c_size_t, _I)
     d("hella_device_sync", _I)
     d("hella_synth_filterbank", _I, C.POINTER(SynthParams), P)
+    d("hella_ring_create", _I, _I, C.c_size_t, C.c_char_p, C.POINTER(P))
+    d("hella_ring_destroy", None, P)
+    d("hella_ring_header", C.c_char_p, P)
+    d("hella_ring_open_write", _I, P, _I, C.POINTER(P))
+    d("hella_ring_close_write", _I, P, _I, C.c_longlong, _I, C.c_size_t)
+    d("hella_ring_end", _I, P)
+    d("hella_ring_abort", _I, P)
+    d("hella_ring_open_read", _I, P, _I, C.POINTER(_RingBlock))
+    d("hella_ring_release", _I, P, _I)
+    d("hella_ring_replay", _I, P, C.POINTER(_ReplayEvent), _I, C.POINTER(P), _I, C.c_size_t,
+      C.c_double, C.POINTER(C.c_longlong))
+    d("hella_server_create", _I, C.c_char_p, C.POINTER(_ServerOpts), C.POINTER(P))
+    d("hella_server_destroy", None, P)
+    d("hella_server_run", _I, P, P)
+    d("hella_server_result_count", _I, P)
+    d("hella_server_get_result", _I, P, _I, C.POINTER(_ChunkResult))
+    d("hella_server_get_candidates", _I, P, _I, C.POINTER(_Cand), _I)
+    d("hella_server_get_clusters", _I, P, _I, C.POINTER(_Cluster), _I)
+    d("hella_server_get_stats", _I, P, C.POINTER(_ServerStats))
+    d("hella_server_footprint", _I, C.c_char_p, C.POINTER(_ServerOpts),
+      C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
 
 
 _orig_declare = _declare
@@ -456,3 +506,151 @@ def synth(nchan, nsamp, fch1, foff, tsamp, seed=1, t_offset=0, dtype="u8", mean=
     out = np.zeros((nsamp, nchan), np.uint8 if dtype == "u8" else np.float32)
     _hchk(lib().hella_synth_filterbank(C.byref(prm), out.ctypes.data))
     return out
+
+
+# ---------------------------------------------------------------- streaming
+class Ring:
+    """PSRDADA-style block ring in pinned host memory (hella_ring_*)."""
+
+    def __init__(self, nblocks, block_bytes, header=""):
+        self._r = _VP()
+        self.block_bytes = int(block_bytes)
+        _hchk(lib().hella_ring_create(nblocks, self.block_bytes, header.encode(), C.byref(self._r)))
+
+    @property
+    def handle(self):
+        return self._r
+
+    def header(self):
+        return lib().hella_ring_header(self._r).decode()
+
+    def write(self, beam, seq, data, flags=0, timeout_ms=-1):
+        a = np.ascontiguousarray(data)
+        if a.nbytes > self.block_bytes:
+            raise ValueError("chunk larger than a ring block")
+        blk = _VP()
+        _hchk(lib().hella_ring_open_write(self._r, timeout_ms, C.byref(blk)))
+        C.memmove(blk.value, a.ctypes.data, a.nbytes)
+        _hchk(lib().hella_ring_close_write(self._r, beam, seq, flags, a.nbytes))
+
+    def end(self):
+        _hchk(lib().hella_ring_end(self._r))
+
+    def read(self, timeout_ms=0):
+        """next block as a dict (data copied), or None; {'end': True} at the end"""
+        b = _RingBlock()
+        _hchk(lib().hella_ring_open_read(self._r, timeout_ms, C.byref(b)))
+        if b.index < 0:
+            return {"end": True} if b.flags & RING_END else None
+        data = np.ctypeslib.as_array((C.c_uint8 * b.bytes).from_address(b.data)).copy()
+        return {"index": b.index, "beam": b.beam, "seq": b.seq, "flags": b.flags,
+                "t_arrival_ns": b.t_arrival_ns, "data": data}
+
+    def release(self, index):
+        _hchk(lib().hella_ring_release(self._r, index))
+
+    def replay(self, events, pool, time_scale=1.0):
+        """events: iterable of (t, beam, pool_index, seq, flags) sorted by t; pool:
+        list of arrays of one chunk each.  Blocks until the stream has been written
+        and returns the largest delay behind schedule in seconds."""
+        ev = list(events)
+        arr = (_ReplayEvent * max(1, len(ev)))()
+        for i, (t, beam, pi, seq, flags) in enumerate(ev):
+            arr[i] = _ReplayEvent(float(t), int(beam), int(pi), int(seq), int(flags))
+        keep = [np.ascontiguousarray(p) for p in pool]
+        ptrs = (_VP * len(keep))(*[k.ctypes.data for k in keep])
+        late = C.c_longlong()
+        _hchk(lib().hella_ring_replay(self._r, arr, len(ev), ptrs, len(keep), keep[0].nbytes,
+                                      time_scale, C.byref(late)))
+        return late.value * 1e-9
+
+    def close(self):
+        if self._r:
+            lib().hella_ring_destroy(self._r)
+            self._r = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Streaming server with continuous batching (hella_server_*)."""
+
+    def __init__(self, cfg, hbm_budget=0, max_batch=16, lanes=2, plain=False):
+        self._s = _VP()
+        self.opts = _ServerOpts(int(hbm_budget), int(max_batch), int(lanes), 1 if plain else 0)
+        _hchk(lib().hella_server_create(config_text(cfg).encode(), C.byref(self.opts),
+                                        C.byref(self._s)))
+
+    @staticmethod
+    def footprint(cfg, max_batch=16, lanes=2, plain=False):
+        """(bytes per resident beam, bytes of the work sets)"""
+        o = _ServerOpts(0, int(max_batch), int(lanes), 1 if plain else 0)
+        bb, wb = C.c_ulonglong(), C.c_ulonglong()
+        _hchk(lib().hella_server_footprint(config_text(cfg).encode(), C.byref(o), C.byref(bb),
+                                           C.byref(wb)))
+        return bb.value, wb.value
+
+    def run(self, ring):
+        _hchk(lib().hella_server_run(self._s, ring.handle))
+
+    def results(self, with_candidates=True):
+        out = []
+        r = _ChunkResult()
+        for i in range(lib().hella_server_result_count(self._s)):
+            _hchk(lib().hella_server_get_result(self._s, i, C.byref(r)))
+            d = {f: getattr(r, f) for f, _ in _ChunkResult._fields_}
+            if with_candidates:
+                n = lib().hella_server_get_candidates(self._s, i, None, 0)
+                c = np.zeros(n, CAND_DTYPE)
+                if n:
+                    lib().hella_server_get_candidates(self._s, i, c.ctypes.data_as(C.POINTER(_Cand)), n)
+                m = lib().hella_server_get_clusters(self._s, i, None, 0)
+                k = np.zeros(m, CLUSTER_DTYPE)
+                if m:
+                    lib().hella_server_get_clusters(self._s, i, k.ctypes.data_as(C.POINTER(_Cluster)), m)
+                d["cands"], d["clusters"] = c, k
+            out.append(d)
+        return out
+
+    def stats(self):
+        st = _ServerStats()
+        _hchk(lib().hella_server_get_stats(self._s, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self._s:
+            lib().hella_server_destroy(self._s)
+            self._s = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve(server, ring, events, pool, time_scale=1.0):
+    """Replay events into the ring from a writer thread while the server runs
+    in this thread; returns the writer's largest delay behind schedule (s)."""
+    import threading
+    res = {}
+
+    def writer():
+        try:
+            res["late"] = ring.replay(events, pool, time_scale)
+        except Exception as e:  # the server aborts the ring on error
+            res["error"] = e
+
+    th = threading.Thread(target=writer, daemon=True)
+    th.start()
+    try:
+        server.run(ring)
+    finally:
+        th.join()
+    if "error" in res:
+        raise res["error"]
+    return res["late"]

@@ -164,6 +164,124 @@ hella_status hella_autotune(hella_ctx ctx, const char *path);
  * prefetched in registers, chunk 8 or 16) */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
 
+/* ---- PSRDADA-style block ring (streaming ingest) -------------------------
+ * Replaces the PSRDADA data block casm-hella reads from (src/hella/src/dada.cpp:16-88;
+ * casm:659-670,812-815): a text header and fixed-size data blocks in pinned host
+ * memory, one writer and one reader, blocks read in order.  Each block carries
+ * the beam it belongs to, the beam's chunk number, flags and its arrival time
+ * (steady clock, ns, stamped when the writer closes it).  The reader may release
+ * blocks in any order; the writer reuses a block once every older one is released. */
+typedef struct hella_ring_s *hella_ring;
+
+enum {
+  HELLA_RING_LAST = 1,   /* block flag: the beam's last chunk (the beam leaves after it) */
+  HELLA_RING_END = 2     /* read result: the writer ended the stream and every block was read */
+};
+
+typedef struct {
+  int index;             /* block index for hella_ring_release (-1: nothing read) */
+  int beam;
+  long long seq;
+  int flags;
+  size_t bytes;
+  long long t_arrival_ns;
+  const void *data;
+} hella_ring_block;
+
+hella_status hella_ring_create(int nblocks, size_t block_bytes, const char *header, hella_ring *out);
+void hella_ring_destroy(hella_ring ring);
+const char *hella_ring_header(hella_ring ring);
+/* writer: wait (timeout_ms < 0: forever) for a free block, fill it, close it */
+hella_status hella_ring_open_write(hella_ring ring, int timeout_ms, void **block);
+hella_status hella_ring_close_write(hella_ring ring, int beam, long long seq, int flags, size_t bytes);
+/* writer: no more blocks */
+hella_status hella_ring_end(hella_ring ring);
+/* reader: give up; waiting writers return HELLA_ERR_STATE */
+hella_status hella_ring_abort(hella_ring ring);
+/* reader: next block in order; info->index = -1 on timeout, and flags = HELLA_RING_END
+ * once the stream has ended and been read completely */
+hella_status hella_ring_open_read(hella_ring ring, int timeout_ms, hella_ring_block *info);
+hella_status hella_ring_release(hella_ring ring, int index);
+
+/* one scheduled chunk for hella_ring_replay */
+typedef struct {
+  double t;              /* arrival time (s) from the start of the replay */
+  int beam;
+  int pool;              /* index of the chunk's data in the pool */
+  long long seq;
+  int flags;             /* HELLA_RING_LAST on a beam's final chunk */
+} hella_replay_event;
+
+/* Writer loop for tests and benchmarks: writes the events (sorted by t) at
+ * t * time_scale seconds after the call, copying pool[pool] into each block,
+ * then ends the stream.  *max_late_ns: the largest delay behind schedule (a full
+ * ring holds the writer back). */
+hella_status hella_ring_replay(hella_ring ring, const hella_replay_event *events, int n,
+                               const void *const *pool, int npool, size_t chunk_bytes,
+                               double time_scale, long long *max_late_ns);
+
+/* ---- streaming server (continuous batching) -------------------------------
+ * Serves beams that join and leave a running stream: each beam keeps its
+ * max-DM window and flagger state resident on the device; beams are admitted
+ * while the HBM budget allows (resident windows + the fixed work sets);
+ * chunks that arrive while other batches are on the GPU are launched as soon
+ * as a launch chain is free, each launch chain carrying up to max_batch beams'
+ * chunks (the batched path, captured as hipGraphs).  plain = 1 runs the plain
+ * per-beam sequence instead, one chunk at a time.  No reference counterpart:
+ * the reference processes a fixed beam set gulp by gulp (casm:659-835). */
+typedef struct hella_server_s *hella_server;
+
+typedef struct {
+  unsigned long long hbm_budget;  /* bytes for resident beams + work sets (0: no limit) */
+  int max_batch;                  /* beams per launch chain (>= 1) */
+  int lanes;                      /* launch chains in flight, one stream each (>= 1) */
+  int plain;                      /* 1: plain per-beam path (max_batch 1, one lane) */
+} hella_server_opts;
+
+typedef struct {
+  int beam;               /* beam id from the ring */
+  long long seq;          /* the beam's chunk number */
+  int searched;           /* 0 while the beam's window was still filling */
+  int ncand;              /* candidates kept (MAX_GIANTS cap per chunk) */
+  long long crossings;    /* threshold crossings before caps */
+  int nclusters;
+  long long t_arrival_ns; /* block closed by the writer */
+  long long t_done_ns;    /* candidates and clusters on the host */
+  int batch;              /* chunks in the launch chain that carried it */
+} hella_chunk_result;
+
+typedef struct {
+  long long chunks;        /* chunks processed */
+  long long batches;       /* launch chains */
+  int beams_admitted;
+  int beams_waited;        /* beams that waited for HBM before admission */
+  int max_resident;        /* most beams resident at once */
+  unsigned long long work_bytes;      /* work sets (fixed) */
+  unsigned long long peak_resident_bytes;
+  double busy_s;           /* wall time with at least one launch chain in flight */
+  double wall_s;           /* duration of hella_server_run */
+} hella_server_stats;
+
+/* config: the hella_create KEY VALUE text (geometry, DM range, widths, GULP =
+ * chunk length, SNR, SCRUNCH, ...).  Every beam uses it. */
+hella_status hella_server_create(const char *config, const hella_server_opts *opts,
+                                 hella_server *out);
+void hella_server_destroy(hella_server srv);
+/* Serve the ring until its writer ends the stream and every chunk is done.
+ * Runs in the calling thread; the writer runs in another. */
+hella_status hella_server_run(hella_server srv, hella_ring ring);
+/* results in completion order */
+int hella_server_result_count(hella_server srv);
+hella_status hella_server_get_result(hella_server srv, int i, hella_chunk_result *out);
+/* candidates / clusters of result i (reference order within the chunk) */
+int hella_server_get_candidates(hella_server srv, int i, hella_candidate *out, int cap);
+int hella_server_get_clusters(hella_server srv, int i, hella_cluster *out, int cap);
+hella_status hella_server_get_stats(hella_server srv, hella_server_stats *out);
+/* bytes one beam holds while resident, and the work-set bytes of the options */
+hella_status hella_server_footprint(const char *config, const hella_server_opts *opts,
+                                    unsigned long long *beam_bytes,
+                                    unsigned long long *work_bytes);
+
 /* ---- device helpers (test / bench plumbing) ------------------------------ */
 int hella_device_count(void);
 void *hella_dev_alloc(size_t bytes);
