@@ -71,7 +71,116 @@ def make_chunk(hamd, g, pulses, rfi, seed=20261205):
 
 
 def pct(xs, q):
-    return float(np.percentile(np.asarray(xs), q))
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+# ---------------------------------------------------------------- serving
+SERVE_CHUNK = 8192           # 1.074 s chunks of the streaming configuration
+SERVE_BUDGET = 48 << 30      # HBM admission budget
+
+
+def serve_config():
+    c = config(1)
+    c.pop("NBEAM")
+    c.update(GULP=SERVE_CHUNK, CAND_CAP=65536)
+    return c
+
+
+def serve_min_gulp(hamd):
+    plan = hamd.DedispPlan(NCH, TSAMP, 500.0, FOFF)
+    plan.generate_dm_list(0.0, 2000.0, 64.0, 1.25)
+    ntime = 32 * ((SERVE_CHUNK + plan.max_delay + MAXW) // 32 + 1)
+    plan.close()
+    return ntime // SERVE_CHUNK
+
+
+def serve_pool(hamd, n, pulses, rfi):
+    """distinct chunks of the streaming configuration (reused across beams)"""
+    return [hamd.synth(NCH, SERVE_CHUNK, 500.0, FOFF, TSAMP, seed=777, t_offset=k * SERVE_CHUNK,
+                       rfi_chan=rfi, rfi_mult=10.0, pulses=pulses,
+                       burst_period=int(round(2.0 / TSAMP)), burst_phase=1000, burst_len=10,
+                       burst_amp=6.0) for k in range(n)]
+
+
+def serve_saturation(hamd, pool, mg, nbeam, nchunk, **opts):
+    """Warm nbeam beams (fill their windows), then push nchunk searched chunks
+    per beam as fast as the server takes them: chunks/s and the latency of
+    those chunks."""
+    srv = hamd.Server(serve_config(), hbm_budget=SERVE_BUDGET, **opts)
+    P = len(pool)
+    ring = hamd.Ring(96, pool[0].nbytes, f"NCHAN {NCH}\nNBIT 8\nTSAMP {TSAMP * 1e6}\n")
+    warm = [(0.0, b, (7 * b + k) % P, k, 0) for k in range(mg) for b in range(nbeam)]
+    hamd.serve(srv, ring, warm, pool, time_scale=0.0)
+    ring.close()
+    n0, b0 = srv.stats()["chunks"], srv.stats()["batches"]
+    ring = hamd.Ring(96, pool[0].nbytes, f"NCHAN {NCH}\n")
+    ev = [(0.0, b, (7 * b + mg + k) % P, mg + k, 0) for k in range(nchunk) for b in range(nbeam)]
+    t0 = time.perf_counter()
+    hamd.serve(srv, ring, ev, pool, time_scale=0.0)
+    dt = time.perf_counter() - t0
+    ring.close()
+    res = srv.results(with_candidates=False)[n0:]
+    st = srv.stats()
+    srv.close()
+    assert all(r["searched"] for r in res)
+    lat = [(r["t_done_ns"] - r["t_arrival_ns"]) * 1e-6 for r in res]
+    rate = len(res) / dt
+    return {"chunks": len(res), "seconds": dt, "chunks_per_s": rate,
+            "realtime_beams": int(rate * SERVE_CHUNK * TSAMP),
+            "mean_batch": len(res) / max(1, st["batches"] - b0),
+            "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)}
+
+
+def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, **opts):
+    """nbeam warm beams stream chunks on the real-time cadence (random phase);
+    beams leave after exponential dwell (mean 20 s) and a new beam joins in
+    their place, filling its window before it is searched.  Latency = chunk
+    arrival to candidates on the host."""
+    rng = np.random.default_rng(seed)
+    chunk_s = SERVE_CHUNK * TSAMP
+    srv = hamd.Server(serve_config(), hbm_budget=SERVE_BUDGET, **opts)
+    P = len(pool)
+    ring = hamd.Ring(256, pool[0].nbytes, f"NCHAN {NCH}\n")
+    warm = [(0.0, b, (3 * b + k) % P, k, 0) for k in range(mg) for b in range(nbeam)]
+    t_w = time.perf_counter()
+    hamd.serve(srv, ring, warm, pool, time_scale=0.0)
+    t_w = time.perf_counter() - t_w
+    ring.close()
+    n0 = srv.stats()["chunks"]
+    ev = []
+    next_id = nbeam
+    for slot in range(nbeam):
+        # a warm beam continues its stream; when it leaves, a new beam joins
+        bid, seq, t = slot, mg, float(rng.uniform(0, chunk_s))
+        leave = float(rng.exponential(20.0))
+        while t < seconds:
+            last = t + chunk_s > leave
+            ev.append((t, bid, (5 * bid + seq) % P, seq, hamd.RING_LAST if last else 0))
+            seq += 1
+            t += chunk_s
+            if last:
+                bid, seq = next_id, 0
+                next_id += 1
+                leave = t + float(rng.exponential(20.0))
+    ev.sort(key=lambda e: (e[0], e[1]))
+    ring = hamd.Ring(256, pool[0].nbytes, f"NCHAN {NCH}\n")
+    t0 = time.perf_counter()
+    late = hamd.serve(srv, ring, ev, pool, time_scale=1.0)
+    wall = time.perf_counter() - t0
+    ring.close()
+    res = srv.results(with_candidates=False)[n0:]
+    st = srv.stats()
+    srv.close()
+    srch = [(r["t_done_ns"] - r["t_arrival_ns"]) * 1e-6 for r in res if r["searched"]]
+    allc = [(r["t_done_ns"] - r["t_arrival_ns"]) * 1e-6 for r in res]
+    return {"beams": nbeam, "seconds": seconds, "wall_s": wall, "warmup_s": t_w,
+            "chunks": len(res), "searched_chunks": len(srch), "joins": next_id - nbeam,
+            "latency_ms_p50": pct(srch, 50), "latency_ms_p99": pct(srch, 99),
+            "latency_ms_p99_all_chunks": pct(allc, 99),
+            "p99_below_chunk": bool(srch) and pct(srch, 99) < chunk_s * 1e3,
+            "writer_max_late_s": late, "max_resident": st["max_resident"],
+            "beams_waited": st["beams_waited"], "gpu_busy_frac": st["busy_s"] / max(st["wall_s"], 1e-9),
+            "work_set_GB": st["work_bytes"] / 2**30}
 
 
 def cpu_baseline(hamd, chunk_tf, sample_rows):
@@ -120,6 +229,16 @@ def main():
                     help="skip the batched path and the CPU baseline (profiling runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the streaming server runs")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the streaming server runs (profiling)")
+    ap.add_argument("--serve-beams", type=int, default=64,
+                    help="beams of the saturation runs (plain vs batched server)")
+    ap.add_argument("--serve-chunks", type=int, default=8, help="timed chunks per beam")
+    ap.add_argument("--serve-rt-beams", type=int, default=256)
+    ap.add_argument("--serve-rt-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-batch", type=int, default=32)
+    ap.add_argument("--serve-lanes", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has a single MI355X "
@@ -130,6 +249,9 @@ def main():
     if hamd.device_count() < 1:
         raise SystemExit("no GPU visible")
     pulses, rfi = stream_params()
+    if args.serve_only:
+        print(json.dumps({"serving": serving(hamd, args, pulses, rfi)}), flush=True)
+        return
     h = hamd.Hella(config(1))
     info = h.info(0)
     need = max(args.warmup, info["min_gulp"] + 1)
@@ -250,9 +372,39 @@ def main():
                              "frac": flag_bytes / (ms_flag * 1e-3) / 1e9 / PEAK_HBM_GBS,
                              "traffic": None},
     }
+    if not args.no_serve:
+        out["serving"] = serving(hamd, args, pulses, rfi)
+        out["realtime_beams"]["serving_plain"] = out["serving"]["saturation"]["plain"]["realtime_beams"]
+        out["realtime_beams"]["serving_batched"] = \
+            out["serving"]["saturation"]["batched"]["realtime_beams"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hamd, host_chunks[0], args.cpu_rows)
     print(json.dumps(out), flush=True)
+
+
+def serving(hamd, args, pulses, rfi):
+    mg = serve_min_gulp(hamd)
+    t0 = time.perf_counter()
+    pool = serve_pool(hamd, 24, pulses, rfi)
+    print(f"# serving: pool in {time.perf_counter() - t0:.1f} s, windows fill in {mg} chunks",
+          file=sys.stderr, flush=True)
+    sat = {}
+    for mode, opts in (("plain", dict(plain=True)),
+                       ("batched", dict(max_batch=args.serve_batch, lanes=args.serve_lanes))):
+        sat[mode] = serve_saturation(hamd, pool, mg, args.serve_beams, args.serve_chunks, **opts)
+        print(f"# serving {mode}: {sat[mode]['chunks_per_s']:.0f} chunks/s -> "
+              f"{sat[mode]['realtime_beams']} real-time beams", file=sys.stderr, flush=True)
+    rt = serve_realtime(hamd, pool, mg, args.serve_rt_beams, args.serve_rt_seconds,
+                        max_batch=args.serve_batch, lanes=args.serve_lanes)
+    print(f"# serving real time: {rt['beams']} beams, p50 {rt['latency_ms_p50']} ms, "
+          f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
+    return {"workload": f"streaming: 1024 ch, {SERVE_CHUNK}-sample chunks "
+                        f"({SERVE_CHUNK * TSAMP:.3f} s), DM 0-2000, widths 1..256, "
+                        f"HBM budget {SERVE_BUDGET >> 30} GB",
+            "chunk_s": SERVE_CHUNK * TSAMP, "window_fill_chunks": mg,
+            "saturation": sat,
+            "batched_over_plain": sat["batched"]["chunks_per_s"] / sat["plain"]["chunks_per_s"],
+            "realtime": rt, "max_batch": args.serve_batch, "lanes": args.serve_lanes}
 
 
 if __name__ == "__main__":

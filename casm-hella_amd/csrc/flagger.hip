@@ -265,24 +265,44 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
 }
 
 // ts_correct (src/hella/src/timeseries.cu:19-74) on the bandpass-corrected
-// plane: ts[t] = sum_c (x / bp[c]) / nch in channel order (double); the
-// division by ts happens in the following FOP_DIVBP_TS pass.
-__global__ __launch_bounds__(64) void k_ts_sum(const FlagBeamDev *__restrict__ beams, int W)
+// plane: ts[t] = (sum_c x / bp[c]) / nch in double, the channels summed as 16
+// partitions of consecutive channels (each in channel order), the partials
+// combined pairwise (p[i] += p[i + h], h = 8, 4, 2, 1).  One wave per
+// partition, 64 samples per workgroup; the division by ts happens in the
+// following FOP_DIVBP_TS pass.
+constexpr int TS_PARTS = 16;
+
+__global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__restrict__ beams,
+                                                           int W)
 {
   const FlagBeamDev &B = beams[blockIdx.z];
-  const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= W) return;
+  __shared__ double sp[TS_PARTS][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + lane;
+  const int per = (B.nch + TS_PARTS - 1) / TS_PARTS;
+  const int c0 = min(B.nch, q * per), c1 = min(B.nch, c0 + per);
   double s = 0.0;
-  const float *col = B.X + t;
-  // the sum stays sequential in channel order; unrolling only lets the loads
-  // of 16 channels be in flight together
-#pragma unroll 16
-  for (int c = 0; c < B.nch; ++c) {
-    const float b = B.bp[c];
-    const float x = col[(int64_t)c * B.xs];
-    s += (double)(b > 0.f ? x / b : 0.f);
+  if (t < W) {
+    const float *col = B.X + t;
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) {
+      const float b = B.bp[c];
+      const float x = col[(int64_t)c * B.xs];
+      s += (double)(b > 0.f ? x / b : 0.f);
+    }
   }
-  B.ts[t] = (float)(s / (double)B.nch);
+  sp[q][lane] = s;
+  __syncthreads();
+  if (q == 0 && t < W) {
+    double p[TS_PARTS];
+#pragma unroll
+    for (int i = 0; i < TS_PARTS; ++i) p[i] = sp[i][lane];
+#pragma unroll
+    for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
+#pragma unroll
+      for (int i = 0; i < h; ++i) p[i] += p[i + h];
+    B.ts[t] = (float)(p[0] / (double)B.nch);
+  }
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -344,7 +364,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   }
   // apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401): bandpass_correct,
   // ts_correct, normalize_data
-  hipLaunchKernelGGL(k_ts_sum, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam), dim3(64), 0,
+  hipLaunchKernelGGL(k_ts_sum, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam), dim3(64 * TS_PARTS), 0,
                      s, d_beams, W);
   apply(FOP_DIVBP_TS, FPART_SUM2, wp);
   stats();

@@ -80,6 +80,80 @@ bool wait_for(hella_ring r, std::unique_lock<std::mutex> &lk, int timeout_ms, Pr
   return r->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
 }
 
+// memcpy split over a fixed set of worker threads (the caller copies the
+// first slice); workers live as long as the object
+class ParallelCopy {
+ public:
+  explicit ParallelCopy(int nthreads) : n_(std::max(1, nthreads))
+  {
+    for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { work(i); });
+  }
+  ~ParallelCopy()
+  {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void copy(void *dst, const void *src, size_t bytes)
+  {
+    if (n_ == 1) {
+      std::memcpy(dst, src, bytes);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      dst_ = static_cast<uint8_t *>(dst);
+      src_ = static_cast<const uint8_t *>(src);
+      bytes_ = bytes;
+      pending_ = n_ - 1;
+      gen_ += 1;
+    }
+    cv_.notify_all();
+    slice(0);
+    std::unique_lock<std::mutex> lk(m_);
+    done_cv_.wait(lk, [&] { return pending_ == 0; });
+  }
+
+ private:
+  void slice(int i)
+  {
+    const size_t per = (bytes_ / (size_t)n_ + 63) & ~(size_t)63;
+    const size_t a = std::min(bytes_, per * (size_t)i), b = std::min(bytes_, a + per);
+    if (b > a) std::memcpy(dst_ + a, src_ + a, b - a);
+  }
+  void work(int i)
+  {
+    long long seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      slice(i);
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        pending_ -= 1;
+      }
+      done_cv_.notify_one();
+    }
+  }
+  int n_;
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  bool stop_ = false;
+  long long gen_ = 0;
+  int pending_ = 0;
+  uint8_t *dst_ = nullptr;
+  const uint8_t *src_ = nullptr;
+  size_t bytes_ = 0;
+};
+
 }  // namespace
 
 extern "C" {
@@ -229,6 +303,9 @@ hella_status hella_ring_replay(hella_ring r, const hella_replay_event *ev, int n
 {
   if (!r || (n > 0 && (!ev || !pool)) || chunk_bytes > r->block_bytes || time_scale < 0)
     return ring_fail(HELLA_ERR_INVALID, "bad argument");
+  // copies of large chunks are split over a few threads: one host thread
+  // cannot fill blocks as fast as the server drains them
+  ParallelCopy pc(chunk_bytes >= (4u << 20) ? 8 : 1);
   const long long t0 = now_ns();
   long long late = 0;
   hella_status st = HELLA_OK;
@@ -244,7 +321,7 @@ hella_status hella_ring_replay(hella_ring r, const hella_replay_event *ev, int n
     void *blk = nullptr;
     st = hella_ring_open_write(r, -1, &blk);
     if (st != HELLA_OK) break;
-    std::memcpy(blk, pool[e.pool], chunk_bytes);
+    pc.copy(blk, pool[e.pool], chunk_bytes);
     late = std::max(late, now_ns() - due);
     st = hella_ring_close_write(r, e.beam, e.seq, e.flags, chunk_bytes);
   }

@@ -13,9 +13,11 @@
  *    delay uses the same round-half-up of the float product.  Output for
  *    out_nbits=32 is float(exact integer sum).
  *  - FP16 arithmetic of the flagger (src/flagger.cu) is restated in FP32;
- *    every reduction is a double-precision canonical blocked sum (orc_csum),
- *    channel sums per sample (timeseries.cu:19-36) are sequential in channel
- *    order, in double.  mu_f = (float)mu, sigma_f = (float)sqrt(var).
+ *    every reduction is a double-precision canonical blocked sum (orc_csum);
+ *    the channel sum of each sample (timeseries.cu:19-36) is taken over 16
+ *    partitions of consecutive channels (each summed in channel order) whose
+ *    partials are combined pairwise, in double (orc_chan_sum).
+ *    mu_f = (float)mu, sigma_f = (float)sqrt(var).
  *  - Normalisation (flagger.cu:198-214, smooth.cpp:95-124) is
  *    y = (float)(((double)x - mu) * (1.0 / (double)sigma_f)).
  *  - NPP filters (convolution.cu:40, smooth.cpp:61) use the documented anchor
@@ -286,12 +288,27 @@ void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
   free(M);
 }
 
+double orc_chan_sum(const float *x, int nch, int W, int t)
+{
+  double p[ORC_TS_PARTS];
+  const int per = (nch + ORC_TS_PARTS - 1) / ORC_TS_PARTS;
+  for (int q = 0; q < ORC_TS_PARTS; ++q) {
+    const int c0 = q * per < nch ? q * per : nch;
+    const int c1 = c0 + per < nch ? c0 + per : nch;
+    double s = 0.0;
+    for (int c = c0; c < c1; ++c) s += (double)x[(size_t)c * W + t];
+    p[q] = s;
+  }
+  for (int h = ORC_TS_PARTS / 2; h >= 1; h >>= 1)
+    for (int q = 0; q < h; ++q) p[q] += p[q + h];
+  return p[0];
+}
+
 /* ts_correct (src/timeseries.cu:19-74) */
 void orc_ts_correct(float *x, int nch, int W, float *ts)
 {
   for (int t = 0; t < W; ++t) {
-    double s = 0.0;
-    for (int c = 0; c < nch; ++c) s += (double)x[(size_t)c * W + t];
+    const double s = orc_chan_sum(x, nch, W, t);
     float tv = (float)(s / (double)nch);
     if (ts) ts[t] = tv;
     for (int c = 0; c < nch; ++c) {

@@ -91,6 +91,11 @@ void orc_spec_flag(float *x, int nch, int W, float smin, float smax);  /* calc_b
 void orc_box_mean(const float *x, int nch, int W, int ts, int fs, float *out); /* npp_convolve_handler */
 void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
                       int flag1, int flag2, uint32_t *flagcount);     /* apply_scrunch flag branch */
+/* channel sum of sample t: ORC_TS_PARTS partitions of ceil(nch/16)
+ * consecutive channels, each summed in channel order, the partials combined
+ * pairwise (p[q] += p[q+h], h = 8, 4, 2, 1); double */
+#define ORC_TS_PARTS 16
+double orc_chan_sum(const float *x, int nch, int W, int t);
 void orc_ts_correct(float *x, int nch, int W, float *ts);              /* ts_correct */
 void orc_quantize(const float *x, int nch, int W, uint8_t *out, size_t out_stride); /* transpose_output */
 /* input conversion (transpose_input): u8 TF [W][nch] -> float FT */

@@ -67,6 +67,7 @@ _d("orc_spec_flag", None, _VP, _I, _I, _F, _F)
 _d("orc_box_mean", None, _VP, _I, _I, _I, _I, _VP)
 _d("orc_scrunch_flag", None, _VP, _I, _I, _I, _I, _F, _I, _I, _VP)
 _d("orc_ts_correct", None, _VP, _I, _I, _VP)
+_d("orc_chan_sum", _D, _VP, _I, _I, _I)
 _d("orc_quantize", None, _VP, _I, _I, _VP, _SZ)
 _d("orc_ingest_u8_tf", None, _VP, _I, _I, _VP)
 _d("orc_ingest_f32_tf", None, _VP, _I, _I, _VP)
@@ -207,6 +208,11 @@ def scrunch_flag(x, ts, fs, thresh, flag1=-1, flag2=-1):
     cnt = np.zeros(x.shape[0], np.uint32)
     L.orc_scrunch_flag(_p(x), x.shape[0], x.shape[1], ts, fs, thresh, flag1, flag2, _p(cnt))
     return x, cnt
+
+
+def chan_sum(x, t):
+    x = np.ascontiguousarray(x, np.float32)
+    return float(L.orc_chan_sum(_p(x), x.shape[0], x.shape[1], t))
 
 
 def ts_correct(x):

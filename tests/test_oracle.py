@@ -265,6 +265,24 @@ def test_ts_correct(orc):
     np.testing.assert_array_equal(y, (x / np.float32(3.0)).astype(np.float32))
 
 
+def test_chan_sum_partitions(orc):
+    # 32 channels -> 16 partitions of 2.  2^53 (partition 0) and -2^53
+    # (partition 1) meet at the last tree step, after the 1 of partition 2 was
+    # absorbed into 2^53: result 0, where a sequential channel sum gives 1.
+    x = np.zeros((32, 3), np.float32)
+    x[0, 0], x[2, 0], x[4, 0] = TWO53, -TWO53, 1.0
+    assert orc.chan_sum(x, 0) == 0.0
+    # partitions 0 and 8 cancel in the first tree step (h = 8), so the 1 of
+    # partition 1 survives: result 1, where summing the partials in order gives 0
+    x[:, 1] = 0
+    x[0, 1], x[16, 1], x[2, 1] = TWO53, -TWO53, 1.0
+    assert orc.chan_sum(x, 1) == 1.0
+    # exact on small integers, any channel count (empty partitions when nch < 16)
+    for nch in (1, 5, 16, 17, 100, 1024):
+        y = np.random.default_rng(nch).integers(-50, 50, (nch, 2)).astype(np.float32)
+        assert orc.chan_sum(y, 1) == float(y[:, 1].astype(np.float64).sum())
+
+
 def test_quantize_edges(orc):
     x = np.array([[-4.0, 10.0, 3.0, np.nan, np.inf, -np.inf, -100.0, 0.0]], np.float32)
     q = orc.quantize(x)[0]
