@@ -145,6 +145,10 @@ void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s)
   if (B.nout <= 0) throw std::runtime_error("gulp too short for the maximum DM delay");
   B.rows = cfg.search_mode == SEARCH_HELLA ? B.ndm - 2 : B.ndm;
   if (B.rows <= 0) throw std::runtime_error("too few DM trials for the search mode");
+  // the exact boxcar statistics keep 16 squared box sums of a thread in u64:
+  // every box sum must stay below 2^30
+  if (cfg.search_mode == SEARCH_BOXCAR && (int64_t)maxW * 255 * B.nlive >= (1LL << 30))
+    throw std::runtime_error("WIDTH_MAX x live channels x 255 must stay below 2^30");
   const int64_t ring_len = cfg.oneshot ? (int64_t)round_up((size_t)W, 16) : B.ntime;
   B.ring_stride = (int64_t)round_up((size_t)ring_len, 64);
   B.ntime = cfg.oneshot ? ring_len : B.ntime;
@@ -228,7 +232,7 @@ static WorkSizes work_sizes(const WorkDims &d)
 size_t WorkSet::bytes_for(const WorkDims &d)
 {
   const WorkSizes z = work_sizes(d);
-  return z.X + z.mask + z.D + z.planes + 2 * z.fpart + z.fts + 2 * z.spart + 2 * z.scnt +
+  return z.X + z.mask + z.D + z.planes + 2 * z.fpart + z.fts + 3 * z.spart + 6 * z.scnt +
          z.stiles + z.scands + 16;
 }
 
@@ -243,9 +247,10 @@ void WorkSet::ensure(const WorkDims &d)
   f_part2.ensure(z.fpart);
   f_ts.ensure(z.fts);
   s_part.ensure(z.spart);
-  s_part2.ensure(z.spart);
+  s_part2.ensure(2 * z.spart);  // BOXCAR statistics: u128 per slot
   s_counts.ensure(z.scnt);
   s_offs.ensure(z.scnt);
+  s_bmax.ensure(4 * z.scnt);    // one maximum per wave of each block
   s_tiles.ensure(z.stiles);
   s_cands.ensure(z.scands);
   s_ncand.ensure(16);
@@ -454,6 +459,7 @@ SearchBeamDev make_sdesc(const Config &cfg, BeamState &B, WorkSet &w)
   s.scal = B.s_scal.as<SScal>();
   s.counts = w.s_counts.as<uint32_t>();
   s.offs = w.s_offs.as<uint32_t>();
+  s.bmax = w.s_bmax.as<uint32_t>();
   s.tiles = w.s_tiles.as<uint32_t>();
   s.cands = w.s_cands.as<Cand>();
   s.ncand = w.s_ncand.as<uint32_t>();

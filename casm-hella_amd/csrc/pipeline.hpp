@@ -81,7 +81,7 @@ struct WorkDims {
 struct WorkSet {
   DevBuf X, mask, D, planes;
   DevBuf f_part, f_part2, f_ts;
-  DevBuf s_part, s_part2, s_counts, s_offs, s_tiles, s_cands, s_ncand;
+  DevBuf s_part, s_part2, s_counts, s_offs, s_bmax, s_tiles, s_cands, s_ncand;
   void ensure(const WorkDims &d);
   static size_t bytes_for(const WorkDims &d);
 };

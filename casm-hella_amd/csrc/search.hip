@@ -20,7 +20,9 @@
 #include "reduce.hpp"
 #include "search.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace hamd {
 
@@ -50,13 +52,302 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total)
   return incl - v;
 }
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// P[1..L] holds a block's dedispersed samples (P[0] = 0); afterwards P[i] is
+// the u32 (wrap-around) sum of the first i.  Thread l scans a contiguous
+// segment; segment totals are scanned within each wave by shuffles and across
+// the four waves through wtot.
+__device__ void block_prefix(uint32_t *P, int L, uint32_t *wtot)
+{
+  const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
+  const int seg = (L + NT - 1) / NT;
+  const int i0 = 1 + l * seg, i1 = min(i0 + seg, L + 1);
+  uint32_t run = 0;
+  for (int i = i0; i < i1; ++i) run += P[i];
+  const uint32_t incl = wave_incl_scan(run);
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  uint32_t acc = incl - run;
+  for (int w = 0; w < wv; ++w) acc += wtot[w];
+  for (int i = i0; i < i1; ++i) {
+    acc += P[i];
+    P[i] = acc;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ float normed(double raw, const SScal &sc, int k)
 {
   return sc.sigf[k] > 0.f ? (float)((raw - sc.mu[k]) * sc.inv[k]) : (float)raw;
 }
 
+__device__ __forceinline__ uint32_t wave_max(uint32_t x)
+{
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) x = max(x, (uint32_t)__shfl_down(x, s, 64));
+  return x;
+}
+
+constexpr int MAXL = SB + MAXBOX;        // samples a block's boxes read
+constexpr int PF = (MAXL + NT - 1) / NT; // prefetch registers per thread
+constexpr int PN = MAXL + 1;
+constexpr int TREE_WORDS = 2 * SEARCH_MAXW * 128 * 2;
+constexpr int LDSW = PN > TREE_WORDS ? PN : TREE_WORDS;
+
+// Statistics passes over a run of bpw blocks of one row.
+//  P_STATS: per width, the block sums of x and x*x (canonical lane order) and
+//           the block's largest raw value (for the count pass)
+//  P_Y0:    sums of y and y*y of the normalised first width
+// BOXCAR builds the block's prefix sums in LDS; the next block's samples are
+// loaded into registers while the current block is processed.
+template <bool HELLA, int NW>
+__global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ beams,
+                                              SearchCommon C, int pass, int bpw)
+{
+  const SearchBeamDev &B = beams[blockIdx.z];
+  const int r = blockIdx.y;
+  if (r >= B.rows) return;
+  const int b0 = blockIdx.x * bpw, b1 = min(C.nblk, b0 + bpw);
+  if (b0 >= b1) return;
+  const int l = threadIdx.x;
+  __shared__ __align__(16) uint32_t lds_u[LDSW];
+  __shared__ uint32_t wtot[NT / 64];
+  uint32_t *P = lds_u;
+  double *shd = reinterpret_cast<double *>(lds_u);
+  const SScal &sc = *B.scal;
+  const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
+  const int c0 = C.maxW / 2;
+  const bool y0 = pass == P_Y0;  // launched with NW = 1
+  const bool direct = !HELLA && y0 && C.minW == 1;
+  const int L = SB + C.maxW;
+
+  uint32_t nx[PF];
+  auto fetch = [&](int b) {
+    const int base = b * SB;
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      const int i = l + NT * t;
+      nx[t] = (i < L && base + i < C.ntime_dd) ? drow[base + i] : 0u;
+    }
+  };
+  if (!HELLA) fetch(b0);
+
+  for (int b = b0; b < b1; ++b) {
+    const int base = b * SB;
+    if (!HELLA) {
+#pragma unroll
+      for (int t = 0; t < PF; ++t) {
+        const int i = l + NT * t;
+        if (i < L) P[1 + i] = nx[t];
+      }
+      if (l == 0) P[0] = 0u;
+      if (b + 1 < b1) fetch(b + 1);
+      __syncthreads();
+      if (!direct) block_prefix(P, L, wtot);
+    }
+    auto raw = [&](int k, int j) -> double {
+      if (HELLA) return (double)B.planes[((int64_t)k * B.rows + r) * B.ps + base + j];
+      if (direct) return (double)P[1 + j + c0];
+      const int w = C.minW << k;
+      const int a = j + c0 - w / 2;
+      return (double)(uint32_t)(P[a + w] - P[a]);
+    };
+    double acc[2 * NW];
+    uint32_t mx[NW];
+#pragma unroll
+    for (int k = 0; k < 2 * NW; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) mx[k] = 0u;
+    for (int kk = 0; kk < SB / NT; ++kk) {
+      const int j = l + NT * kk;
+      if (base + j >= C.nout) break;
+      const bool in_stats = base + j < C.wp;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        double v = raw(k, j);
+        if (!y0) {
+          const uint32_t bits = HELLA ? __float_as_uint((float)v) : (uint32_t)v;
+          mx[k] = max(mx[k], bits);
+        }
+        if (y0) v = (double)normed(v, sc, 0);
+        if (in_stats) {
+          acc[k] += v;
+          acc[NW + k] += v * v;
+        }
+      }
+    }
+    if (!y0) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const uint32_t m = wave_max(mx[k]);
+        if ((l & 63) == 0)
+          B.bmax[(((int64_t)k * B.rows + r) * C.nblk + b) * 4 + (l >> 6)] = m;
+      }
+    }
+    __syncthreads();  // P is dead; its LDS becomes the tree scratch
+    tree256_multi(acc, shd);
+    if (l == 0) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const int64_t slot = ((int64_t)k * B.rows + r) * C.nblk + b;
+        B.part[slot] = acc[k];
+        B.part2[slot] = acc[NW + k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t bhi)
+{
+  const uint64_t t = lo + blo;
+  hi += bhi + (t < lo ? 1ull : 0ull);
+  lo = t;
+}
+
+// BOXCAR statistics over a run of bpw blocks of one row: per width the exact
+// integer sums S1 = sum box (u64) and S2 = sum box^2 (u128) over the
+// statistics columns, so any summation order gives the same totals; and per
+// (block, wave) the largest box sum for the count pass.  The next block's
+// samples are loaded into registers while the current block is processed.
+// Every box sum is below 2^30 (checked on the host), so a thread's 16
+// squares of one block fit a u64 before they are folded into the u128.
+// W1: widths 1, 2, 4, ... (WIDTH_MIN 1), compile-time LDS offsets.
+template <int NW, bool W1>
+__global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restrict__ beams,
+                                                  SearchCommon C, int bpw)
+{
+  const SearchBeamDev &B = beams[blockIdx.z];
+  const int r = blockIdx.y;
+  if (r >= B.rows) return;
+  const int g = blockIdx.x;
+  const int ng = (C.nblk + bpw - 1) / bpw;
+  const int b0 = g * bpw, b1 = min(C.nblk, b0 + bpw);
+  if (b0 >= b1) return;
+  const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
+  __shared__ __align__(16) uint32_t P[PN];
+  __shared__ uint32_t wtot[NT / 64];
+  __shared__ uint64_t red[NT / 64][NW][3];
+  const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
+  const int c0 = C.maxW / 2;
+  const int L = SB + C.maxW;
+  uint64_t s1[NW], q[NW], s2lo[NW], s2hi[NW];
+  uint32_t mx[NW];
+#pragma unroll
+  for (int k = 0; k < NW; ++k) s1[k] = s2lo[k] = s2hi[k] = 0ull;
+  uint32_t nx[PF];
+  auto fetch = [&](int b) {
+    const int base = b * SB;
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      const int i = l + NT * t;
+      const int col = min(base + i, C.ntime_dd - 1);
+      const uint32_t v = drow[col];
+      nx[t] = (i < L && base + i < C.ntime_dd) ? v : 0u;
+    }
+  };
+  auto body = [&](int j, auto in_stats) {
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int w = W1 ? (1 << k) : (C.minW << k);
+      const int a = j + c0 - w / 2;
+      const uint32_t box = P[a + w] - P[a];
+      mx[k] = max(mx[k], box);
+      if constexpr (decltype(in_stats)::value) {
+        s1[k] += box;
+        q[k] += (uint64_t)box * box;
+      }
+    }
+  };
+  fetch(b0);
+  for (int b = b0; b < b1; ++b) {
+    const int base = b * SB;
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      const int i = l + NT * t;
+      if (i < L) P[1 + i] = nx[t];
+    }
+    if (l == 0) P[0] = 0u;
+    if (b + 1 < b1) fetch(b + 1);
+    __syncthreads();
+    block_prefix(P, L, wtot);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      q[k] = 0ull;
+      mx[k] = 0u;
+    }
+    // iterations whose 256 samples are all searched / all in the statistics
+    // columns are uniform over the workgroup (wp is a multiple of 256)
+    const int nrem = C.nout - base;
+    const int nfull = min(SB / NT, nrem / NT);
+    const int nst = min(nfull, max(0, (C.wp - base) / NT));
+    int kk = 0;
+    for (; kk < nst; ++kk) body(l + NT * kk, std::true_type{});
+    for (; kk < nfull; ++kk) body(l + NT * kk, std::false_type{});
+    if (kk < SB / NT) {
+      const int j = l + NT * kk;
+      if (j < nrem) {
+        if (base + j < C.wp)
+          body(j, std::true_type{});
+        else
+          body(j, std::false_type{});
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      add128(s2lo[k], s2hi[k], q[k], 0ull);
+      const uint32_t m = wave_max(mx[k]);
+      if (lane == 0) B.bmax[(((int64_t)k * B.rows + r) * C.nblk + b) * 4 + wv] = m;
+    }
+    __syncthreads();  // P is rewritten by the next block
+  }
+  // integer totals: wave sums by shuffles, then the four waves
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    uint64_t a = s1[k], lo = s2lo[k], hi = s2hi[k];
+#pragma unroll
+    for (int sft = 32; sft >= 1; sft >>= 1) {
+      a += __shfl_down(a, sft, 64);
+      const uint64_t olo = __shfl_down(lo, sft, 64), ohi = __shfl_down(hi, sft, 64);
+      add128(lo, hi, olo, ohi);
+    }
+    if (lane == 0) {
+      red[wv][k][0] = a;
+      red[wv][k][1] = lo;
+      red[wv][k][2] = hi;
+    }
+  }
+  __syncthreads();
+  if (l < NW) {
+    const int k = l;
+    uint64_t a = 0, lo = 0, hi = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      a += red[w][k][0];
+      add128(lo, hi, red[w][k][1], red[w][k][2]);
+    }
+    const int64_t slot = ((int64_t)k * B.rows + r) * ng + g;
+    reinterpret_cast<uint64_t *>(B.part)[slot] = a;
+    reinterpret_cast<uint64_t *>(B.part2)[2 * slot] = lo;
+    reinterpret_cast<uint64_t *>(B.part2)[2 * slot + 1] = hi;
+  }
+}
+
+// Threshold passes over one (row, block):
+//  P_COUNT: crossings per width; widths whose block maximum cannot cross the
+//           threshold are 0 without reading D
+//  P_WRITE: candidates of the block, ordered by (width, row, sample)
 template <bool HELLA>
-__global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ beams,
+__global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ beams,
                                              SearchCommon C, int pass)
 {
   const SearchBeamDev &B = beams[blockIdx.z];
@@ -65,31 +356,37 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
   const int b = blockIdx.x;
   const int base = b * SB;
   const int l = threadIdx.x;
-  // P (prefix sums) and the reduction scratch share the LDS
-  constexpr int PN = SB + MAXBOX + 1;
-  constexpr int TREE_WORDS = 2 * SEARCH_MAXW * 128 * 2;
-  __shared__ __align__(16) uint32_t lds_u[(PN > TREE_WORDS ? PN : TREE_WORDS)];
+  __shared__ __align__(16) uint32_t P[PN];
+  __shared__ uint32_t wtot[NT / 64];
   __shared__ uint32_t shu[NT];
   __shared__ uint32_t shc[SEARCH_MAXW];
-  uint32_t *P = lds_u;
-  double *shd = reinterpret_cast<double *>(lds_u);
   const SScal &sc = *B.scal;
   const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
   const int c0 = C.maxW / 2;
+  const float thr = sc.thr;
+  auto slot = [&](int k) { return ((int64_t)k * B.rows + r) * C.nblk + b; };
 
-  if (pass == P_COUNT && sc.gated) {
-    for (int k = l; k < C.nw; k += NT) B.counts[((int64_t)k * B.rows + r) * C.nblk + b] = 0u;
-    return;
+  // widths of this block that can have crossings
+  uint32_t can = 0;
+  if (pass == P_COUNT) {
+    if (!sc.gated) {
+      for (int k = 0; k < C.nw; ++k) {
+        const uint32_t *bm = B.bmax + slot(k) * 4;
+        const uint32_t m = max(max(bm[0], bm[1]), max(bm[2], bm[3]));
+        const bool c = HELLA ? normed((double)__uint_as_float(m), sc, k) > thr
+                             : (uint64_t)m >= sc.rthr[k];
+        can |= c ? (1u << k) : 0u;
+      }
+    }
+    if (!can) {
+      for (int k = l; k < C.nw; k += NT) B.counts[slot(k)] = 0u;
+      return;
+    }
+  } else {
+    for (int k = 0; k < C.nw; ++k) can |= B.counts[slot(k)] ? (1u << k) : 0u;
+    if (!can) return;
   }
-  if (pass == P_WRITE) {
-    uint32_t any = 0;
-    for (int k = 0; k < C.nw; ++k) any |= B.counts[((int64_t)k * B.rows + r) * C.nblk + b];
-    if (!any) return;
-  }
-  // width 1 needs no prefix sums: its box is the dedispersed sample itself
-  const bool direct = !HELLA && pass == P_Y0 && C.minW == 1;
-  if (!HELLA && !direct) {
-    // prefix sums of D[row][base .. base + L) in LDS (u32, wrap-around)
+  if (!HELLA) {
     const int L = SB + C.maxW;
     for (int i = l; i < L; i += NT) {
       const int col = base + i;
@@ -97,69 +394,15 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
     }
     if (l == 0) P[0] = 0u;
     __syncthreads();
-    const int seg = (L + NT - 1) / NT;
-    const int i0 = 1 + l * seg, i1 = min(1 + (l + 1) * seg, L + 1);
-    uint32_t run = 0;
-    for (int i = i0; i < i1; ++i) run += P[i];
-    uint32_t tot;
-    const uint32_t off = block_excl_scan(run, shu, &tot);
-    uint32_t acc = off;
-    for (int i = i0; i < i1; ++i) {
-      acc += P[i];
-      P[i] = acc;
-    }
-    __syncthreads();
+    block_prefix(P, L, wtot);
   }
-
   auto raw = [&](int k, int j) -> double {
     if (HELLA) return (double)B.planes[((int64_t)k * B.rows + r) * B.ps + base + j];
-    if (direct) return (double)drow[base + j + c0];
     const int w = C.minW << k;
     const int a = j + c0 - w / 2;
     return (double)(uint32_t)(P[a + w] - P[a]);
   };
 
-  if (pass == P_STATS || pass == P_Y0) {
-    const int nwp = pass == P_Y0 ? 1 : C.nw;
-    double acc[2 * SEARCH_MAXW];
-#pragma unroll
-    for (int k = 0; k < 2 * SEARCH_MAXW; ++k) acc[k] = 0.0;
-    for (int kk = 0; kk < SB / NT; ++kk) {
-      const int j = l + NT * kk;
-      if (base + j >= C.wp) break;
-#pragma unroll
-      for (int k = 0; k < SEARCH_MAXW; ++k) {
-        if (k < nwp) {
-          double v = raw(k, j);
-          if (pass == P_Y0) v = (double)normed(v, sc, 0);
-          acc[k] += v;
-          acc[SEARCH_MAXW + k] += v * v;
-        }
-      }
-    }
-    __syncthreads();  // P is dead; its LDS becomes the tree scratch
-    if (pass == P_Y0) {
-      double two[2] = {acc[0], acc[SEARCH_MAXW]};
-      tree256_multi(two, shd);
-      acc[0] = two[0];
-      acc[SEARCH_MAXW] = two[1];
-    } else {
-      tree256_multi(acc, shd);
-    }
-    if (l == 0) {
-#pragma unroll
-      for (int k = 0; k < SEARCH_MAXW; ++k) {
-        if (k < nwp) {
-          const int64_t slot = ((int64_t)k * B.rows + r) * C.nblk + b;
-          B.part[slot] = acc[k];
-          B.part2[slot] = acc[SEARCH_MAXW + k];
-        }
-      }
-    }
-    return;
-  }
-
-  const float thr = sc.thr;
   if (pass == P_COUNT) {
     if (l < SEARCH_MAXW) shc[l] = 0u;
     __syncthreads();
@@ -171,7 +414,7 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
       if (base + j >= C.nout) break;
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k)
-        if (k < C.nw) {
+        if (k < C.nw && ((can >> k) & 1u)) {
           if (HELLA)
             cnt[k] += normed(raw(k, j), sc, k) > thr ? 1u : 0u;
           else
@@ -189,15 +432,14 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
       }
     }
     __syncthreads();
-    if (l < C.nw) B.counts[((int64_t)l * B.rows + r) * C.nblk + b] = shc[l];
+    if (l < C.nw) B.counts[slot(l)] = shc[l];
     return;
   }
 
   // P_WRITE: ordered by (width, row, sample)
   for (int k = 0; k < C.nw; ++k) {
-    const int64_t slot = ((int64_t)k * B.rows + r) * C.nblk + b;
-    if (B.counts[slot] == 0u) continue;
-    uint32_t pos = B.offs[slot];
+    if (!((can >> k) & 1u)) continue;
+    uint32_t pos = B.offs[slot(k)];
     for (int kk = 0; kk < SB / NT; ++kk) {
       const int j = l + NT * kk;
       const bool in = base + j < C.nout;
@@ -224,11 +466,39 @@ __global__ __launch_bounds__(NT) void k_pass(const SearchBeamDev *__restrict__ b
 }
 
 // Statistics from the block sums (calculate_stddev, normalization.cu:125-162):
-// S1_r, S2_r = ascending sums of the row's block partials; mu = canonical
-// sum over rows of S1_r / n, divided by the row count; the row variance about
-// mu is ((S2_r - 2 mu S1_r) + n mu mu) / n, combined the same way.
+// S1_r, S2_r = the row's sums of x and x*x; mu = canonical sum over rows of
+// S1_r / n, divided by the row count; the row variance about mu is
+// ((S2_r - 2 mu S1_r) + n mu mu) / n, combined the same way.
+// Row sums: BOXCAR statistics hold exact integer partials per group of blocks
+// (S1 u64, S2 u128 as lo/hi), summed exactly and rounded to double once
+// (S2 = hi * 2^64 + lo); otherwise the double block partials are added in
+// ascending block order.
+__device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, int r, bool exact,
+                         int ngroups, bool want2, double &s1, double &s2)
+{
+  if (exact) {
+    const uint64_t *p1 = reinterpret_cast<const uint64_t *>(B.part);
+    const uint64_t *p2 = reinterpret_cast<const uint64_t *>(B.part2);
+    const int64_t base = ((int64_t)k * B.rows + r) * ngroups;
+    uint64_t a = 0, lo = 0, hi = 0;
+    for (int g = 0; g < ngroups; ++g) {
+      a += p1[base + g];
+      if (want2) add128(lo, hi, p2[2 * (base + g)], p2[2 * (base + g) + 1]);
+    }
+    s1 = (double)a;
+    s2 = want2 ? (double)hi * 18446744073709551616.0 + (double)lo : 0.0;
+    return;
+  }
+  const int64_t base = ((int64_t)k * B.rows + r) * C.nblk;
+  s1 = 0.0;
+  s2 = 0.0;
+  for (int b = 0; b < C.nblk; ++b) s1 += B.part[base + b];
+  if (want2)
+    for (int b = 0; b < C.nblk; ++b) s2 += B.part2[base + b];
+}
+
 __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k, double mu,
-                            bool var, double *sh)
+                            bool var, bool exact, int ngroups, double *sh)
 {
   const int l = threadIdx.x;
   const double n = (double)C.wp;
@@ -238,16 +508,12 @@ __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k
     for (int kk = 0; kk < SB / NT; ++kk) {
       const int r = r0 + l + NT * kk;
       if (r >= B.rows) break;
-      const int64_t base = ((int64_t)k * B.rows + r) * C.nblk;
-      double s1 = 0.0;
-      for (int b = 0; b < C.nblk; ++b) s1 += B.part[base + b];
-      if (!var) {
+      double s1, s2;
+      row_sums(B, C, k, r, exact, ngroups, var, s1, s2);
+      if (!var)
         acc[0] += s1 / n;
-      } else {
-        double s2 = 0.0;
-        for (int b = 0; b < C.nblk; ++b) s2 += B.part2[base + b];
+      else
         acc[0] += ((s2 - (2.0 * mu) * s1) + (n * mu) * mu) / n;
-      }
     }
     tree256_multi(acc, sh);
     if (l == 0) sh[256] = acc[0];
@@ -259,13 +525,14 @@ __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k
 }
 
 __global__ __launch_bounds__(NT) void k_finish(const SearchBeamDev *__restrict__ beams,
-                                               SearchCommon C, int which)
+                                               SearchCommon C, int which, int ngroups)
 {
   const SearchBeamDev &B = beams[blockIdx.y];
   const int k = blockIdx.x;
   __shared__ double sh[260];  // 128 tree slots + the broadcast slot 256
-  const double mu = rows_csum(B, C, k, 0.0, false, sh);
-  const double var = rows_csum(B, C, k, mu, true, sh);
+  const bool exact = which == P_STATS && C.mode == SEARCH_BOXCAR;
+  const double mu = rows_csum(B, C, k, 0.0, false, exact, ngroups, sh);
+  const double var = rows_csum(B, C, k, mu, true, exact, ngroups, sh);
   SScal &sc = *B.scal;
   const float sf = (float)sqrt(var);
   if (which == P_STATS) {
@@ -421,6 +688,54 @@ void hella_kernel_host(int sm, std::vector<float> &k)
   for (auto &x : k) x /= filtSum;
 }
 
+template <bool HELLA>
+static void launch_stats(const SearchBeamDev *d_beams, const SearchCommon &C, int pass, int bpw,
+                         dim3 grid, hipStream_t s)
+{
+  const int nw = pass == P_Y0 ? 1 : C.nw;
+  switch (nw) {
+#define HAMD_STATS_CASE(N)                                                                 \
+  case N:                                                                                  \
+    hipLaunchKernelGGL((k_stats<HELLA, N>), grid, dim3(NT), 0, s, d_beams, C, pass, bpw); \
+    break;
+    HAMD_STATS_CASE(1) HAMD_STATS_CASE(2) HAMD_STATS_CASE(3) HAMD_STATS_CASE(4)
+    HAMD_STATS_CASE(5) HAMD_STATS_CASE(6) HAMD_STATS_CASE(7) HAMD_STATS_CASE(8)
+    HAMD_STATS_CASE(9) HAMD_STATS_CASE(10) HAMD_STATS_CASE(11) HAMD_STATS_CASE(12)
+#undef HAMD_STATS_CASE
+    default:
+      throw HipError{hipErrorInvalidValue, "width count", __LINE__};
+  }
+}
+
+static void launch_stats_box(const SearchBeamDev *d_beams, const SearchCommon &C, int bpw,
+                             dim3 grid, hipStream_t s)
+{
+  switch (C.nw) {
+#define HAMD_BOX_CASE(N)                                                                  \
+  case N:                                                                                 \
+    if (C.minW == 1)                                                                      \
+      hipLaunchKernelGGL((k_stats_box<N, true>), grid, dim3(NT), 0, s, d_beams, C, bpw);   \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_stats_box<N, false>), grid, dim3(NT), 0, s, d_beams, C, bpw);  \
+    break;
+    HAMD_BOX_CASE(1) HAMD_BOX_CASE(2) HAMD_BOX_CASE(3) HAMD_BOX_CASE(4)
+    HAMD_BOX_CASE(5) HAMD_BOX_CASE(6) HAMD_BOX_CASE(7) HAMD_BOX_CASE(8)
+    HAMD_BOX_CASE(9) HAMD_BOX_CASE(10) HAMD_BOX_CASE(11) HAMD_BOX_CASE(12)
+#undef HAMD_BOX_CASE
+    default:
+      throw HipError{hipErrorInvalidValue, "width count", __LINE__};
+  }
+}
+
+// blocks per workgroup of the statistics passes: enough workgroups for a few
+// per CU, the rest of a row's blocks walked with register prefetch
+static int stats_bpw(const SearchCommon &C, int max_rows, int nbeam)
+{
+  const long long wgs_wanted = 256LL * 12;
+  const long long per = (long long)C.nblk * max_rows * nbeam / wgs_wanted;
+  return (int)std::max(1LL, std::min((long long)C.nblk, per));
+}
+
 void SearchRunner::run(const SearchBeamDev *d_beams, int nbeam, int max_rows,
                        const SearchCommon &C, const float *d_kern, const std::vector<int> &kofs,
                        hipStream_t s)
@@ -436,7 +751,8 @@ void SearchRunner::smooth(const SearchBeamDev *d_beams, int nbeam, int max_rows,
   if (nbeam <= 0 || max_rows <= 0 || C.nout <= 0) return;
   if (C.nw > SEARCH_MAXW || C.maxW > MAXBOX) throw HipError{hipErrorInvalidValue, "widths", __LINE__};
   const bool hella = C.mode == SEARCH_HELLA;
-  const dim3 gp((unsigned)C.nblk, (unsigned)max_rows, (unsigned)nbeam);
+  const int bpw = stats_bpw(C, max_rows, nbeam);
+  const dim3 gp((unsigned)((C.nblk + bpw - 1) / bpw), (unsigned)max_rows, (unsigned)nbeam);
   if (hella) {
     for (int k = 0; k < C.nw; ++k) {
       const int sm = C.minW << k;
@@ -446,11 +762,11 @@ void SearchRunner::smooth(const SearchBeamDev *d_beams, int nbeam, int max_rows,
     }
   }
   if (hella)
-    hipLaunchKernelGGL(k_pass<true>, gp, dim3(NT), 0, s, d_beams, C, (int)P_STATS);
+    launch_stats<true>(d_beams, C, (int)P_STATS, bpw, gp, s);
   else
-    hipLaunchKernelGGL(k_pass<false>, gp, dim3(NT), 0, s, d_beams, C, (int)P_STATS);
+    launch_stats_box(d_beams, C, bpw, gp, s);
   hipLaunchKernelGGL(k_finish, dim3((unsigned)C.nw, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C,
-                     (int)P_STATS);
+                     (int)P_STATS, (int)gp.x);
   HAMD_CHECK(hipGetLastError());
 }
 
@@ -459,15 +775,23 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
 {
   if (nbeam <= 0 || max_rows <= 0 || C.nout <= 0) return;
   const bool hella = C.mode == SEARCH_HELLA;
+  const int bpw = stats_bpw(C, max_rows, nbeam);
+  const dim3 gs((unsigned)((C.nblk + bpw - 1) / bpw), (unsigned)max_rows, (unsigned)nbeam);
   const dim3 gp((unsigned)C.nblk, (unsigned)max_rows, (unsigned)nbeam);
   auto pass = [&](int p) {
-    if (hella)
-      hipLaunchKernelGGL(k_pass<true>, gp, dim3(NT), 0, s, d_beams, C, p);
-    else
-      hipLaunchKernelGGL(k_pass<false>, gp, dim3(NT), 0, s, d_beams, C, p);
+    if (p == P_Y0) {
+      if (hella)
+        launch_stats<true>(d_beams, C, p, bpw, gs, s);
+      else
+        launch_stats<false>(d_beams, C, p, bpw, gs, s);
+    } else if (hella) {
+      hipLaunchKernelGGL(k_hits<true>, gp, dim3(NT), 0, s, d_beams, C, p);
+    } else {
+      hipLaunchKernelGGL(k_hits<false>, gp, dim3(NT), 0, s, d_beams, C, p);
+    }
   };
   auto fin = [&](int p) {
-    hipLaunchKernelGGL(k_finish, dim3(1u, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C, p);
+    hipLaunchKernelGGL(k_finish, dim3(1u, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C, p, (int)gs.x);
   };
   pass(P_Y0);
   fin(P_Y0);

@@ -39,11 +39,16 @@ struct SearchBeamDev {
   int row0;           // first D row searched
   float *planes;      // hella mode: raw smoothed planes [nw][rows][ps]
   int64_t ps;
-  double *part;       // [nw][rows][nblk] block sums of x
-  double *part2;      // [nw][rows][nblk] block sums of x*x
+  double *part;       // [nw][rows][nblk] block sums of x (BOXCAR statistics:
+                      // exact u64 sums per group of blocks)
+  double *part2;      // [nw][rows][nblk] block sums of x*x (BOXCAR statistics:
+                      // exact u128 sums per group, lo/hi u64 pairs; 2x the size)
   SScal *scal;
   uint32_t *counts;   // [nw][rows][nblk]
   uint32_t *offs;     // exclusive scan of counts
+  uint32_t *bmax;     // [nw][rows][nblk][4] largest raw value of each wave's
+                      // samples of a block (u32 box sum, or the bits of a
+                      // non-negative float in hella mode)
   uint32_t *tiles;    // scan scratch [ceil(n/SEARCH_BLOCK)]
   Cand *cands;        // ordered candidates (first cap stored)
   uint32_t *ncand;    // total crossings (uncapped)

@@ -27,6 +27,9 @@
  *  - Quantise (transpose.cu:65-75) maps NaN to 0.
  *  - Statistics use 256*floor(w/256) columns (normalization.cu:129); when
  *    w < 256 (where the reference divides by zero) all w columns are used.
+ *  - BOXCAR width statistics take the row sums of the box values and of
+ *    their squares exactly (integers), rounded to double once; every other
+ *    statistic uses the canonical blocked double sum.
  */
 #include "oracle.h"
 
@@ -503,9 +506,23 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
       double *S2 = (double *)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
       for (int r = 0; r < rows; ++r) {
         const double *row = rawplane + (size_t)r * nout;
-        for (size_t j = 0; j < wp; ++j) tmp[j] = row[j] * row[j];
-        S1[r] = orc_csum(row, wp);
-        S2[r] = orc_csum(tmp, wp);
+        if (cfg->mode == ORC_MODE_BOXCAR) {
+          /* exact integer sums of the box values and their squares, rounded
+           * to double once (S2 = hi * 2^64 + lo) */
+          uint64_t s1 = 0;
+          unsigned __int128 s2 = 0;
+          for (size_t j = 0; j < wp; ++j) {
+            const uint64_t b = (uint64_t)row[j];
+            s1 += b;
+            s2 += (unsigned __int128)b * b;
+          }
+          S1[r] = (double)s1;
+          S2[r] = (double)(uint64_t)(s2 >> 64) * 18446744073709551616.0 + (double)(uint64_t)s2;
+        } else {
+          for (size_t j = 0; j < wp; ++j) tmp[j] = row[j] * row[j];
+          S1[r] = orc_csum(row, wp);
+          S2[r] = orc_csum(tmp, wp);
+        }
       }
       orc_stats_rows(S1, S2, (size_t)rows, (double)wp, &m, &v);
       free(S1);

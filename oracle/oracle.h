@@ -136,7 +136,10 @@ int orc_width_of(const orc_search_cfg *cfg, int k);
 void orc_hella_kernel(int sm, float *k);
 
 /* Full search of one beam's dedispersed plane D[ndm][ntime_dd] (row stride
- * d_stride, exact integer sums).  Writes per-width mu/sigma (nwidth each),
+ * d_stride, exact integer sums).  Per width, the statistics follow
+ * orc_stats_rows; in BOXCAR mode the row sums S1 = sum box and S2 = sum box^2
+ * are exact integers rounded to double once (S2 as hi * 2^64 + lo), in HELLA
+ * mode they are canonical double sums of the float plane.  Writes per-width mu/sigma (nwidth each),
  * sigma0, returns the number of threshold crossings (all, uncapped); stores
  * the first min(count, cfg->max_giants) into cands (in width, row, sample
  * order).  plane_out (may be NULL) receives the normalised planes
