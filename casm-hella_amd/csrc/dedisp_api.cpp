@@ -19,7 +19,8 @@ struct dedisp_plan_struct {
   std::vector<dedisp_bool> killmask_pub;
   // device-side copies, rebuilt when the DM list or the killmask change
   bool dirty = true;
-  DevBuf d_delta, d_live;
+  DevBuf d_delta, d_live, d_delta_t;
+  int dt_stride = 0;
   int nlive = 0;
   std::vector<int32_t> delta_host, live_host;
   DDTile tile{16, 32, 1};
@@ -37,23 +38,20 @@ namespace hamd {
 DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
                       int lanes_pref, size_t lds_budget)
 {
-  static const int gs[] = {8, 16, 32, 64};
-  for (int gi = 0; gi < 4; ++gi) {
-    int G = gs[gi];
+  // measured on MI355X: 32 lanes per trial with 8-channel chunks prefetched
+  // in registers is fastest; other lane counts when the spread does not fit
+  static const int gs[] = {32, 64, 16, 8};
+  for (int G : gs) {
     if (G < lanes_pref) continue;
-    int nd = 256 / G;
-    int spread = dd_max_spread(delta, ndm, nlive, nd);
-    for (int cc : {32, 16, 8}) {
-      DDTile t{G, cc, 1};
-      if (dd_lds_bytes(t, spread) <= lds_budget) {
-        *spread_out = spread;
-        return t;
-      }
+    const int spread = dd_max_spread(delta, ndm, nlive, 256 / G);
+    const DDTile t{G, 8, 2};
+    if (dd_lds_bytes(t, spread) <= lds_budget) {
+      *spread_out = spread;
+      return t;
     }
   }
-  DDTile t{64, 8, 1};
   *spread_out = dd_max_spread(delta, ndm, nlive, 4);
-  return t;
+  return DDTile{64, 8, 0};
 }
 }  // namespace hamd
 
@@ -74,6 +72,10 @@ dedisp_error refresh_device(dedisp_plan plan)
                          hipMemcpyHostToDevice));
     HAMD_CHECK(hipMemcpy(plan->d_live.p, plan->live_host.data(), plan->live_host.size() * 4,
                          hipMemcpyHostToDevice));
+    std::vector<int32_t> dt;
+    delays_transposed(plan->delta_host, ndm, plan->nlive, dt, plan->dt_stride);
+    plan->d_delta_t.ensure(dt.size() * 4);
+    HAMD_CHECK(hipMemcpy(plan->d_delta_t.p, dt.data(), dt.size() * 4, hipMemcpyHostToDevice));
   }
   plan->dirty = false;
   return DEDISP_NO_ERROR;
@@ -233,6 +235,8 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
     B.start = 0;
     B.delta = plan->d_delta.as<int32_t>() + first_dm_idx * (size_t)plan->nlive;
     B.live = plan->d_live.as<int32_t>();
+    B.delta_t = plan->d_delta_t.as<int32_t>() + first_dm_idx;
+    B.dt_stride = plan->dt_stride;
     B.nlive = plan->nlive;
     B.ndm = (int)dm_count;
     B.nout = (int)nout;
@@ -255,7 +259,7 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
       plan->d_beam.ensure(sizeof(DDBeamDev));
       HAMD_CHECK(hipMemcpyAsync(plan->d_beam.p, &B, sizeof(DDBeamDev), hipMemcpyHostToDevice, s));
       dd_launch(plan->d_beam.as<DDBeamDev>(), plan->d_tiles.as<DDTileDesc>(),
-                (int)plan->tiles.size(), plan->tile, plan->max_spread, s);
+                (int)plan->tiles.size(), plan->tile, plan->max_spread, s, (int64_t)ring);
     }
     if (!dev)
       HAMD_CHECK(hipMemcpyAsync(out, d_out, dm_count * out_stride, hipMemcpyDeviceToHost, s));

@@ -33,6 +33,25 @@ constexpr int NW = V / 4 + 1;  // LDS dwords read per channel per thread
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// Read-only table entry at a wave-uniform index through the scalar cache (a
+// constant-address-space load): it waits on lgkmcnt, so it does not force the
+// window loads in flight on vmcnt to complete first.
+template <class T>
+__device__ __forceinline__ T sload(const T *p, int64_t i)
+{
+  typedef __attribute__((address_space(4))) const T ct;
+  return ((ct *)p)[i];
+}
+
+// load through a global-address-space pointer: a global (not flat) load,
+// tracked by the vector memory counter only
+template <class T>
+__device__ __forceinline__ T gload(const T *p)
+{
+  typedef __attribute__((address_space(1))) const T gt;
+  return *((gt *)p);
+}
+
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
   return __builtin_amdgcn_perm(hi, lo, sel);
@@ -70,6 +89,38 @@ __device__ __forceinline__ void store_out(const DDBeamDev &B, int d, int tb, con
   }
 }
 
+// One DM trial's place in a staged chunk: the byte address of the LDS dword
+// holding the trial's first sample for lane group 0 (row byte offset + q
+// rounded down to a dword; q = the trial's offset into the channel window)
+// and the v_perm selector that extracts the even samples at that byte
+// alignment (the odd-sample selector is one more in each half).  Built once
+// per (channel, trial) while the chunk is staged.
+__device__ __forceinline__ uint2 tab_entry(int row_byte, int q)
+{
+  const uint32_t s = (uint32_t)(q & 3);
+  return make_uint2((uint32_t)(row_byte + (q & ~3)), s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u);
+}
+
+// add the nc staged channels of a chunk to a thread's V sums (u16x2 lanes)
+template <int ND>
+__device__ __forceinline__ void sum_chunk(uint32_t (&acc)[V / 2], const uint8_t *sdat,
+                                          const uint2 *tab, int nc, int dl, int g)
+{
+  for (int cc = 0; cc < nc; ++cc) {
+    const uint2 e = tab[cc * ND + dl];
+    const uint32_t *wrow = reinterpret_cast<const uint32_t *>(sdat + e.x) + g * (V / 4);
+    const uint32_t sel_o = e.y + 0x00010001u;
+    uint32_t w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = wrow[i];
+#pragma unroll
+    for (int k = 0; k < V / 4; ++k) {
+      acc[2 * k] += perm(w[k + 1], w[k], e.y);
+      acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
+    }
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ beams,
                                                const DDTileDesc *__restrict__ tiles, int chunk,
@@ -83,14 +134,14 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
   const DDBeamDev B = beams[td.beam];
   uint8_t *sdat = lds;                                                      // [chunk][lrow]
   int32_t *sdly = reinterpret_cast<int32_t *>(lds + (size_t)chunk * lrow);  // [chunk][ND]
-  int32_t *soff = sdly + chunk * ND;                                        // [chunk]
+  uint2 *tab = reinterpret_cast<uint2 *>(sdly + chunk * ND + chunk);        // [chunk][ND]
 
   const int tid = threadIdx.x;
   const int dl = tid / G;
   const int g = tid % G;
   const int d0 = td.d0;
   const int t0 = td.t0;
-  const int wave = tid >> 6, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int nlive = B.nlive, ndm = B.ndm;
 
   uint32_t acc[V / 2];
@@ -107,7 +158,7 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
     if (i >= nc * ND) return 0;
     const int cc = i / ND, dd = i - (i / ND) * ND;
     const int dm = min(d0 + dd, ndm - 1);
-    return B.delta[(int64_t)dm * nlive + k0 + cc];
+    return gload(B.delta + (int64_t)dm * nlive + k0 + cc);
   };
   int32_t pre = dly_load(0, tid);
   const int ring = (int)B.ring;
@@ -128,31 +179,17 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
       const int al = first & ~15;
       const int off = first - al;
       const int n16 = (off + T + spread + 4 + 15) >> 4;
-      const uint8_t *row = B.base + (int64_t)B.live[k0 + cc] * B.row_stride;
+      const uint8_t *row = B.base + (int64_t)sload(B.live, k0 + cc) * B.row_stride;
       uint8_t *dst = sdat + (size_t)cc * lrow;
       for (int u = lane; u < n16; u += 64) {
         int col = al + 16 * u;
         while (col >= ring) col -= ring;
-        *reinterpret_cast<uint4 *>(dst + 16 * u) = *reinterpret_cast<const uint4 *>(row + col);
+        *reinterpret_cast<u32x4 *>(dst + 16 * u) = gload(reinterpret_cast<const u32x4 *>(row + col));
       }
-      if (lane == 0) soff[cc] = off;
+      if (lane < ND) tab[cc * ND + lane] = tab_entry(cc * lrow, off + sdly[cc * ND + lane] - base_d);
     }
     __syncthreads();
-    for (int cc = 0; cc < nc; ++cc) {
-      const int p = g * V + soff[cc] + sdly[cc * ND + dl] - sdly[cc * ND];
-      const uint32_t *wrow = reinterpret_cast<const uint32_t *>(sdat + (size_t)cc * lrow) + (p >> 2);
-      const uint32_t s = (uint32_t)(p & 3);
-      const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
-      const uint32_t sel_o = (s + 1u) | 0x0c00u | ((s + 3u) << 16) | 0x0c000000u;
-      uint32_t w[NW];
-#pragma unroll
-      for (int i = 0; i < NW; ++i) w[i] = wrow[i];
-#pragma unroll
-      for (int k = 0; k < V / 4; ++k) {
-        acc[2 * k] += perm(w[k + 1], w[k], sel_e);
-        acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
-      }
-    }
+    sum_chunk<ND>(acc, sdat, tab, nc, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + chunk >= nlive) {
 #pragma unroll
@@ -190,9 +227,9 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
   const DDTileDesc td = tiles[blockIdx.x];
   if (td.beam < 0) return;
   const DDBeamDev B = beams[td.beam];
-  // layout: data[2][chunk][lrow] | rel[2][chunk][ND] (window offset of each DM)
+  // layout: data[2][chunk][lrow] | tab[2][chunk][ND]
   uint8_t *sdat = lds;
-  int32_t *srel = reinterpret_cast<int32_t *>(lds + (size_t)2 * chunk * lrow);
+  uint2 *tab = reinterpret_cast<uint2 *>(lds + (size_t)2 * chunk * lrow);
 
   const int tid = threadIdx.x;
   const int dl = tid / G;
@@ -209,16 +246,17 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
   constexpr int MAXC = CC / (NT / 64);
   const int ring = (int)B.ring;
   int32_t pb[MAXC], pl[MAXC], pr[MAXC];
+  const int nlast = dlast - d0;
+  const int mylane = min(lane, nlast);
   auto prefetch = [&](int k0) {
     const int nc = min(chunk, nlive - k0);
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
-      const int cc = wave + 4 * i;
-      if (cc < nc) {
-        pb[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)d0 * nlive + k0 + cc]);
-        pl[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)dlast * nlive + k0 + cc]);
-        pr[i] = lane < ND ? B.delta[(int64_t)min(d0 + lane, ndm - 1) * nlive + k0 + cc] : 0;
-      }
+      const int cc = min(wave + 4 * i, nc - 1);
+      const int32_t *dt = B.delta_t + (int64_t)(k0 + cc) * B.dt_stride + d0;
+      pb[i] = sload(dt, 0);
+      pl[i] = sload(dt, nlast);
+      pr[i] = gload(dt + mylane);
     }
   };
   auto stage = [&](int k0, int buf) {
@@ -234,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       const int al = first & ~15;
       const int off = first - al;
       const int n16 = (off + T + spread + 4 + 15) >> 4;
-      const uint8_t *row = B.base + (int64_t)B.live[k0 + cc] * B.row_stride;
+      const uint8_t *row = B.base + (int64_t)sload(B.live, k0 + cc) * B.row_stride;
       uint8_t *dst = sdat + ((size_t)buf * chunk + cc) * lrow;
       for (int j = 0; j * 64 < n16; ++j) {
         const int u = j * 64 + lane;
@@ -245,7 +283,9 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
                                            16, 0, 0);
         }
       }
-      if (lane < ND) srel[((size_t)buf * chunk + cc) * ND + lane] = off + pr[i] - base_d;
+      if (lane < ND)
+        tab[((size_t)buf * chunk + cc) * ND + lane] =
+            tab_entry((buf * chunk + cc) * lrow, off + pr[i] - base_d);
     }
   };
 
@@ -268,23 +308,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       stage(k0 + chunk, buf ^ 1);
       if (k0 + 2 * chunk < nlive) prefetch(k0 + 2 * chunk);
     }
-    const uint8_t *dat = sdat + (size_t)buf * chunk * lrow;
-    const int32_t *rel = srel + (size_t)buf * chunk * ND;
-    for (int cc = 0; cc < nc; ++cc) {
-      const int p = g * V + rel[cc * ND + dl];
-      const uint32_t *wrow = reinterpret_cast<const uint32_t *>(dat + (size_t)cc * lrow) + (p >> 2);
-      const uint32_t s = (uint32_t)(p & 3);
-      const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
-      const uint32_t sel_o = (s + 1u) | 0x0c00u | ((s + 3u) << 16) | 0x0c000000u;
-      uint32_t w[NW];
-#pragma unroll
-      for (int i = 0; i < NW; ++i) w[i] = wrow[i];
-#pragma unroll
-      for (int k = 0; k < V / 4; ++k) {
-        acc[2 * k] += perm(w[k + 1], w[k], sel_e);
-        acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
-      }
-    }
+    sum_chunk<ND>(acc, sdat, tab + (size_t)buf * chunk * ND, nc, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + chunk >= nlive) {
 #pragma unroll
@@ -304,9 +328,13 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
 
 // Register-pipelined variant: while chunk k is summed from LDS, the 16-byte
 // units of chunk k+1's windows are already loading into registers (UPL units
-// per lane per staged channel); after the sums they are written to the single
-// LDS buffer.  Delays for the staging are loaded two chunks ahead.  Units
-// beyond UPL*64 of a window (very wide delay spreads) are copied directly.
+// per lane per staged channel, through global loads that only the vector
+// memory counter tracks); after the sums they are written to the single LDS
+// buffer.  The first and last trial delays of a staged channel are scalar
+// loads from the channel-major table two chunks ahead; each lane's own trial
+// delay travels with the window units.  Every load is unconditional (clamped
+// indices), so none has to complete where a branch joins.  Units beyond
+// UPL*64 of a window (very wide delay spreads) are copied directly.
 template <int G, int CC, int UPL>
 __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ beams,
                                                   const DDTileDesc *__restrict__ tiles, int lrow)
@@ -318,8 +346,8 @@ __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ 
   const DDTileDesc td = tiles[blockIdx.x];
   if (td.beam < 0) return;
   const DDBeamDev B = beams[td.beam];
-  uint8_t *sdat = lds;                                                   // [CC][lrow]
-  int32_t *srel = reinterpret_cast<int32_t *>(lds + (size_t)CC * lrow);  // [CC][ND]
+  uint8_t *sdat = lds;                                             // [CC][lrow]
+  uint2 *tab = reinterpret_cast<uint2 *>(lds + (size_t)CC * lrow);  // [CC][ND]
 
   const int tid = threadIdx.x;
   const int dl = tid / G;
@@ -327,68 +355,51 @@ __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ 
   const int d0 = td.d0;
   const int t0 = td.t0;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  const int nlive = B.nlive, ndm = B.ndm;
-  const int dlast = min(d0 + ND - 1, ndm - 1);
+  const int nlive = B.nlive;
+  const int nlast = min(ND - 1, B.ndm - 1 - d0);  // last trial of the block (relative)
+  const int mylane = min(lane, nlast);
   const int ring = (int)B.ring;
 
-  // per staged channel slot: delays (first DM, last DM, this lane's DM) and
-  // the registers holding its window units
-  int32_t pb[MAXC], pl[MAXC], pr[MAXC];
+  int32_t pb[MAXC], pl[MAXC];  // scalar: first / last trial delay of each slot
+  int32_t pr[MAXC];            // this lane's trial delay
   u32x4 ureg[MAXC * UPL];
-  int uoff[MAXC], un16[MAXC], ualgn[MAXC], urel[MAXC];
+  int ubias[MAXC], un16[MAXC], ualgn[MAXC];
   int64_t urow[MAXC];
 
-#define HAMD_PF_PREFETCH_DLY(K0)                                                           \
-  {                                                                                        \
-    const int nc_ = min(CC, nlive - (K0));                                                 \
-    _Pragma("unroll") for (int i = 0; i < MAXC; ++i) {                                     \
-      const int cc = wave + 4 * i;                                                         \
-      if (cc < nc_) {                                                                      \
-        pb[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)d0 * nlive + (K0) + cc]);   \
-        pl[i] = __builtin_amdgcn_readfirstlane(B.delta[(int64_t)dlast * nlive + (K0) + cc]);\
-        pr[i] = lane < ND ? B.delta[(int64_t)min(d0 + lane, ndm - 1) * nlive + (K0) + cc] : 0; \
-      }                                                                                    \
-    }                                                                                      \
-  }
-#define HAMD_PF_LOAD(K0)                                                                   \
-  {                                                                                        \
-    const int nc_ = min(CC, nlive - (K0));                                                 \
-    _Pragma("unroll") for (int i = 0; i < MAXC; ++i) {                                     \
-      const int cc = wave + 4 * i;                                                         \
-      if (cc < nc_) {                                                                      \
-        int first = (int)B.start + t0 + pb[i];                                             \
-        first -= first >= ring ? ring : 0;                                                 \
-        const int al = first & ~15;                                                        \
-        uoff[i] = first - al;                                                              \
-        urel[i] = uoff[i] + pr[i] - pb[i];                                                 \
-        ualgn[i] = al;                                                                     \
-        un16[i] = (uoff[i] + T + (pl[i] - pb[i]) + 4 + 15) >> 4;                           \
-        urow[i] = (int64_t)B.live[(K0) + cc] * B.row_stride;                               \
-        _Pragma("unroll") for (int j = 0; j < UPL; ++j) {                                  \
-          const int u = j * 64 + lane;                                                     \
-          int col = al + 16 * u;                                                           \
-          col -= col >= ring ? ring : 0;                                                   \
-          if (u < un16[i])                                                                 \
-            ureg[i * UPL + j] = *reinterpret_cast<const u32x4 *>(B.base + urow[i] + col);  \
-        }                                                                                  \
-      }                                                                                    \
-    }                                                                                      \
-  }
-
-  uint32_t acc[V / 2];
-  uint32_t tot[V];
-#pragma unroll
-  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
-#pragma unroll
-  for (int i = 0; i < V; ++i) tot[i] = 0;
-
-  HAMD_PF_PREFETCH_DLY(0)
-  HAMD_PF_LOAD(0)
-  if (CC < nlive) HAMD_PF_PREFETCH_DLY(CC)
-  int since_flush = 0;
-  for (int k0 = 0; k0 < nlive; k0 += CC) {
+  auto dly = [&](int k0) {
     const int nc = min(CC, nlive - k0);
-    __syncthreads();  // everyone is done with the previous chunk
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int cc = min(wave + 4 * i, nc - 1);
+      const int32_t *dt = B.delta_t + (int64_t)(k0 + cc) * B.dt_stride + d0;
+      pb[i] = sload(dt, 0);
+      pl[i] = sload(dt, nlast);
+    }
+  };
+  auto load = [&](int k0) {
+    const int nc = min(CC, nlive - k0);
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int cc = min(wave + 4 * i, nc - 1);
+      pr[i] = gload(B.delta_t + (int64_t)(k0 + cc) * B.dt_stride + d0 + mylane);
+      int first = (int)B.start + t0 + pb[i];
+      first -= first >= ring ? ring : 0;
+      const int al = first & ~15;
+      ubias[i] = (first - al) - pb[i];
+      ualgn[i] = al;
+      un16[i] = ((first - al) + T + (pl[i] - pb[i]) + 4 + 15) >> 4;
+      urow[i] = (int64_t)sload(B.live, k0 + cc) * B.row_stride;
+#pragma unroll
+      for (int j = 0; j < UPL; ++j) {
+        const int u = min(j * 64 + lane, un16[i] - 1);
+        int col = al + 16 * u;
+        col -= col >= ring ? ring : 0;
+        ureg[i * UPL + j] = gload(reinterpret_cast<const u32x4 *>(B.base + urow[i] + col));
+      }
+    }
+  };
+  auto store = [&](int k0) {
+    const int nc = min(CC, nlive - k0);
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       const int cc = wave + 4 * i;
@@ -402,32 +413,35 @@ __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ 
         for (int u = UPL * 64 + lane; u < un16[i]; u += 64) {
           int col = ualgn[i] + 16 * u;
           col -= col >= ring ? ring : 0;
-          *reinterpret_cast<uint4 *>(dst + 16 * u) =
-              *reinterpret_cast<const uint4 *>(B.base + urow[i] + col);
+          *reinterpret_cast<u32x4 *>(dst + 16 * u) =
+              gload(reinterpret_cast<const u32x4 *>(B.base + urow[i] + col));
         }
-        if (lane < ND) srel[cc * ND + lane] = urel[i];
+        if (lane < ND) tab[cc * ND + lane] = tab_entry(cc * lrow, ubias[i] + pr[i]);
       }
     }
+  };
+
+  uint32_t acc[V / 2];
+  uint32_t tot[V];
+#pragma unroll
+  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i) tot[i] = 0;
+
+  dly(0);
+  load(0);
+  if (CC < nlive) dly(CC);
+  int since_flush = 0;
+  for (int k0 = 0; k0 < nlive; k0 += CC) {
+    const int nc = min(CC, nlive - k0);
+    __syncthreads();  // everyone is done with the previous chunk
+    store(k0);
     __syncthreads();
     if (k0 + CC < nlive) {
-      HAMD_PF_LOAD(k0 + CC)  // in flight during the sums below
-      if (k0 + 2 * CC < nlive) HAMD_PF_PREFETCH_DLY(k0 + 2 * CC)
+      load(k0 + CC);  // in flight during the sums below
+      if (k0 + 2 * CC < nlive) dly(k0 + 2 * CC);
     }
-    for (int cc = 0; cc < nc; ++cc) {
-      const int p = g * V + srel[cc * ND + dl];
-      const uint32_t *wrow = reinterpret_cast<const uint32_t *>(sdat + (size_t)cc * lrow) + (p >> 2);
-      const uint32_t s = (uint32_t)(p & 3);
-      const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
-      const uint32_t sel_o = sel_e + 0x00010001u;
-      uint32_t w[NW];
-#pragma unroll
-      for (int i = 0; i < NW; ++i) w[i] = wrow[i];
-#pragma unroll
-      for (int k = 0; k < V / 4; ++k) {
-        acc[2 * k] += perm(w[k + 1], w[k], sel_e);
-        acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
-      }
-    }
+    sum_chunk<ND>(acc, sdat, tab, nc, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + CC >= nlive) {
 #pragma unroll
@@ -442,8 +456,6 @@ __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ 
       since_flush = 0;
     }
   }
-#undef HAMD_PF_PREFETCH_DLY
-#undef HAMD_PF_LOAD
   store_out(B, d0 + dl, t0 + g * V, tot);
 }
 
@@ -452,7 +464,7 @@ void launch_pf(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
                hipStream_t s)
 {
   constexpr int ND = NT / G;
-  const size_t lds = (size_t)CC * lrow + (size_t)CC * ND * 4;
+  const size_t lds = (size_t)CC * lrow + (size_t)CC * ND * sizeof(uint2);
   const int upl = (lrow + 1023) / 1024;
   if (upl <= 1)
     hipLaunchKernelGGL((k_dedisp_pf<G, CC, 1>), dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams,
@@ -474,7 +486,7 @@ void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, i
     if (chunk == 8) launch_pf<G, 8>(d_beams, d_tiles, ntiles, lrow, s);
     else launch_pf<G, 16>(d_beams, d_tiles, ntiles, lrow, s);
   } else if (dma == 1) {
-    const size_t lds = (size_t)2 * chunk * lrow + (size_t)2 * chunk * ND * 4;
+    const size_t lds = (size_t)2 * chunk * lrow + (size_t)2 * chunk * ND * sizeof(uint2);
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles, lrow);
     };
@@ -482,7 +494,8 @@ void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, i
     else if (chunk == 16) go(k_dedisp_dma<G, 16>);
     else go(k_dedisp_dma<G, 32>);
   } else {
-    const size_t lds = (size_t)chunk * lrow + (size_t)chunk * ND * 4 + (size_t)chunk * 4;
+    const size_t lds = (size_t)chunk * lrow + (size_t)chunk * ND * 4 + (size_t)chunk * 4 +
+                       (size_t)chunk * ND * sizeof(uint2);
     hipLaunchKernelGGL(k_dedisp<G>, dim3((unsigned)ntiles), dim3(NT), lds, s, d_beams, d_tiles,
                        chunk, lrow);
   }
@@ -546,8 +559,10 @@ static int lrow_for(const DDTile &t, int max_spread)
 size_t dd_lds_bytes(const DDTile &t, int max_spread)
 {
   const int ND = NT / t.lanes_per_dm;
+  // window rows, the chunk's delays (register loader), a pad word per
+  // channel and the (address, selector) table of every (channel, trial)
   const size_t one = (size_t)t.chunk * lrow_for(t, max_spread) + (size_t)t.chunk * ND * 4 +
-                     (size_t)t.chunk * 4;
+                     (size_t)t.chunk * 4 + (size_t)t.chunk * ND * 8;
   return t.dma == 1 ? 2 * one : one;
 }
 

@@ -93,6 +93,8 @@ struct DDBeamDev {
   int64_t start;
   const int32_t *delta;  // [ndm][nlive] delays of live channels
   const int32_t *live;   // [nlive] channel index of each live channel
+  const int32_t *delta_t;  // channel-major delays: delta_t[c * dt_stride + d] = delta[d][c]
+  int dt_stride;           // (from this beam's first trial; >= ndm + 64, padded)
   int nlive;
   int ndm;
   int nout;              // output samples per DM row
@@ -100,6 +102,12 @@ struct DDBeamDev {
   void *out;
   int64_t out_stride;    // elements between DM rows
 };
+
+// Channel-major copy of a [ndm][nlive] delay table: [nlive][stride] with
+// stride = ndm + 64, the pad repeating each channel's last delay, so any
+// run of up to 64 consecutive trials can be read.
+void delays_transposed(const std::vector<int32_t> &delta, int ndm, int nlive,
+                       std::vector<int32_t> &out, int &stride);
 
 // One workgroup of work: beam, first DM trial, first output sample.  beam < 0
 // marks a padding entry.

@@ -165,6 +165,11 @@ void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s)
                          hipMemcpyHostToDevice));
     B.delta = B.d_delta.as<int32_t>();
     B.live = B.d_live.as<int32_t>();
+    std::vector<int32_t> dt;
+    delays_transposed(B.delta_host, B.ndm, B.nlive, dt, B.dt_stride);
+    B.d_delta_t.ensure(dt.size() * 4);
+    HAMD_CHECK(hipMemcpy(B.d_delta_t.p, dt.data(), dt.size() * 4, hipMemcpyHostToDevice));
+    B.delta_t = B.d_delta_t.as<int32_t>();
   }
   B.f_bp.ensure((size_t)B.nch * 4);
   alloc_zero(B.f_scal, sizeof(FScal), s);
@@ -259,17 +264,18 @@ void WorkSet::ensure(const WorkDims &d)
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
 {
   // measured on MI355X (1024 ch, DM 0-2000): 32 lanes per DM trial, 8-channel
-  // chunks staged through registers, time-major tile order; wider lane
-  // groups take over when the delay spread of 8 DM trials does not fit
-  DDTile best{64, 8, 0};
+  // chunks, the next chunk prefetched in registers, time-major tile order;
+  // wider lane groups take over when the delay spread of 8 DM trials does not
+  // fit (dd_launch falls back to the register loader for short rings)
+  DDTile best{64, 8, 2};
   for (int G : {32, 64}) {
     bool fits = true;
     for (const BeamState *B : beams) {
       if (B->nlive == 0) continue;
       const int sp = dd_max_spread(B->delta_host.data(), B->ndm, B->nlive, 256 / G);
-      if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
+      if (dd_lds_bytes(DDTile{G, 8, 2}, sp) > DD_LDS_BUDGET) { fits = false; break; }
     }
-    if (fits) { best = DDTile{G, 8, 0}; break; }
+    if (fits) { best = DDTile{G, 8, 2}; break; }
   }
   tile = best;
   order = 1;
@@ -436,6 +442,8 @@ DDBeamDev make_ddesc(const Config &cfg, int W, BeamState &B, WorkSet &w)
   d.start = window_start(cfg, W, B);
   d.delta = B.delta;
   d.live = B.live;
+  d.delta_t = B.delta_t;
+  d.dt_stride = B.dt_stride;
   d.nlive = B.nlive;
   d.ndm = B.ndm;
   d.nout = B.ntime_dd;

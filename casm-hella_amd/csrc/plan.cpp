@@ -79,4 +79,16 @@ void PlanHost::delays_live(size_t d_first, size_t nd, std::vector<int32_t> &delt
       delta[d * live.size() + i] = delay_of(d_first + d, (size_t)live[i]);
 }
 
+void delays_transposed(const std::vector<int32_t> &delta, int ndm, int nlive,
+                       std::vector<int32_t> &out, int &stride)
+{
+  stride = ndm + 64;
+  out.assign((size_t)nlive * stride, 0);
+  for (int c = 0; c < nlive; ++c) {
+    int32_t *row = out.data() + (size_t)c * stride;
+    for (int d = 0; d < ndm; ++d) row[d] = delta[(size_t)d * nlive + c];
+    for (int d = ndm; d < stride; ++d) row[d] = ndm > 0 ? row[ndm - 1] : 0;
+  }
+}
+
 }  // namespace hamd

@@ -211,7 +211,8 @@ void footprint(const BeamState &g, const WorkDims &dims, size_t chunk_bytes,
 {
   beam_bytes = resident_bytes(g);
   const size_t per = WorkSet::bytes_for(dims) + round_up(chunk_bytes, 256);
-  const size_t tables = (size_t)g.ndm * g.nlive * 4 + (size_t)g.nlive * 4;
+  const size_t tables = (size_t)g.ndm * g.nlive * 4 + (size_t)g.nlive * 4 +
+                        (size_t)g.nlive * g.dt_stride * 4;
   work_bytes = (size_t)o.lanes * o.max_batch * per + tables;
 }
 
@@ -266,6 +267,8 @@ std::unique_ptr<BeamState> new_resident(hella_server s)
     B->plan = g.plan;
     B->delta = g.delta;
     B->live = g.live;
+    B->delta_t = g.delta_t;
+    B->dt_stride = g.dt_stride;
     B->nlive = g.nlive;
     B->ndm = g.ndm;
     B->md = g.md;
