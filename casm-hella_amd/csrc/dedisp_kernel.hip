@@ -101,13 +101,15 @@ __device__ __forceinline__ uint2 tab_entry(int row_byte, int q)
   return make_uint2((uint32_t)(row_byte + (q & ~3)), s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u);
 }
 
-// add the nc staged channels of a chunk to a thread's V sums (u16x2 lanes)
+// add the nc staged channels of a chunk to a thread's V sums (u16x2 lanes).
+// Channels go in pairs; the next pair's table entries are read while the
+// current pair is summed, so only the window reads wait on LDS.  tab holds
+// at least cap entries per trial column (reads past nc are never used).
 template <int ND>
 __device__ __forceinline__ void sum_chunk(uint32_t (&acc)[V / 2], const uint8_t *sdat,
-                                          const uint2 *tab, int nc, int dl, int g)
+                                          const uint2 *tab, int nc, int cap, int dl, int g)
 {
-  for (int cc = 0; cc < nc; ++cc) {
-    const uint2 e = tab[cc * ND + dl];
+  auto one = [&](uint2 e) {
     const uint32_t *wrow = reinterpret_cast<const uint32_t *>(sdat + e.x) + g * (V / 4);
     const uint32_t sel_o = e.y + 0x00010001u;
     uint32_t w[NW];
@@ -118,7 +120,18 @@ __device__ __forceinline__ void sum_chunk(uint32_t (&acc)[V / 2], const uint8_t 
       acc[2 * k] += perm(w[k + 1], w[k], e.y);
       acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
     }
+  };
+  uint2 e0 = tab[dl], e1 = tab[min(1, cap - 1) * ND + dl];
+  int cc = 0;
+  for (; cc + 1 < nc; cc += 2) {
+    const uint2 n0 = tab[min(cc + 2, cap - 1) * ND + dl];
+    const uint2 n1 = tab[min(cc + 3, cap - 1) * ND + dl];
+    one(e0);
+    one(e1);
+    e0 = n0;
+    e1 = n1;
   }
+  if (cc < nc) one(e0);
 }
 
 template <int G>
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_dedisp(const DDBeamDev *__restrict__ bea
       if (lane < ND) tab[cc * ND + lane] = tab_entry(cc * lrow, off + sdly[cc * ND + lane] - base_d);
     }
     __syncthreads();
-    sum_chunk<ND>(acc, sdat, tab, nc, dl, g);
+    sum_chunk<ND>(acc, sdat, tab, nc, chunk, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + chunk >= nlive) {
 #pragma unroll
@@ -308,7 +321,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       stage(k0 + chunk, buf ^ 1);
       if (k0 + 2 * chunk < nlive) prefetch(k0 + 2 * chunk);
     }
-    sum_chunk<ND>(acc, sdat, tab + (size_t)buf * chunk * ND, nc, dl, g);
+    sum_chunk<ND>(acc, sdat, tab + (size_t)buf * chunk * ND, nc, chunk, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + chunk >= nlive) {
 #pragma unroll
@@ -441,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_pf(const DDBeamDev *__restrict__ 
       load(k0 + CC);  // in flight during the sums below
       if (k0 + 2 * CC < nlive) dly(k0 + 2 * CC);
     }
-    sum_chunk<ND>(acc, sdat, tab, nc, dl, g);
+    sum_chunk<ND>(acc, sdat, tab, nc, CC, dl, g);
     since_flush += nc;
     if (since_flush >= 256 || k0 + CC >= nlive) {
 #pragma unroll
