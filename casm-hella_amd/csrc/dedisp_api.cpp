@@ -38,13 +38,13 @@ namespace hamd {
 DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
                       int lanes_pref, size_t lds_budget)
 {
-  // measured on MI355X: 32 lanes per trial with 8-channel chunks prefetched
-  // in registers is fastest; other lane counts when the spread does not fit
+  // measured on MI355X: 32 lanes per trial with 8-channel chunks staged
+  // through registers is fastest; other lane counts when the spread does not fit
   static const int gs[] = {32, 64, 16, 8};
   for (int G : gs) {
     if (G < lanes_pref) continue;
     const int spread = dd_max_spread(delta, ndm, nlive, 256 / G);
-    const DDTile t{G, 8, 2};
+    const DDTile t{G, 8, 0};
     if (dd_lds_bytes(t, spread) <= lds_budget) {
       *spread_out = spread;
       return t;

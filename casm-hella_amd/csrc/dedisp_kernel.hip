@@ -12,12 +12,12 @@
 // consecutive DM trials x a tile of T output samples.  It walks the live
 // channels in chunks of CC: each channel's window
 //   [t0 + delay(d_first,c), t0 + T + delay(d_last,c) + 20)
-// is staged in LDS with 16-byte loads, then every thread (one DM, V = 36
+// is staged in LDS with 16-byte loads, then every thread (one DM, V = 28
 // consecutive samples) adds its bytes.  Four samples of one channel come
 // from two LDS dwords; v_perm_b32 extracts the even and the odd samples into
 // two u16x2 lanes at the thread's byte alignment, so one 32-bit add performs
 // two sample additions.  u16 lanes hold at most 257*255, so they are widened
-// into u32 totals every 256 channels.  V/4 = 9 is odd, which spreads the
+// into u32 totals every 256 channels.  V/4 = 7 is odd, which spreads the
 // threads of one DM over all 32 LDS banks.
 #include <type_traits>
 
@@ -28,7 +28,7 @@ namespace hamd {
 namespace {
 
 constexpr int NT = 256;
-constexpr int V = 36;          // samples per thread
+constexpr int V = 28;          // samples per thread
 constexpr int NW = V / 4 + 1;  // LDS dwords read per channel per thread
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -272,6 +272,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       pr[i] = gload(dt + mylane);
     }
   };
+  // ring >= lrow + 16 (checked by the launcher): one subtraction wraps a column
   auto stage = [&](int k0, int buf) {
     const int nc = min(chunk, nlive - k0);
 #pragma unroll
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       const int base_d = pb[i];
       const int spread = pl[i] - base_d;
       int first = (int)B.start + t0 + base_d;
-      while (first >= ring) first -= ring;
+      first -= first >= ring ? ring : 0;
       const int al = first & ~15;
       const int off = first - al;
       const int n16 = (off + T + spread + 4 + 15) >> 4;
@@ -289,12 +290,11 @@ __global__ __launch_bounds__(NT) void k_dedisp_dma(const DDBeamDev *__restrict__
       uint8_t *dst = sdat + ((size_t)buf * chunk + cc) * lrow;
       for (int j = 0; j * 64 < n16; ++j) {
         const int u = j * 64 + lane;
-        if (u < n16) {
-          int col = al + 16 * u;
-          while (col >= ring) col -= ring;
+        int col = al + 16 * u;
+        col -= col >= ring ? ring : 0;
+        if (u < n16)
           __builtin_amdgcn_global_load_lds((glb_void_t *)(row + col), (lds_void_t *)(dst + 1024 * j),
                                            16, 0, 0);
-        }
       }
       if (lane < ND)
         tab[((size_t)buf * chunk + cc) * ND + lane] =
@@ -617,8 +617,9 @@ void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
     throw HipError{hipErrorInvalidValue, "dedispersion window exceeds LDS (delay spread too wide)",
                    __LINE__};
   const int lrow = lrow_for(t, max_spread);
-  // the register-prefetch loader wraps columns with a single subtraction
-  if (t.dma == 2 && (min_ring < (int64_t)lrow + 16 || t.chunk > 16)) t.dma = 0;
+  // the prefetch and DMA loaders wrap columns with a single subtraction
+  if (t.dma >= 1 && min_ring < (int64_t)lrow + 16) t.dma = 0;
+  if (t.dma == 2 && t.chunk > 16) t.dma = 0;
   switch (t.lanes_per_dm) {
     case 8: launch_g<8>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;
     case 16: launch_g<16>(d_beams, d_tiles, ntiles, t.chunk, lrow, t.dma, s); break;

@@ -263,19 +263,19 @@ void WorkSet::ensure(const WorkDims &d)
 
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
 {
-  // measured on MI355X (1024 ch, DM 0-2000): 32 lanes per DM trial, 8-channel
-  // chunks, the next chunk prefetched in registers, time-major tile order;
-  // wider lane groups take over when the delay spread of 8 DM trials does not
-  // fit (dd_launch falls back to the register loader for short rings)
-  DDTile best{64, 8, 2};
+  // measured on MI355X (1024 ch, DM 0-2000, 28 samples per thread): 32 lanes
+  // per DM trial, 8-channel chunks staged through registers, time-major tile
+  // order (the next-chunk prefetch loader is within 2%); wider lane groups
+  // take over when the delay spread of 8 DM trials does not fit
+  DDTile best{64, 8, 0};
   for (int G : {32, 64}) {
     bool fits = true;
     for (const BeamState *B : beams) {
       if (B->nlive == 0) continue;
       const int sp = dd_max_spread(B->delta_host.data(), B->ndm, B->nlive, 256 / G);
-      if (dd_lds_bytes(DDTile{G, 8, 2}, sp) > DD_LDS_BUDGET) { fits = false; break; }
+      if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
     }
-    if (fits) { best = DDTile{G, 8, 2}; break; }
+    if (fits) { best = DDTile{G, 8, 0}; break; }
   }
   tile = best;
   order = 1;
