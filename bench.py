@@ -37,6 +37,24 @@ PEAK_HBM_GBS = 8000.0
 PEAK_ADD_T = 78.6  # packed u16 integer adds/s (256 CU x 64 lanes x 2 x 2.4 GHz), 1e12
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_1024ch_dm2000_counters.md")
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel (or per chunk of a stage) from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload, or
+    None when the summary does not list it."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if line.startswith("machine-readable: traffic_bytes_per_launch") and parts[2] == name:
+                    return float(parts[3])
+    except OSError:
+        pass
+    return None
+
+
 def load_pkg():
     spec = importlib.util.spec_from_file_location(
         "hella_amd", os.path.join(ROOT, "casm-hella_amd", "hella_amd.py"))
@@ -236,7 +254,8 @@ def main():
                     help="beams of the saturation runs (plain vs batched server)")
     ap.add_argument("--serve-chunks", type=int, default=8, help="timed chunks per beam")
     ap.add_argument("--serve-rt-beams", type=int, default=256)
-    ap.add_argument("--serve-rt-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-rt-seconds", type=float, default=12.0,
+                    help="length of the real-time run (0 skips it)")
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
     args = ap.parse_args()
@@ -366,11 +385,13 @@ def main():
         "kernels_ms": {"dedisperse": ms_dd, "flagger": ms_flag, "search": ms_search},
         "roofline": {"bound": "valu", "kernel": "k_dedisp", "achieved": achieved_add,
                      "peak": PEAK_ADD_T, "unit": "Tadd/s", "frac": achieved_add / PEAK_ADD_T,
-                     "traffic": None},
+                     "traffic": profiled_traffic("k_dedisp"),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "roofline_flagger": {"bound": "hbm", "achieved": flag_bytes / (ms_flag * 1e-3) / 1e9,
                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": flag_bytes / (ms_flag * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                             "traffic": None},
+                             "traffic": profiled_traffic("flagger_chunk"),
+                             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
     }
     if not args.no_serve:
         out["serving"] = serving(hamd, args, pulses, rfi)
@@ -394,10 +415,12 @@ def serving(hamd, args, pulses, rfi):
         sat[mode] = serve_saturation(hamd, pool, mg, args.serve_beams, args.serve_chunks, **opts)
         print(f"# serving {mode}: {sat[mode]['chunks_per_s']:.0f} chunks/s -> "
               f"{sat[mode]['realtime_beams']} real-time beams", file=sys.stderr, flush=True)
-    rt = serve_realtime(hamd, pool, mg, args.serve_rt_beams, args.serve_rt_seconds,
-                        max_batch=args.serve_batch, lanes=args.serve_lanes)
-    print(f"# serving real time: {rt['beams']} beams, p50 {rt['latency_ms_p50']} ms, "
-          f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
+    rt = None
+    if args.serve_rt_seconds > 0:
+        rt = serve_realtime(hamd, pool, mg, args.serve_rt_beams, args.serve_rt_seconds,
+                            max_batch=args.serve_batch, lanes=args.serve_lanes)
+        print(f"# serving real time: {rt['beams']} beams, p50 {rt['latency_ms_p50']} ms, "
+              f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
     return {"workload": f"streaming: 1024 ch, {SERVE_CHUNK}-sample chunks "
                         f"({SERVE_CHUNK * TSAMP:.3f} s), DM 0-2000, widths 1..256, "
                         f"HBM budget {SERVE_BUDGET >> 30} GB",
