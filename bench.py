@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--tile", type=str, default=None,
+                    help="force the dedispersion tile of the plain path: lanes,chunk,order,loader")
     ap.add_argument("--plain-only", action="store_true",
                     help="skip the batched path and the CPU baseline (profiling runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -272,6 +274,9 @@ def main():
         print(json.dumps({"serving": serving(hamd, args, pulses, rfi)}), flush=True)
         return
     h = hamd.Hella(config(1))
+    if args.tile:
+        lanes, chunk, order, loader = (int(v) for v in args.tile.split(","))
+        h.set_tile(lanes, chunk, order | (loader << 1))
     info = h.info(0)
     need = max(args.warmup, info["min_gulp"] + 1)
     nchunks = need + args.steps

@@ -354,13 +354,16 @@ hella_status hella_set_tile(hella_ctx c, int lanes_per_dm, int chunk, int order)
 {
   if (!c) return fail(HELLA_ERR_INVALID, "ctx is NULL");
   if (c->built) return fail(HELLA_ERR_STATE, "set the tile before the first gulp");
-  if (!(lanes_per_dm == 8 || lanes_per_dm == 16 || lanes_per_dm == 32 || lanes_per_dm == 64) ||
-      !(chunk == 8 || chunk == 16 || chunk == 32) || order < 0 || order > 5 ||
-      (order >= 4 && chunk == 32))
-    return fail(HELLA_ERR_INVALID, "tile shape out of range");
-  // order bits 1-2: loader (0 LDS DMA, 1 registers, 2 register prefetch)
+  // order bits 1-2: loader (0 LDS DMA, 1 registers, 2 register prefetch,
+  // 3 one DM trial per wave: 64 lanes, chunk = 4 or 8 waves per workgroup)
   const int loader = (order >> 1) & 3;
-  c->tile = DDTile{lanes_per_dm, chunk, loader == 0 ? 1 : (loader == 1 ? 0 : 2)};
+  const bool ok = loader == 3
+                      ? lanes_per_dm == 64 && (chunk == 4 || chunk == 8)
+                      : (lanes_per_dm == 8 || lanes_per_dm == 16 || lanes_per_dm == 32 ||
+                         lanes_per_dm == 64) &&
+                            (chunk == 8 || chunk == 16 || chunk == 32) && !(loader == 2 && chunk == 32);
+  if (!ok || order < 0 || order > 7) return fail(HELLA_ERR_INVALID, "tile shape out of range");
+  c->tile = DDTile{lanes_per_dm, chunk, loader == 0 ? 1 : loader == 1 ? 0 : loader};
   c->tile_order = order & 1;
   c->tile_forced = true;
   return HELLA_OK;

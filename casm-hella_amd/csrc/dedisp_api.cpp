@@ -66,7 +66,7 @@ dedisp_error refresh_device(dedisp_plan plan)
   if (plan->nlive > 0) {
     plan->tile = choose_dd_tile(plan->delta_host.data(), ndm, plan->nlive, &plan->max_spread,
                                 16, DD_LDS_BUDGET);
-    plan->d_delta.ensure(plan->delta_host.size() * 4);
+    plan->d_delta.ensure((plan->delta_host.size() + DD_DELTA_PAD) * 4);
     plan->d_live.ensure(plan->live_host.size() * 4);
     HAMD_CHECK(hipMemcpy(plan->d_delta.p, plan->delta_host.data(), plan->delta_host.size() * 4,
                          hipMemcpyHostToDevice));

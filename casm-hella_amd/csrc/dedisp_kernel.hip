@@ -490,6 +490,198 @@ void launch_pf(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
                        d_tiles, lrow);
 }
 
+// One DM trial per wave.  A workgroup of NDW waves owns NDW consecutive
+// trials x T = 64 * V output samples.  Every delay a wave needs is uniform
+// across the wave, so the scalar unit loads the delays (one scalar load per
+// chunk from the DM-major table) and derives each channel's LDS offset and
+// v_perm selectors; the vector ALU does little besides the additions.  Wave
+// w stages channel k0 + w of every NDW-channel chunk: NU 16-byte units per
+// lane (NU * 1024 >= lrow) are read into registers one chunk ahead through a
+// buffer resource spanning the channel's ring row, and written to the other
+// of two LDS buffers once the current chunk is summed, so one barrier per
+// chunk suffices.  Channels past the last live one stage zeros, so every
+// chunk is a full NDW channels and the sums are straight-line code.
+// Requirements: ring % 16 == 0 (every beam's ring is a multiple of 32 or
+// 64 columns) and the DM-major delay table readable for DD_DELTA_PAD
+// entries past its end.
+template <int NDW, int NU>
+__global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restrict__ beams,
+                                                     const DDTileDesc *__restrict__ tiles,
+                                                     int lrow)
+{
+  constexpr int CC = NDW;
+  constexpr int T = 64 * V;
+  typedef int32_t dvec __attribute__((ext_vector_type(CC), aligned(4)));
+  typedef __attribute__((address_space(4))) const dvec cdvec;
+  extern __shared__ __align__(16) uint8_t lds[];  // [2][CC][lrow]
+  const DDTileDesc td = tiles[blockIdx.x];
+  if (td.beam < 0) return;
+  const DDBeamDev B = beams[td.beam];
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int d0 = td.d0, t0 = td.t0;
+  const int ndm = B.ndm, nlive = B.nlive;
+  const int ring = (int)B.ring;
+  // ring column of output sample t0 (window starts stay below 2 * ring)
+  const int S = (int)B.start + t0;
+  const int32_t *own_row = B.delta + (int64_t)min(d0 + wave, ndm - 1) * nlive;
+  const int32_t *first_row = B.delta + (int64_t)d0 * nlive;
+  const int32_t *last_row = B.delta + (int64_t)min(d0 + NDW - 1, ndm - 1) * nlive;
+
+  // staging: scalar parameters of this wave's channel of a chunk, then its
+  // window in registers
+  int sb = 0, sl = 0, sch = 0;
+  auto stage_params = [&](int k) {
+    const int kk = min(k, nlive - 1);
+    sb = sload(first_row, kk);
+    sl = sload(last_row, kk);
+    sch = k < nlive ? sload(B.live, kk) : -1;
+  };
+  (void)sl;
+  u32x4 P[NU];
+  const int lane16 = lane * 16;
+  auto fetch = [&]() {
+    if (sch >= 0) {
+      int first = S + sb;
+      first -= first >= ring ? ring : 0;
+      const int al = first & ~15;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(B.base + (int64_t)sch * B.row_stride), (short)0, ring, 0x00020000);
+      if (al + NU * 1024 <= ring) {
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+          P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, al + lane16 + u * 1024, 0, 0);
+      } else {
+        // the staged units run past the ring end (short rings may wrap more
+        // than once)
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          int col = al + lane16 + u * 1024;
+          while (col >= ring) col -= ring;
+          P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) P[u] = u32x4{0u, 0u, 0u, 0u};
+    }
+  };
+  auto store = [&](int buf) {
+    uint8_t *dst = lds + (size_t)(buf * CC + wave) * lrow + lane16;
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+      if (u + 1 < NU || lane16 + u * 1024 < lrow) *reinterpret_cast<u32x4 *>(dst + u * 1024) = P[u];
+  };
+
+  // this wave's trial delay and the block's first-trial delay of the CC
+  // channels of a chunk (scalar registers); past the last live channel the
+  // two are made equal, so the zero rows are read from their start
+  int32_t own[CC], bas[CC];
+  auto entries = [&](int k0) {
+    const dvec o = *(cdvec *)(own_row + k0);
+    const dvec f = *(cdvec *)(first_row + k0);
+#pragma unroll
+    for (int cc = 0; cc < CC; ++cc) {
+      own[cc] = o[cc];
+      bas[cc] = f[cc];
+    }
+    if (k0 + CC > nlive) {
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc)
+        if (k0 + cc >= nlive) own[cc] = bas[cc];
+    }
+  };
+
+  uint32_t acc[V / 2];
+  uint32_t tot[V];
+#pragma unroll
+  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i) tot[i] = 0;
+
+  const int lane_off = lane * V;  // byte offset of the lane's first sample
+  auto one = [&](int buf, int cc) {
+    // window offset of this trial in the staged row: the row starts at the
+    // 16-byte aligned column below the first trial's start
+    const int q = ((S + bas[cc]) & 15) + own[cc] - bas[cc];
+    const uint32_t s = (uint32_t)(q & 3);
+    const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
+    const uint32_t sel_o = sel_e + 0x00010001u;
+    const uint32_t *wrow = reinterpret_cast<const uint32_t *>(
+        lds + (buf * CC + cc) * lrow + (q & ~3) + lane_off);
+    uint32_t w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = wrow[i];
+#pragma unroll
+    for (int k = 0; k < V / 4; ++k) {
+      acc[2 * k] += perm(w[k + 1], w[k], sel_e);
+      acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
+    }
+  };
+
+  const int nchunks = (nlive + CC - 1) / CC;
+  stage_params(wave);
+  fetch();
+  entries(0);
+  store(0);
+  if (CC < nlive) stage_params(CC + wave);
+  __syncthreads();
+  int since_flush = 0;
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int buf = ci & 1;
+    const bool more = ci + 1 < nchunks;
+    if (more) fetch();  // the next chunk's window, in flight during the sums
+#pragma unroll
+    for (int cc = 0; cc < CC; cc += 2) {
+      one(buf, cc);
+      one(buf, cc + 1);
+    }
+    since_flush += CC;
+    if (since_flush >= 256 || !more) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        tot[4 * k + 0] += acc[2 * k] & 0xffffu;
+        tot[4 * k + 2] += acc[2 * k] >> 16;
+        tot[4 * k + 1] += acc[2 * k + 1] & 0xffffu;
+        tot[4 * k + 3] += acc[2 * k + 1] >> 16;
+        acc[2 * k] = 0;
+        acc[2 * k + 1] = 0;
+      }
+      since_flush = 0;
+    }
+    if (more) {
+      entries((ci + 1) * CC);
+      if (ci + 2 < nchunks) stage_params((ci + 2) * CC + wave);
+      store(buf ^ 1);
+    }
+    __syncthreads();
+  }
+  store_out(B, d0 + wave, t0 + lane * V, tot);
+}
+
+template <int NDW>
+void launch_w(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int lrow,
+              hipStream_t s)
+{
+  const size_t lds = (size_t)2 * NDW * lrow;
+  const int nu = (lrow + 1023) / 1024;
+  auto go = [&](auto kern) {
+    if (lds > 64 * 1024)
+      HAMD_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(64 * NDW), lds, s, d_beams, d_tiles,
+                       lrow);
+  };
+  switch (nu < 2 ? 2 : nu) {
+    case 2: go(k_dedisp_w<NDW, 2>); break;
+    case 3: go(k_dedisp_w<NDW, 3>); break;
+    case 4: go(k_dedisp_w<NDW, 4>); break;
+    case 5: go(k_dedisp_w<NDW, 5>); break;
+    case 6: go(k_dedisp_w<NDW, 6>); break;
+    default: throw HipError{hipErrorInvalidValue, "dedispersion window too wide for k_dedisp_w", __LINE__};
+  }
+}
+
 template <int G>
 void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int chunk,
               int lrow, int dma, hipStream_t s)
@@ -571,6 +763,7 @@ static int lrow_for(const DDTile &t, int max_spread)
 
 size_t dd_lds_bytes(const DDTile &t, int max_spread)
 {
+  if (t.dma == 3) return (size_t)2 * t.chunk * lrow_for(t, max_spread);
   const int ND = NT / t.lanes_per_dm;
   // window rows, the chunk's delays (register loader), a pad word per
   // channel and the (address, selector) table of every (channel, trial)
@@ -604,10 +797,47 @@ void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
   for (size_t k = 0; k < list.size(); ++k) tiles[(k % q) * nx + k / q] = list[k];
 }
 
+size_t dd_lds_budget(const DDTile &t)
+{
+  if (t.dma != 3) return DD_LDS_BUDGET;
+  // the one-trial-per-wave kernel may use the device's whole per-workgroup LDS
+  static size_t limit = 0;
+  if (limit == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      limit = (size_t)v;
+    else
+      limit = DD_LDS_BUDGET;
+  }
+  return limit;
+}
+
+bool dd_tile_fits(const DDTile &t, int max_spread, int64_t min_ring)
+{
+  if (t.dma != 3) return dd_lds_bytes(t, max_spread) <= dd_lds_budget(t);
+  const int lrow = lrow_for(t, max_spread);
+  (void)min_ring;  // windows wrap any number of times
+  return t.lanes_per_dm == 64 && (t.chunk == 4 || t.chunk == 8) &&
+         dd_lds_bytes(t, max_spread) <= dd_lds_budget(t) && lrow <= 6 * 1024;
+}
+
 void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, const DDTile &t_in,
                int max_spread, hipStream_t s, int64_t min_ring)
 {
   if (ntiles <= 0) return;
+  if (t_in.dma == 3) {
+    // the DM block of the tile table is the workgroup's wave count: no
+    // shape can be changed here, so a shape that does not fit is an error
+    if (!dd_tile_fits(t_in, max_spread, min_ring))
+      throw HipError{hipErrorInvalidValue, "dedispersion tile (one trial per wave) does not fit",
+                     __LINE__};
+    const int lrow = lrow_for(t_in, max_spread);
+    if (t_in.chunk == 4) launch_w<4>(d_beams, d_tiles, ntiles, lrow, s);
+    else launch_w<8>(d_beams, d_tiles, ntiles, lrow, s);
+    HAMD_CHECK(hipGetLastError());
+    return;
+  }
   DDTile t = t_in;
   // shrink the channel chunk (then drop the second DMA buffer) until the
   // workgroup's LDS fits the launch limit

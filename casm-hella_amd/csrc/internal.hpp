@@ -115,22 +115,32 @@ struct DDTileDesc {
   int beam, d0, t0, pad;
 };
 
+// The DM-major delay table [ndm][nlive] of a beam is followed by at least
+// this many readable int32 (vector scalar loads may run past its end).
+constexpr int DD_DELTA_PAD = 16;
+
 // Tile shape of the dedispersion kernel.
 struct DDTile {
   int lanes_per_dm;  // G: threads sharing one DM trial (time groups); ND = 256 / G
   int chunk;         // CC: channels staged in LDS per step (divides 256)
   int dma = 1;       // 0: loads through registers; 1: double-buffered global->LDS DMA;
-                     // 2: next chunk prefetched in registers (chunk 8 or 16)
+                     // 2: next chunk prefetched in registers (chunk 8 or 16);
+                     // 3: one DM trial per wave (lanes_per_dm 64): chunk = 4 or 8 waves,
+                     //    i.e. trials per workgroup, scalar delay arithmetic
 };
 
 int dd_samples_per_thread();
 // LDS bytes a dedispersion workgroup may use (the per-workgroup launch limit)
 constexpr size_t DD_LDS_BUDGET = 64 * 1024;
 inline int dd_tile_T(const DDTile &t) { return t.lanes_per_dm * dd_samples_per_thread(); }
-inline int dd_tile_ND(const DDTile &t) { return 256 / t.lanes_per_dm; }
+inline int dd_tile_ND(const DDTile &t) { return t.dma == 3 ? t.chunk : 256 / t.lanes_per_dm; }
 // largest per-channel spread of delays inside any block of nd consecutive DMs
 int dd_max_spread(const int32_t *delta, int ndm, int nlive, int nd);
 size_t dd_lds_bytes(const DDTile &t, int max_spread);
+// LDS a workgroup of this tile kind may use
+size_t dd_lds_budget(const DDTile &t);
+// the tile shape can run on windows of this spread and rings of this length
+bool dd_tile_fits(const DDTile &t, int max_spread, int64_t min_ring);
 // Tile table over beams: entries ordered so that the workgroups that the
 // dispatcher sends to one XCD cover consecutive time tiles of one DM block.
 // order 0: DM-block-major (consecutive time tiles of one DM block per XCD run);

@@ -157,7 +157,7 @@ void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s)
   B.dstride = (int64_t)round_up((size_t)B.ntime_dd, 64);
   B.in_bytes = (int64_t)W * B.nch * input_esize(cfg);
   if (B.nlive > 0 && !B.delta) {
-    B.d_delta.ensure(B.delta_host.size() * 4);
+    B.d_delta.ensure((B.delta_host.size() + DD_DELTA_PAD) * 4);
     B.d_live.ensure(B.live_host.size() * 4);
     HAMD_CHECK(hipMemcpy(B.d_delta.p, B.delta_host.data(), B.delta_host.size() * 4,
                          hipMemcpyHostToDevice));
@@ -925,19 +925,22 @@ void ctx_autotune(hella_ctx c, const char *path)
   HAMD_CHECK(hipEventCreate(&e0));
   HAMD_CHECK(hipEventCreate(&e1));
   DevBuf d_tiles;
+  std::vector<DDTile> shapes;
+  for (int cc : {8, 4}) shapes.push_back(DDTile{64, cc, 3});
   for (int dma : {2, 1, 0})
-  for (int G : {8, 16, 32, 64})
-    for (int cc : {8, 16, 32})
+    for (int G : {8, 16, 32, 64})
+      for (int cc : {8, 16, 32})
+        if (!(dma == 2 && cc == 32)) shapes.push_back(DDTile{G, cc, dma});
+  for (const DDTile &t : shapes)
       for (int order : {0, 1}) {
-        if (dma == 2 && cc == 32) continue;
-        DDTile t{G, cc, dma};
+        const int G = t.lanes_per_dm, cc = t.chunk, dma = t.dma;
         int spread = 0;
         bool fits = true;
         for (auto &B : c->beams) {
           if (!B.nlive) continue;
-          const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, 256 / G);
+          const int sp = dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(t));
           spread = std::max(spread, sp);
-          if (dd_lds_bytes(t, sp) > DD_LDS_BUDGET) fits = false;
+          if (!dd_tile_fits(t, sp, min_ring)) fits = false;
         }
         if (!fits) continue;
         std::vector<DDTileDesc> tiles;

@@ -161,7 +161,8 @@ hella_status hella_autotune(hella_ctx ctx, const char *path);
 /* force a tile shape: lanes per DM trial (8/16/32/64), channels per LDS chunk
  * (8/16/32); order bit 0: tile order (0 DM-block-major, 1 time-major); order
  * bits 1-2: window loader (0 LDS DMA, 1 through registers, 2 next chunk
- * prefetched in registers, chunk 8 or 16) */
+ * prefetched in registers, chunk 8 or 16; 3 one DM trial per wave: lanes 64,
+ * chunk 4 or 8 = DM trials per workgroup) */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
 
 /* ---- PSRDADA-style block ring (streaming ingest) -------------------------

@@ -123,11 +123,13 @@ def test_stream_windows(hamd, orc):
     assert searched == 3
 
 
-def _multibeam(hamd, batched, beams=(0, 1, 2)):
+def _multibeam(hamd, batched, beams=(0, 1, 2), tile=None):
     W = 2048
     cfg = _cfg(INPUT="FILTERBANK", GULP=W, NBEAM=len(beams), DM_MAX=80, WIDTH_MAX=8)
     geo = {0: dict(), 1: dict(NCHAN=32, DM_MAX=50), 2: dict(NCHAN=96, FOFF=-1.0, DM_MAX=120)}
     h = hamd.Hella(cfg)
+    if tile is not None:
+        h.set_tile(*tile)
     for j, b in enumerate(beams):
         if geo[b]:
             h.set_beam(j, **geo[b])
@@ -156,6 +158,17 @@ def test_batched_equals_plain(hamd):
         np.testing.assert_array_equal(pc, bc)
         total += len(pp)
     assert total > 0
+
+
+@pytest.mark.parametrize("chunk,order", [(8, 1), (4, 0)])
+def test_wave_per_trial_batched(hamd, chunk, order):
+    """The one-trial-per-wave kernel in the batched graph (uneven beams, a
+    killmask leaving 93 live channels) finds what the plain default path finds."""
+    plain = _multibeam(hamd, False)
+    batched = _multibeam(hamd, True, tile=(64, chunk, order | (3 << 1)))
+    for (pp, pc), (bp, bc) in zip(plain, batched):
+        np.testing.assert_array_equal(pp, bp)
+        np.testing.assert_array_equal(pc, bc)
 
 
 def test_alone_equals_mixed(hamd):
@@ -192,6 +205,35 @@ def test_injection_recovery_c1(hamd, orc):
     assert best["snr"] > cfg["SNR"]
     # same answer from the oracle on the same plane
     _check_search(orc, h, cfg, h.dedisp_plane(0))
+
+
+@pytest.mark.parametrize("chunk,order,kill", [(8, 1, ()), (4, 0, ()), (8, 0, (3, 11, 40)),
+                                              (4, 1, (0, 63, 7))])
+def test_wave_per_trial_loader(hamd, orc, chunk, order, kill):
+    """One DM trial per wave (scalar delay arithmetic, double-buffered
+    staging): exact sums, also when the live channels do not fill the last
+    chunk and the DM count is not a multiple of the trials per workgroup."""
+    W = 4096
+    cfg = _cfg(INPUT="FILTERBANK", GULP=W, DM_MAX=140, WIDTH_MAX=8)
+    h = hamd.Hella(cfg)
+    km = np.ones(64, np.int32)
+    km[list(kill)] = 0
+    if kill:
+        h.set_killmask(0, km)
+    h.set_tile(64, chunk, order | (3 << 1))
+    i = h.info(0)
+    assert i["ndm"] % 8 != 0
+    op = _oracle_plan(orc, cfg)
+    flagged = []
+    for g in range(i["min_gulp"] + 2):
+        gulp = _synth(hamd, cfg, W, seed=7, t_offset=g * W, pulses=[(1.5, 90.0, 1.0, 30.0)])
+        h.process_gulp(gulp)
+        flagged.append(_oracle_flag(orc, cfg, gulp))
+    F = np.concatenate(flagged, axis=1)
+    a0 = (i["min_gulp"] + 2) * W - i["ntime"]
+    D = h.dedisp_plane(0)
+    np.testing.assert_array_equal(D, orc.dedisperse_ft(F[:, a0:a0 + i["ntime"]], op.delta,
+                                                       i["ntime_dd"], killmask=km))
 
 
 @pytest.mark.parametrize("lanes,chunk,order", [(16, 8, 0), (32, 8, 1), (64, 16, 1), (8, 32, 0)])
