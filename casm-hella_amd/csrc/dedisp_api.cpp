@@ -38,13 +38,22 @@ namespace hamd {
 DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
                       int lanes_pref, size_t lds_budget)
 {
-  // measured on MI355X: 32 lanes per trial with 8-channel chunks staged
-  // through registers is fastest; other lane counts when the spread does not fit
+  // measured on MI355X: one DM trial per wave in workgroups of 4 trials is
+  // fastest, then 32 lanes per trial with 8-channel chunks staged through
+  // registers; other lane counts when the spread does not fit
+  const DDTile w{64, 4, 3};
+  if (lanes_pref <= 64) {
+    const int spread = dd_max_spread(delta, ndm, nlive, dd_tile_ND(w));
+    if (dd_tile_fits(w, spread, INT64_MAX)) {
+      *spread_out = spread;
+      return w;
+    }
+  }
   static const int gs[] = {32, 64, 16, 8};
   for (int G : gs) {
     if (G < lanes_pref) continue;
-    const int spread = dd_max_spread(delta, ndm, nlive, 256 / G);
     const DDTile t{G, 8, 0};
+    const int spread = dd_max_spread(delta, ndm, nlive, dd_tile_ND(t));
     if (dd_lds_bytes(t, spread) <= lds_budget) {
       *spread_out = spread;
       return t;

@@ -263,22 +263,27 @@ void WorkSet::ensure(const WorkDims &d)
 
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
 {
-  // measured on MI355X (1024 ch, DM 0-2000, 28 samples per thread): 32 lanes
-  // per DM trial, 8-channel chunks staged through registers, time-major tile
-  // order (the next-chunk prefetch loader is within 2%); wider lane groups
-  // take over when the delay spread of 8 DM trials does not fit
-  DDTile best{64, 8, 0};
-  for (int G : {32, 64}) {
+  // measured on MI355X (1024 ch, DM 0-2000, 28 samples per thread): one DM
+  // trial per wave in workgroups of 4 trials, time-major tile order, 4.92 ms;
+  // then 32 lanes per trial with 8-channel chunks staged through registers
+  // (5.10 ms); wider lane groups take over when the delay spread of 8 DM
+  // trials does not fit
+  int64_t min_ring = INT64_MAX;
+  for (const BeamState *B : beams) min_ring = std::min(min_ring, B->ntime);
+  order = 1;
+  for (const DDTile t : {DDTile{64, 4, 3}, DDTile{32, 8, 0}, DDTile{64, 8, 0}}) {
     bool fits = true;
     for (const BeamState *B : beams) {
       if (B->nlive == 0) continue;
-      const int sp = dd_max_spread(B->delta_host.data(), B->ndm, B->nlive, 256 / G);
-      if (dd_lds_bytes(DDTile{G, 8, 0}, sp) > DD_LDS_BUDGET) { fits = false; break; }
+      const int sp = dd_max_spread(B->delta_host.data(), B->ndm, B->nlive, dd_tile_ND(t));
+      if (!dd_tile_fits(t, sp, min_ring)) { fits = false; break; }
     }
-    if (fits) { best = DDTile{G, 8, 0}; break; }
+    if (fits) {
+      tile = t;
+      return;
+    }
   }
-  tile = best;
-  order = 1;
+  tile = DDTile{64, 8, 0};
 }
 
 void build_plain_tiles(BeamState &B, const DDTile &t, int order)
