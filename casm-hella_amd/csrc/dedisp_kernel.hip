@@ -510,7 +510,6 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
                                                      int lrow)
 {
   constexpr int CC = NDW;
-  constexpr int T = 64 * V;
   typedef int32_t dvec __attribute__((ext_vector_type(CC), aligned(4)));
   typedef __attribute__((address_space(4))) const dvec cdvec;
   extern __shared__ __align__(16) uint8_t lds[];  // [2][CC][lrow]

@@ -305,6 +305,150 @@ __global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__r
   }
 }
 
+// ---- value-table path -------------------------------------------------
+// The same operations as k_apply, applied to one value of channel c; the
+// statistics (bp, mu, inv) are those k_apply would read in the same pass.
+__device__ __forceinline__ float value_op(int op, float x, float bpv, bool bad, bool fixed,
+                                          const FScal &sc, float a0, float a1, bool &g)
+{
+  g = false;
+  switch (op) {
+    case FOP_DIVBP:
+    case FOP_DIVBP_TS: return bpv > 0.f ? x / bpv : 0.f;
+    case FOP_NORM: return (float)(((double)x - sc.mu) * sc.inv);
+    case FOP_SPEC: return bad ? 1.f : x + 1.f;
+    case FOP_NORM_FLAG1: {
+      x = (float)(((double)x - sc.mu) * sc.inv);
+      g = (x > a0) || (x < a1);
+      if (g || fixed) x = 0.f;
+      return x + 1.f;
+    }
+    default: return x;  // FOP_NONE; FOP_NORM_QUANT acts after the time-series division
+  }
+}
+
+// u8 time-major gulp -> u8 channel-major plane X8
+__global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict__ beams, int W)
+{
+  const FlagBeamDev &B = beams[blockIdx.z];
+  const int c0 = blockIdx.y * 64;
+  const int t0 = blockIdx.x * 64;
+  if (c0 >= B.nch) return;
+  __shared__ uint8_t tile[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const uint8_t *in = static_cast<const uint8_t *>(B.in);
+  for (int r = ty; r < 64; r += 4) {
+    const int t = t0 + r, c = c0 + tx;
+    tile[r][tx] = (t < W && c < B.nch) ? in[(int64_t)t * B.in_stride + c] : (uint8_t)0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int c = c0 + r, t = t0 + tx;
+    if (c < B.nch && t < W) B.X8[(int64_t)c * B.xs + t] = tile[tx][r];
+  }
+}
+
+// One pass of the value-table path: the workgroup builds its channel's table
+// for this pass from the previous table (lin < 0: the identity) and the
+// current statistics, workgroup 0 of the channel stores it to slot lout, and
+// every sample of the block is looked up; FOP_DIVBP_TS and FOP_NORM_QUANT
+// then divide by ts[t], FOP_NORM_QUANT normalises and quantises into the
+// ring.  Samples, partial sums and flag counts are those of k_apply.
+__global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict__ beams, int W,
+                                                  int op, int part, int npart, float a0, float a1,
+                                                  int lin, int lout)
+{
+  const FlagBeamDev &B = beams[blockIdx.z];
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int blk = blockIdx.x;
+  const int l = threadIdx.x;
+  __shared__ float tab[256];
+  __shared__ uint8_t flg[256];
+  __shared__ double sh[2 * 128];
+  const FScal sc = *B.scal;
+  const float bpv = B.bp[c];
+  const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
+  const bool fixed = c >= B.flag1 && c < B.flag2;
+  {
+    const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+    bool g;
+    const float y = value_op(op, x0, bpv, bad, fixed, sc, a0, a1, g);
+    tab[l] = y;
+    flg[l] = g ? 1 : 0;
+    if (blk == 0 && lout >= 0 && lout != lin) B.lut[((int64_t)lout * B.nch + c) * 256 + l] = y;
+  }
+  __syncthreads();
+  const uint8_t *row = B.X8 + (int64_t)c * B.xs;
+  const int base = blk * FLAG_BLOCK;
+  const bool tsdiv = op == FOP_DIVBP_TS || op == FOP_NORM_QUANT;
+  uint32_t cnt = 0;
+  double acc[2] = {0.0, 0.0};
+#pragma unroll 4
+  for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+    const int t = base + l + NT * k;
+    if (t >= W) break;
+    const int v = row[t];
+    float x = tab[v];
+    cnt += flg[v];
+    if (tsdiv) x = x / B.ts[t];
+    if (op == FOP_NORM_QUANT) {
+      x = (float)(((double)x - sc.mu) * sc.inv);
+      int64_t col = B.col0 + t;
+      if (col >= B.ring_len) col -= B.ring_len;
+      B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
+    }
+    if (t < npart && part != FPART_NONE) {
+      const double d = (double)x;
+      acc[0] += d;
+      acc[1] += d * d;
+    }
+  }
+  if (cnt) atomicAdd(&B.flagcount[c], cnt);
+  if (part == FPART_NONE) return;
+  tree256_multi(acc, sh);
+  if (l == 0) {
+    B.part[(int64_t)c * B.nblk + blk] = acc[0];
+    if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk] = acc[1];
+  }
+}
+
+// k_ts_sum on the value-table path: x = table slot lin of channel c at the
+// sample's input byte
+__global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum_lut(const FlagBeamDev *__restrict__ beams,
+                                                               int W, int lin)
+{
+  const FlagBeamDev &B = beams[blockIdx.z];
+  __shared__ double sp[TS_PARTS][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + lane;
+  const int per = (B.nch + TS_PARTS - 1) / TS_PARTS;
+  const int c0 = min(B.nch, q * per), c1 = min(B.nch, c0 + per);
+  double s = 0.0;
+  if (t < W) {
+    const uint8_t *col = B.X8 + t;
+    const float *tabs = B.lut + (int64_t)lin * B.nch * 256;
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) {
+      const float b = B.bp[c];
+      const float x = tabs[(int64_t)c * 256 + col[(int64_t)c * B.xs]];
+      s += (double)(b > 0.f ? x / b : 0.f);
+    }
+  }
+  sp[q][lane] = s;
+  __syncthreads();
+  if (q == 0 && t < W) {
+    double p[TS_PARTS];
+#pragma unroll
+    for (int i = 0; i < TS_PARTS; ++i) p[i] = sp[i][lane];
+#pragma unroll
+    for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
+#pragma unroll
+      for (int i = 0; i < h; ++i) p[i] += p[i + h];
+    B.ts[t] = (float)(p[0] / (double)B.nch);
+  }
+}
+
 __global__ void k_zero_u32(uint32_t *p, int n)
 {
   const int i = blockIdx.x * NT + threadIdx.x;
@@ -315,8 +459,12 @@ __global__ void k_zero_u32(uint32_t *p, int n)
 
 // ------------------------------------------------------------------ host
 void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
-                     const FlagParams &P, hipStream_t s)
+                     const FlagParams &P, hipStream_t s, bool value_tables)
 {
+  if (value_tables) {
+    run_tables(d_beams, nbeam, max_nch, W, P, s);
+    return;
+  }
   const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
   const int n32 = (W / 32) * 32;
   int wp = (W / 256) * 256;
@@ -369,6 +517,58 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   apply(FOP_DIVBP_TS, FPART_SUM2, wp);
   stats();
   // transpose_output_handler (transpose.cu:120-145): quantise into the ring
+  apply(FOP_NORM_QUANT, FPART_NONE, 0);
+  HAMD_CHECK(hipGetLastError());
+}
+
+// The pass sequence of FlagRunner::run on the value-table path: the same
+// passes and statistics, each pass reading the uint8 plane through the
+// channel tables instead of reading and rewriting the FP32 plane.
+void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
+                            const FlagParams &P, hipStream_t s)
+{
+  const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
+  const int n32 = (W / 32) * 32;
+  int wp = (W / 256) * 256;
+  if (wp == 0) wp = W;
+  const dim3 gplane((unsigned)nblk, (unsigned)max_nch, (unsigned)nbeam);
+  const dim3 grow((unsigned)((max_nch + NT - 1) / NT), 1, (unsigned)nbeam);
+  const dim3 gbeam((unsigned)nbeam);
+  int cur = -1;  // table slot holding the current values (-1: the input bytes)
+  auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
+    const bool same = op == FOP_NONE || op == FOP_NORM_QUANT;
+    const int out = same ? cur : (cur == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_apply_lut, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0, a1,
+                       cur, out);
+    cur = out;
+  };
+  auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
+  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(NT), 0, s, d_beams, wp); };
+
+  hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
+                                       (unsigned)nbeam),
+                     dim3(256), 0, s, d_beams, W);
+  apply(FOP_NONE, FPART_SUM, n32);
+  bp();
+  apply(FOP_DIVBP, FPART_SUM2, wp);
+  stats();
+  apply(FOP_NORM, FPART_SUM, n32);
+  bp();
+  apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
+  bp();
+  for (int i = 0; i < P.nscrunch; ++i) {
+    const FlagScrunch &S = P.scrunch[i];
+    apply(FOP_DIVBP, FPART_SUM2, wp);
+    stats();
+    const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
+    const float nthr = (float)(-1. * (double)thr);
+    apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
+    bp();
+  }
+  hipLaunchKernelGGL(k_ts_sum_lut, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam),
+                     dim3(64 * TS_PARTS), 0, s, d_beams, W, cur);
+  apply(FOP_DIVBP_TS, FPART_SUM2, wp);
+  stats();
   apply(FOP_NORM_QUANT, FPART_NONE, 0);
   HAMD_CHECK(hipGetLastError());
 }

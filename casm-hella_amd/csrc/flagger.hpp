@@ -48,6 +48,11 @@ struct FlagBeamDev {
   uint32_t *flagcount; // [nch] grown-mask counts (accumulated)
   float *ts;           // [W] time series of the final pass
   uint8_t *mask;       // [nch][xs] threshold mask scratch
+  // value-table path (uint8 input, 1 x 1 scrunches only): the input as a
+  // uint8 channel-major plane, and two [nch][256] tables holding each
+  // channel's current value of every input byte
+  uint8_t *X8;         // [nch][xs]
+  float *lut;          // [2][nch][256]
 };
 
 struct FlagScrunch {
@@ -63,10 +68,37 @@ struct FlagParams {
 
 struct FlagRunner {
   // Runs the whole flagger for nbeam beams described by d_beams (device
-  // array); max_nch is the largest channel count among them.
+  // array); max_nch is the largest channel count among them.  value_tables:
+  // take the value-table path (every beam's input is uint8 and every
+  // scrunch is 1 x 1; see flag_value_tables).
   static void run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W, const FlagParams &P,
-                  hipStream_t s);
+                  hipStream_t s, bool value_tables = false);
+
+ private:
+  static void run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
+                         const FlagParams &P, hipStream_t s);
 };
+
+// Up to the time-series correction every flagger operation on uint8 input
+// with 1 x 1 scrunches maps a channel's sample value through a function of
+// (channel, input byte) alone, so the plane never has to be materialised:
+// each pass reads the uint8 plane and a 256-entry table per channel.
+inline bool flag_value_tables(int in_fmt, const FlagParams &P)
+{
+  if (in_fmt != FLAG_IN_U8_TF) return false;
+  for (int i = 0; i < P.nscrunch; ++i)
+    if (P.scrunch[i].tscrunch != 1 || P.scrunch[i].fscrunch != 1) return false;
+  return true;
+}
+// byte offset of the tables behind the uint8 plane in a work buffer
+inline size_t flag_table_offset(int nch, int64_t xs) { return ((size_t)nch * xs + 255) / 256 * 256; }
+// a box scrunch (ts x fs > 1) needs the mask plane
+inline bool flag_needs_mask(const FlagParams &P)
+{
+  for (int i = 0; i < P.nscrunch; ++i)
+    if (P.scrunch[i].tscrunch != 1 || P.scrunch[i].fscrunch != 1) return true;
+  return false;
+}
 
 void zero_u32(uint32_t *p, int n, hipStream_t s);
 

@@ -191,7 +191,8 @@ WorkDims beam_dims(const BeamState &B, const Config &cfg, int W, int nw)
   d.nout = B.nout;
   d.nw = nw;
   d.cand_cap = cfg.cand_cap;
-  d.mask = cfg.flag.nscrunch > 0;
+  d.mask = flag_needs_mask(cfg.flag);
+  d.tables = flag_value_tables(cfg.input_fmt, cfg.flag);
   d.planes = cfg.search_mode == SEARCH_HELLA;
   return d;
 }
@@ -209,6 +210,7 @@ void WorkDims::merge(const WorkDims &o)
   dstride = std::max(dstride, o.dstride);
   mask = mask || o.mask;
   planes = planes || o.planes;
+  tables = tables || o.tables;
 }
 
 struct WorkSizes {
@@ -221,7 +223,10 @@ static WorkSizes work_sizes(const WorkDims &d)
   const size_t nblk = (size_t)(d.W + FLAG_BLOCK - 1) / FLAG_BLOCK;
   const size_t sblk = (size_t)(d.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
   const size_t ncnt = (size_t)d.nw * d.rows * sblk;
-  z.X = (size_t)d.nch * d.xs * 4;
+  // FP32 plane, or (value tables) the uint8 plane and two [nch][256] tables;
+  // a batch mixing both paths cannot occur (one input format per context)
+  z.X = d.tables ? flag_table_offset(d.nch, d.xs) + (size_t)2 * d.nch * 256 * 4
+                 : (size_t)d.nch * d.xs * 4;
   z.mask = d.mask ? (size_t)d.nch * d.xs : 0;
   z.D = (size_t)d.ndm * d.dstride * 4;
   z.planes = d.planes ? (size_t)d.nw * d.rows * round_up((size_t)d.nout, 64) * 4 : 0;
@@ -435,6 +440,10 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
   f.flagcount = B.f_count.as<uint32_t>();
   f.ts = w.f_ts.as<float>();
   f.mask = w.mask.as<uint8_t>();
+  if (flag_value_tables(cfg.input_fmt, cfg.flag)) {
+    f.X8 = w.X.as<uint8_t>();
+    f.lut = reinterpret_cast<float *>(w.X.as<uint8_t>() + flag_table_offset(B.nch, f.xs));
+  }
   return f;
 }
 
@@ -521,7 +530,8 @@ void ctx_flag_beam(hella_ctx c, int b, const void *d_gulp)
   FlagBeamDev *dd = c->d_fdesc.as<FlagBeamDev>() + b;
   HAMD_CHECK(hipMemcpyAsync(dd, &c->h_fdesc[b], sizeof(FlagBeamDev), hipMemcpyHostToDevice,
                             c->stream));
-  FlagRunner::run(dd, 1, B.nch, c->W, c->cfg.flag, c->stream);
+  FlagRunner::run(dd, 1, B.nch, c->W, c->cfg.flag, c->stream,
+                  flag_value_tables(c->cfg.input_fmt, c->cfg.flag));
 }
 
 void ctx_dedisperse(hella_ctx c, int b)
@@ -686,7 +696,8 @@ static void batch_record(hella_ctx c)
   hipStream_t s = c->stream;
   HAMD_CHECK(hipMemcpyAsync(c->d_fdesc.p, c->h_fdesc, nb * sizeof(FlagBeamDev),
                             hipMemcpyHostToDevice, s));
-  FlagRunner::run(c->d_fdesc.as<FlagBeamDev>(), (int)nb, c->max_nch, c->W, c->cfg.flag, s);
+  FlagRunner::run(c->d_fdesc.as<FlagBeamDev>(), (int)nb, c->max_nch, c->W, c->cfg.flag, s,
+                  flag_value_tables(c->cfg.input_fmt, c->cfg.flag));
   const int nr = (int)bt.beams.size();
   if (nr == 0) return;
   HAMD_CHECK(hipMemcpyAsync(c->d_ddesc.p, c->h_ddesc, (size_t)nr * sizeof(DDBeamDev),
@@ -880,7 +891,8 @@ float ctx_bench_stage(hella_ctx c, int b, int which, int reps)
   hipStream_t s = c->stream;
   auto launch = [&]() {
     if (which == 0) {
-      FlagRunner::run(c->d_fdesc.as<FlagBeamDev>() + b, 1, B.nch, c->W, c->cfg.flag, s);
+      FlagRunner::run(c->d_fdesc.as<FlagBeamDev>() + b, 1, B.nch, c->W, c->cfg.flag, s,
+                      flag_value_tables(c->cfg.input_fmt, c->cfg.flag));
     } else if (which == 1) {
       if (B.nlive)
         dd_launch(c->d_ddesc.as<DDBeamDev>() + b, B.d_tiles.as<DDTileDesc>(), B.ntiles, c->tile,

@@ -74,6 +74,7 @@ struct WorkDims {
   int nch = 0, W = 0, ndm = 0, rows = 0, nout = 0, nw = 0, cand_cap = 0;
   int64_t xs = 0, dstride = 0;
   bool mask = false, planes = false;
+  bool tables = false;  // flagger value-table path: uint8 plane + tables instead of FP32
   void merge(const WorkDims &o);
 };
 

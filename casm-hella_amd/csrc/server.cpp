@@ -357,7 +357,8 @@ void record_flag(hella_server s, Lane &L, int n)
 {
   HAMD_CHECK(hipMemcpyAsync(L.d_fdesc.p, L.h_fdesc, (size_t)n * sizeof(FlagBeamDev),
                             hipMemcpyHostToDevice, L.stream));
-  FlagRunner::run(L.d_fdesc.as<FlagBeamDev>(), n, s->geom.nch, s->W, s->cfg.flag, L.stream);
+  FlagRunner::run(L.d_fdesc.as<FlagBeamDev>(), n, s->geom.nch, s->W, s->cfg.flag, L.stream,
+                  flag_value_tables(s->cfg.input_fmt, s->cfg.flag));
 }
 
 void record_search(hella_server s, Lane &L, int nr, const DDTileDesc *tiles, int ntiles)
@@ -457,7 +458,8 @@ void launch_plain(hella_server s, Lane &L)
   L.h_fdesc[0] = make_fdesc(s->cfg, s->W, B, w, din);
   HAMD_CHECK(hipMemcpyAsync(L.d_fdesc.p, L.h_fdesc, sizeof(FlagBeamDev), hipMemcpyHostToDevice,
                             L.stream));
-  FlagRunner::run(L.d_fdesc.as<FlagBeamDev>(), 1, B.nch, s->W, s->cfg.flag, L.stream);
+  FlagRunner::run(L.d_fdesc.as<FlagBeamDev>(), 1, B.nch, s->W, s->cfg.flag, L.stream,
+                  flag_value_tables(s->cfg.input_fmt, s->cfg.flag));
   if (j.searched) {
     L.h_ddesc[0] = make_ddesc(s->cfg, s->W, B, w);
     if (B.nlive == 0) {

@@ -56,10 +56,13 @@ def _check_search(orc, h, cfg, D, beam=0, mode=0):
     return ref
 
 
-@pytest.mark.parametrize("scr", [(), ((1, 1, 5.0), (4, 8, 3.0))])
+@pytest.mark.parametrize("scr", [(), ((1, 1, 5.0), (4, 8, 3.0)), ((1, 1, 5.0),),
+                                 ((1, 1, 4.0), (1, 1, 2.5))])
 @pytest.mark.parametrize("mode", ["BOXCAR", "HELLA"])
 def test_oneshot_chain(hamd, orc, scr, mode):
-    """FILE sizing: one window through flag -> dedisperse -> search."""
+    """FILE sizing: one window through flag -> dedisperse -> search.  Without
+    box scrunches the flagger takes the value-table path; with one it works
+    on the FP32 plane; both must give the oracle's bytes."""
     cfg = _cfg(INPUT="FILE", GULP=8192, FLAG1=10, FLAG2=12, SEARCH=mode)
     if scr:
         cfg["SCRUNCH"] = scr
