@@ -154,24 +154,44 @@ __global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict_
 // normalization.cu:125-162): S1_r, S2_r = ascending sums of row r's block
 // partials; mu = canonical sum over rows of S1_r / n, over the row count; the
 // variance is the same combination of ((S2_r - 2 mu S1_r) + n mu mu) / n.
-__device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool var, double *sh)
+// The row values of a 4096-row block are formed by all 1024 threads at once
+// (rowv), then lane l < 256 adds rows l + 256k in ascending k and the lanes
+// are folded by the canonical tree.
+constexpr int FS_NT = 1024;
+
+__device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool var, double *rowv,
+                                 double *sh)
 {
   const int l = threadIdx.x;
   const double dn = (double)n;
   double total = 0.0;
   for (int r0 = 0; r0 < B.nch; r0 += FLAG_BLOCK) {
+    for (int i = l; i < FLAG_BLOCK; i += FS_NT) {
+      const int r = r0 + i;
+      double val = 0.0;
+      if (r < B.nch) {
+        const double *p1 = B.part + (int64_t)r * B.nblk;
+        double s1 = 0.0;
+#pragma unroll 8
+        for (int b = 0; b < B.nblk; ++b) s1 += p1[b];
+        if (!var) {
+          val = s1 / dn;
+        } else {
+          const double *p2 = B.part2 + (int64_t)r * B.nblk;
+          double s2 = 0.0;
+#pragma unroll 8
+          for (int b = 0; b < B.nblk; ++b) s2 += p2[b];
+          val = ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
+        }
+      }
+      rowv[i] = val;
+    }
+    __syncthreads();
     double acc[1] = {0.0};
-    for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
-      const int r = r0 + l + NT * k;
-      if (r >= B.nch) break;
-      double s1 = 0.0;
-      for (int b = 0; b < B.nblk; ++b) s1 += B.part[(int64_t)r * B.nblk + b];
-      if (!var) {
-        acc[0] += s1 / dn;
-      } else {
-        double s2 = 0.0;
-        for (int b = 0; b < B.nblk; ++b) s2 += B.part2[(int64_t)r * B.nblk + b];
-        acc[0] += ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
+    if (l < NT) {
+      for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+        if (r0 + l + NT * k >= B.nch) break;
+        acc[0] += rowv[l + NT * k];
       }
     }
     tree256_multi(acc, sh);
@@ -183,12 +203,13 @@ __device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool va
   return total / (double)B.nch;
 }
 
-__global__ __launch_bounds__(NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams, int n)
+__global__ __launch_bounds__(FS_NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams, int n)
 {
   const FlagBeamDev &B = beams[blockIdx.x];
+  __shared__ double rowv[FLAG_BLOCK];
   __shared__ double sh[136];
-  const double mu = flag_rows_csum(B, n, 0.0, false, sh);
-  const double var = flag_rows_csum(B, n, mu, true, sh);
+  const double mu = flag_rows_csum(B, n, 0.0, false, rowv, sh);
+  const double var = flag_rows_csum(B, n, mu, true, rowv, sh);
   if (threadIdx.x == 0) {
     const float sf = (float)sqrt(var);
     B.scal->mu = mu;
@@ -327,33 +348,58 @@ __device__ __forceinline__ float value_op(int op, float x, float bpv, bool bad, 
   }
 }
 
-// u8 time-major gulp -> u8 channel-major plane X8
+// u8 time-major gulp -> u8 channel-major plane X8, 64 x 64 byte tiles: a
+// thread reads 16 channels of one sample (one 16-byte load when the rows
+// allow it), scatters them into the transposed LDS tile and writes 16
+// samples of one channel with one 16-byte store.
+typedef uint32_t u32x4f __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict__ beams, int W)
 {
   const FlagBeamDev &B = beams[blockIdx.z];
   const int c0 = blockIdx.y * 64;
   const int t0 = blockIdx.x * 64;
   if (c0 >= B.nch) return;
-  __shared__ uint8_t tile[64][65];
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  __shared__ __align__(16) uint8_t tile[64][80];  // [channel][sample]
+  const int r = threadIdx.x >> 2, q = (threadIdx.x & 3) * 16;
   const uint8_t *in = static_cast<const uint8_t *>(B.in);
-  for (int r = ty; r < 64; r += 4) {
-    const int t = t0 + r, c = c0 + tx;
-    tile[r][tx] = (t < W && c < B.nch) ? in[(int64_t)t * B.in_stride + c] : (uint8_t)0;
+  {
+    const int t = t0 + r, c = c0 + q;
+    const uint8_t *src = in + (int64_t)t * B.in_stride + c;
+    uint8_t v[16];
+    if (t < W && c + 16 <= B.nch && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+      const u32x4f w = *reinterpret_cast<const u32x4f *>(src);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = (t < W && c + i < B.nch) ? src[i] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tile[q + i][r] = v[i];
   }
   __syncthreads();
-  for (int r = ty; r < 64; r += 4) {
-    const int c = c0 + r, t = t0 + tx;
-    if (c < B.nch && t < W) B.X8[(int64_t)c * B.xs + t] = tile[tx][r];
+  const int c = c0 + r, t = t0 + q;
+  if (c >= B.nch || t >= W) return;
+  uint8_t *dst = B.X8 + (int64_t)c * B.xs + t;
+  if (t + 16 <= W) {
+    *reinterpret_cast<u32x4f *>(dst) = *reinterpret_cast<const u32x4f *>(&tile[r][q]);
+  } else {
+    for (int i = 0; i < 16 && t + i < W; ++i) dst[i] = tile[r][q + i];
   }
 }
 
-// One pass of the value-table path: the workgroup builds its channel's table
-// for this pass from the previous table (lin < 0: the identity) and the
-// current statistics, workgroup 0 of the channel stores it to slot lout, and
-// every sample of the block is looked up; FOP_DIVBP_TS and FOP_NORM_QUANT
-// then divide by ts[t], FOP_NORM_QUANT normalises and quantises into the
-// ring.  Samples, partial sums and flag counts are those of k_apply.
+// One pass of the value-table path over NB consecutive blocks of a channel:
+// the workgroup builds the channel's table for this pass from the previous
+// table (lin < 0: the identity) and the current statistics, the workgroup of
+// the channel's first blocks stores it to slot lout, and every sample is
+// looked up.  TSD (FOP_TSDIV, FOP_NORM_QUANT): the value is then divided by
+// ts[t]; FOP_NORM_QUANT normalises and quantises into the ring.  The bytes
+// (and time series) arrive with 16-byte loads, one per thread and block, and
+// are read back from LDS in the canonical lane order; the NB blocks' partial
+// sums share one tree.  Samples, partial sums and flag counts are those of
+// k_apply.
+template <int NB, bool TSD>
 __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict__ beams, int W,
                                                   int op, int part, int npart, float a0, float a1,
                                                   int lin, int lout)
@@ -361,92 +407,176 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
   const FlagBeamDev &B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
-  const int blk = blockIdx.x;
+  const int blk0 = blockIdx.x * NB;
+  if (blk0 >= B.nblk) return;
   const int l = threadIdx.x;
   __shared__ float tab[256];
   __shared__ uint8_t flg[256];
-  __shared__ double sh[2 * 128];
+  __shared__ __align__(16) uint8_t bytes[NB][FLAG_BLOCK];
+  __shared__ __align__(16) float tsb[TSD ? FLAG_BLOCK : 4];
+  __shared__ double sh[2 * NB * 128];
+  const uint8_t *row = B.X8 + (int64_t)c * B.xs;
+  // loads first: the blocks' bytes (the row holds xs >= W rounded up to 64
+  // bytes, so a 16-byte load that starts below W stays in the row) and the
+  // previous table entry
+  u32x4f d[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int t = (blk0 + j) * FLAG_BLOCK + 16 * l;
+    d[j] = u32x4f{0u, 0u, 0u, 0u};
+    if (t < W) d[j] = *reinterpret_cast<const u32x4f *>(row + t);
+  }
+  const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  if (TSD) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = blk0 * FLAG_BLOCK + 4 * (l + NT * j);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t + 4 <= W) {
+        v = *reinterpret_cast<const float4 *>(B.ts + t);
+      } else if (t < W) {
+        v.x = B.ts[t];
+        if (t + 1 < W) v.y = B.ts[t + 1];
+        if (t + 2 < W) v.z = B.ts[t + 2];
+      }
+      *reinterpret_cast<float4 *>(&tsb[4 * (l + NT * j)]) = v;
+    }
+  }
   const FScal sc = *B.scal;
   const float bpv = B.bp[c];
   const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
   const bool fixed = c >= B.flag1 && c < B.flag2;
   {
-    const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
     bool g;
     const float y = value_op(op, x0, bpv, bad, fixed, sc, a0, a1, g);
     tab[l] = y;
     flg[l] = g ? 1 : 0;
-    if (blk == 0 && lout >= 0 && lout != lin) B.lut[((int64_t)lout * B.nch + c) * 256 + l] = y;
+    if (blk0 == 0 && lout >= 0 && lout != lin) B.lut[((int64_t)lout * B.nch + c) * 256 + l] = y;
   }
+#pragma unroll
+  for (int j = 0; j < NB; ++j) *reinterpret_cast<u32x4f *>(&bytes[j][16 * l]) = d[j];
   __syncthreads();
-  const uint8_t *row = B.X8 + (int64_t)c * B.xs;
-  const int base = blk * FLAG_BLOCK;
-  const bool tsdiv = op == FOP_DIVBP_TS || op == FOP_NORM_QUANT;
   uint32_t cnt = 0;
-  double acc[2] = {0.0, 0.0};
+  double acc[2 * NB];
+#pragma unroll
+  for (int j = 0; j < 2 * NB; ++j) acc[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int base = (blk0 + j) * FLAG_BLOCK;
 #pragma unroll 4
-  for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
-    const int t = base + l + NT * k;
-    if (t >= W) break;
-    const int v = row[t];
-    float x = tab[v];
-    cnt += flg[v];
-    if (tsdiv) x = x / B.ts[t];
-    if (op == FOP_NORM_QUANT) {
-      x = (float)(((double)x - sc.mu) * sc.inv);
-      int64_t col = B.col0 + t;
-      if (col >= B.ring_len) col -= B.ring_len;
-      B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
-    }
-    if (t < npart && part != FPART_NONE) {
-      const double d = (double)x;
-      acc[0] += d;
-      acc[1] += d * d;
+    for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+      const int t = base + l + NT * k;
+      if (t >= W) break;
+      const int v = bytes[j][l + NT * k];
+      float x = tab[v];
+      cnt += flg[v];
+      if (TSD) x = x / tsb[l + NT * k];
+      if (op == FOP_NORM_QUANT) {
+        x = (float)(((double)x - sc.mu) * sc.inv);
+        int64_t col = B.col0 + t;
+        if (col >= B.ring_len) col -= B.ring_len;
+        B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
+      }
+      if (t < npart && part != FPART_NONE) {
+        const double dd = (double)x;
+        acc[2 * j] += dd;
+        acc[2 * j + 1] += dd * dd;
+      }
     }
   }
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
   if (part == FPART_NONE) return;
   tree256_multi(acc, sh);
   if (l == 0) {
-    B.part[(int64_t)c * B.nblk + blk] = acc[0];
-    if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk] = acc[1];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (blk0 + j >= B.nblk) break;
+      B.part[(int64_t)c * B.nblk + blk0 + j] = acc[2 * j];
+      if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk0 + j] = acc[2 * j + 1];
+    }
   }
 }
 
-// k_ts_sum on the value-table path: x = table slot lin of channel c at the
-// sample's input byte
-__global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum_lut(const FlagBeamDev *__restrict__ beams,
-                                                               int W, int lin)
+// a channel's table of the next pass on its own (the pass that reads it
+// needs the table before its own statistics exist)
+__global__ __launch_bounds__(NT) void k_table(const FlagBeamDev *__restrict__ beams, int op,
+                                              float a0, float a1, int lin, int lout)
 {
   const FlagBeamDev &B = beams[blockIdx.z];
-  __shared__ double sp[TS_PARTS][64];
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int t = blockIdx.x * 64 + lane;
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int l = threadIdx.x;
+  const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  const FScal sc = *B.scal;
+  const float bpv = B.bp[c];
+  const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
+  const bool fixed = c >= B.flag1 && c < B.flag2;
+  bool g;
+  B.lut[((int64_t)lout * B.nch + c) * 256 + l] = value_op(op, x0, bpv, bad, fixed, sc, a0, a1, g);
+}
+
+// k_ts_sum on the value-table path, in two steps.  k_ts_part: workgroup
+// (time block, partition q) adds the values x / bp[c] (table slot lin,
+// already divided) over the partition's channels in channel order for 2048
+// samples, 8 consecutive ones per thread read with one 8-byte load per
+// channel; the tables of 32 channels at a time sit in LDS.
+// k_ts_combine: the 16 partition sums of a sample are combined pairwise
+// exactly as k_ts_sum does.
+constexpr int TS_GROUP = 32;
+constexpr int TS_SPT = 8;                // samples per thread
+constexpr int TS_BLOCK = NT * TS_SPT;    // samples per workgroup
+
+__global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ beams, int W,
+                                                int lin)
+{
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  const FlagBeamDev &B = beams[blockIdx.z];
+  const int q = blockIdx.y;
+  const int l = threadIdx.x;
+  const int t0 = blockIdx.x * TS_BLOCK + TS_SPT * l;
+  __shared__ float tab[TS_GROUP][256];
   const int per = (B.nch + TS_PARTS - 1) / TS_PARTS;
   const int c0 = min(B.nch, q * per), c1 = min(B.nch, c0 + per);
-  double s = 0.0;
-  if (t < W) {
-    const uint8_t *col = B.X8 + t;
-    const float *tabs = B.lut + (int64_t)lin * B.nch * 256;
-#pragma unroll 8
-    for (int c = c0; c < c1; ++c) {
-      const float b = B.bp[c];
-      const float x = tabs[(int64_t)c * 256 + col[(int64_t)c * B.xs]];
-      s += (double)(b > 0.f ? x / b : 0.f);
+  const float *tabs = B.lut + (int64_t)lin * B.nch * 256;
+  double s[TS_SPT];
+#pragma unroll
+  for (int j = 0; j < TS_SPT; ++j) s[j] = 0.0;
+  // the row holds xs >= W rounded up to 64 bytes: an 8-byte load that starts
+  // below W stays in the row
+  const bool in = t0 < W;
+  for (int g0 = c0; g0 < c1; g0 += TS_GROUP) {
+    const int ng = min(TS_GROUP, c1 - g0);
+    __syncthreads();
+    for (int i = 0; i < ng; ++i) tab[i][l] = tabs[(int64_t)(g0 + i) * 256 + l];
+    __syncthreads();
+    if (!in) continue;
+    const uint8_t *row = B.X8 + (int64_t)g0 * B.xs + t0;
+#pragma unroll 4
+    for (int i = 0; i < ng; ++i) {
+      const u32x2 w = *reinterpret_cast<const u32x2 *>(row + (int64_t)i * B.xs);
+#pragma unroll
+      for (int j = 0; j < TS_SPT; ++j) s[j] += (double)tab[i][(w[j >> 2] >> (8 * (j & 3))) & 0xffu];
     }
   }
-  sp[q][lane] = s;
-  __syncthreads();
-  if (q == 0 && t < W) {
-    double p[TS_PARTS];
+  double *out = B.tsp + (int64_t)q * B.xs + t0;
 #pragma unroll
-    for (int i = 0; i < TS_PARTS; ++i) p[i] = sp[i][lane];
+  for (int j = 0; j < TS_SPT; ++j)
+    if (t0 + j < W) out[j] = s[j];
+}
+
+__global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict__ beams, int W)
+{
+  const FlagBeamDev &B = beams[blockIdx.z];
+  const int t = blockIdx.x * NT + threadIdx.x;
+  if (t >= W) return;
+  double p[TS_PARTS];
 #pragma unroll
-    for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
+  for (int i = 0; i < TS_PARTS; ++i) p[i] = B.tsp[(int64_t)i * B.xs + t];
 #pragma unroll
-      for (int i = 0; i < h; ++i) p[i] += p[i + h];
-    B.ts[t] = (float)(p[0] / (double)B.nch);
-  }
+  for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
+#pragma unroll
+    for (int i = 0; i < h; ++i) p[i] += p[i + h];
+  B.ts[t] = (float)(p[0] / (double)B.nch);
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n)
@@ -476,7 +606,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
     hipLaunchKernelGGL(k_apply, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0, a1);
   };
   auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
-  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(NT), 0, s, d_beams, wp); };
+  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
 
   hipLaunchKernelGGL(k_ingest, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                     (unsigned)nbeam),
@@ -535,15 +665,20 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
   const dim3 grow((unsigned)((max_nch + NT - 1) / NT), 1, (unsigned)nbeam);
   const dim3 gbeam((unsigned)nbeam);
   int cur = -1;  // table slot holding the current values (-1: the input bytes)
+  const dim3 gplane4((unsigned)((nblk + 3) / 4), (unsigned)max_nch, (unsigned)nbeam);
   auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
-    const bool same = op == FOP_NONE || op == FOP_NORM_QUANT;
+    const bool same = op == FOP_NONE || op == FOP_NORM_QUANT || op == FOP_TSDIV;
     const int out = same ? cur : (cur == 0 ? 1 : 0);
-    hipLaunchKernelGGL(k_apply_lut, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0, a1,
-                       cur, out);
+    if (op == FOP_TSDIV || op == FOP_NORM_QUANT)
+      hipLaunchKernelGGL((k_apply_lut<1, true>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
+                         npart, a0, a1, cur, out);
+    else
+      hipLaunchKernelGGL((k_apply_lut<4, false>), gplane4, dim3(NT), 0, s, d_beams, W, op, part,
+                         npart, a0, a1, cur, out);
     cur = out;
   };
   auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
-  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(NT), 0, s, d_beams, wp); };
+  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
 
   hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                        (unsigned)nbeam),
@@ -565,9 +700,16 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
     apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
     bp();
   }
-  hipLaunchKernelGGL(k_ts_sum_lut, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam),
-                     dim3(64 * TS_PARTS), 0, s, d_beams, W, cur);
-  apply(FOP_DIVBP_TS, FPART_SUM2, wp);
+  // bandpass_correct of the final scrunch as a table, then ts_correct
+  const int nxt = cur == 0 ? 1 : 0;
+  hipLaunchKernelGGL(k_table, dim3(1u, (unsigned)max_nch, (unsigned)nbeam), dim3(NT), 0, s,
+                     d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur, nxt);
+  cur = nxt;
+  hipLaunchKernelGGL(k_ts_part, dim3((unsigned)((W + TS_BLOCK - 1) / TS_BLOCK), TS_PARTS, (unsigned)nbeam),
+                     dim3(NT), 0, s, d_beams, W, cur);
+  hipLaunchKernelGGL(k_ts_combine, dim3((unsigned)((W + NT - 1) / NT), 1, (unsigned)nbeam),
+                     dim3(NT), 0, s, d_beams, W);
+  apply(FOP_TSDIV, FPART_SUM2, wp);
   stats();
   apply(FOP_NORM_QUANT, FPART_NONE, 0);
   HAMD_CHECK(hipGetLastError());

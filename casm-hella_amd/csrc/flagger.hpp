@@ -15,7 +15,8 @@ enum FlagOp {
   FOP_SPEC = 3,        // 1 if the channel mean is out of range, else x + 1
   FOP_NORM_QUANT = 4,  // normalise and quantise into the ring
   FOP_NORM_FLAG1 = 5,  // normalise, then the 1 x 1 scrunch: clip |x| > thr to 0, +1
-  FOP_DIVBP_TS = 6     // (x / bp[c]) / ts[t]
+  FOP_DIVBP_TS = 6,    // (x / bp[c]) / ts[t]
+  FOP_TSDIV = 7        // x / ts[t] (value-table path: the table already holds x / bp[c])
 };
 enum FlagPart { FPART_NONE = 0, FPART_SUM = 1, FPART_SUM2 = 2 };
 
@@ -53,6 +54,7 @@ struct FlagBeamDev {
   // channel's current value of every input byte
   uint8_t *X8;         // [nch][xs]
   float *lut;          // [2][nch][256]
+  double *tsp;         // [16][xs] partition sums of the time series
 };
 
 struct FlagScrunch {
@@ -92,6 +94,12 @@ inline bool flag_value_tables(int in_fmt, const FlagParams &P)
 }
 // byte offset of the tables behind the uint8 plane in a work buffer
 inline size_t flag_table_offset(int nch, int64_t xs) { return ((size_t)nch * xs + 255) / 256 * 256; }
+// bytes of the value-table path's work buffer: plane, two tables and the
+// 16 partition sums of the time series
+inline size_t flag_table_bytes(int nch, int64_t xs)
+{
+  return flag_table_offset(nch, xs) + (size_t)2 * nch * 256 * 4 + (size_t)16 * xs * 8;
+}
 // a box scrunch (ts x fs > 1) needs the mask plane
 inline bool flag_needs_mask(const FlagParams &P)
 {

@@ -225,8 +225,7 @@ static WorkSizes work_sizes(const WorkDims &d)
   const size_t ncnt = (size_t)d.nw * d.rows * sblk;
   // FP32 plane, or (value tables) the uint8 plane and two [nch][256] tables;
   // a batch mixing both paths cannot occur (one input format per context)
-  z.X = d.tables ? flag_table_offset(d.nch, d.xs) + (size_t)2 * d.nch * 256 * 4
-                 : (size_t)d.nch * d.xs * 4;
+  z.X = d.tables ? flag_table_bytes(d.nch, d.xs) : (size_t)d.nch * d.xs * 4;
   z.mask = d.mask ? (size_t)d.nch * d.xs : 0;
   z.D = (size_t)d.ndm * d.dstride * 4;
   z.planes = d.planes ? (size_t)d.nw * d.rows * round_up((size_t)d.nout, 64) * 4 : 0;
@@ -443,6 +442,7 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
   if (flag_value_tables(cfg.input_fmt, cfg.flag)) {
     f.X8 = w.X.as<uint8_t>();
     f.lut = reinterpret_cast<float *>(w.X.as<uint8_t>() + flag_table_offset(B.nch, f.xs));
+    f.tsp = reinterpret_cast<double *>(f.lut + (size_t)2 * B.nch * 256);
   }
   return f;
 }

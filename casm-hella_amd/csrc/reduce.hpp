@@ -9,12 +9,13 @@ namespace hamd {
 //   tree(v) with steps p[l] += p[l + s] for s = 128, 64, ..., 1
 // for every k.  The two cross-wave steps go through LDS (sh: N x 128
 // doubles), the six in-wave steps use shuffles; the additions and their order
-// are those of the sequential pairwise tree.
+// are those of the sequential pairwise tree.  In a larger workgroup the
+// threads from 256 on only take part in the barriers.
 template <int N>
 __device__ void tree256_multi(double (&v)[N], double *sh)
 {
   const int l = threadIdx.x;
-  if (l >= 128) {
+  if (l >= 128 && l < 256) {
 #pragma unroll
     for (int k = 0; k < N; ++k) sh[k * 128 + (l - 128)] = v[k];
   }

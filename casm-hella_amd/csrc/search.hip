@@ -497,23 +497,35 @@ __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, i
     for (int b = 0; b < C.nblk; ++b) s2 += B.part2[base + b];
 }
 
+// The row values of a block of SB rows are formed by all FIN_NT threads at
+// once (rowv); lane l < 256 then adds rows l + 256k in ascending k and the
+// lanes are folded by the canonical tree.
+constexpr int FIN_NT = 1024;
+
 __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k, double mu,
-                            bool var, bool exact, int ngroups, double *sh)
+                            bool var, bool exact, int ngroups, double *rowv, double *sh)
 {
   const int l = threadIdx.x;
   const double n = (double)C.wp;
   double total = 0.0;
   for (int r0 = 0; r0 < B.rows; r0 += SB) {
+    for (int i = l; i < SB; i += FIN_NT) {
+      const int r = r0 + i;
+      double val = 0.0;
+      if (r < B.rows) {
+        double s1, s2;
+        row_sums(B, C, k, r, exact, ngroups, var, s1, s2);
+        val = !var ? s1 / n : ((s2 - (2.0 * mu) * s1) + (n * mu) * mu) / n;
+      }
+      rowv[i] = val;
+    }
+    __syncthreads();
     double acc[1] = {0.0};
-    for (int kk = 0; kk < SB / NT; ++kk) {
-      const int r = r0 + l + NT * kk;
-      if (r >= B.rows) break;
-      double s1, s2;
-      row_sums(B, C, k, r, exact, ngroups, var, s1, s2);
-      if (!var)
-        acc[0] += s1 / n;
-      else
-        acc[0] += ((s2 - (2.0 * mu) * s1) + (n * mu) * mu) / n;
+    if (l < NT) {
+      for (int kk = 0; kk < SB / NT; ++kk) {
+        if (r0 + l + NT * kk >= B.rows) break;
+        acc[0] += rowv[l + NT * kk];
+      }
     }
     tree256_multi(acc, sh);
     if (l == 0) sh[256] = acc[0];
@@ -524,15 +536,16 @@ __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k
   return total / (double)B.rows;
 }
 
-__global__ __launch_bounds__(NT) void k_finish(const SearchBeamDev *__restrict__ beams,
-                                               SearchCommon C, int which, int ngroups)
+__global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restrict__ beams,
+                                                   SearchCommon C, int which, int ngroups)
 {
   const SearchBeamDev &B = beams[blockIdx.y];
   const int k = blockIdx.x;
+  __shared__ double rowv[SB];
   __shared__ double sh[260];  // 128 tree slots + the broadcast slot 256
   const bool exact = which == P_STATS && C.mode == SEARCH_BOXCAR;
-  const double mu = rows_csum(B, C, k, 0.0, false, exact, ngroups, sh);
-  const double var = rows_csum(B, C, k, mu, true, exact, ngroups, sh);
+  const double mu = rows_csum(B, C, k, 0.0, false, exact, ngroups, rowv, sh);
+  const double var = rows_csum(B, C, k, mu, true, exact, ngroups, rowv, sh);
   SScal &sc = *B.scal;
   const float sf = (float)sqrt(var);
   if (which == P_STATS) {
@@ -765,7 +778,7 @@ void SearchRunner::smooth(const SearchBeamDev *d_beams, int nbeam, int max_rows,
     launch_stats<true>(d_beams, C, (int)P_STATS, bpw, gp, s);
   else
     launch_stats_box(d_beams, C, bpw, gp, s);
-  hipLaunchKernelGGL(k_finish, dim3((unsigned)C.nw, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C,
+  hipLaunchKernelGGL(k_finish, dim3((unsigned)C.nw, (unsigned)nbeam), dim3(FIN_NT), 0, s, d_beams, C,
                      (int)P_STATS, (int)gp.x);
   HAMD_CHECK(hipGetLastError());
 }
@@ -791,7 +804,7 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
     }
   };
   auto fin = [&](int p) {
-    hipLaunchKernelGGL(k_finish, dim3(1u, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C, p, (int)gs.x);
+    hipLaunchKernelGGL(k_finish, dim3(1u, (unsigned)nbeam), dim3(FIN_NT), 0, s, d_beams, C, p, (int)gs.x);
   };
   pass(P_Y0);
   fin(P_Y0);
