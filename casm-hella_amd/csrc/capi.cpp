@@ -261,6 +261,25 @@ int hella_get_clusters(hella_ctx c, hella_cluster *out, int cap)
   return n;
 }
 
+int hella_get_flag_counts(hella_ctx c, int beam, unsigned *out, int cap)
+{
+  if (!valid_beam(c, beam) || cap < 0 || (cap > 0 && !out)) {
+    fail(HELLA_ERR_INVALID, "bad argument");
+    return -1;
+  }
+  int n = -1;
+  const hella_status st = guard([&] {
+    ctx_build(c);
+    const BeamState &B = c->beams[(size_t)beam];
+    n = B.nch;
+    if (cap > 0) {
+      HAMD_CHECK(hipStreamSynchronize(c->stream));
+      HAMD_CHECK(hipMemcpy(out, B.f_count.p, (size_t)(cap < n ? cap : n) * 4, hipMemcpyDeviceToHost));
+    }
+  });
+  return st == HELLA_OK ? n : -1;
+}
+
 long long hella_last_crossings(hella_ctx c, int beam)
 {
   if (!valid_beam(c, beam)) return -1;
@@ -355,15 +374,18 @@ hella_status hella_set_tile(hella_ctx c, int lanes_per_dm, int chunk, int order)
   if (!c) return fail(HELLA_ERR_INVALID, "ctx is NULL");
   if (c->built) return fail(HELLA_ERR_STATE, "set the tile before the first gulp");
   // order bits 1-2: loader (0 LDS DMA, 1 registers, 2 register prefetch,
-  // 3 one DM trial per wave: 64 lanes, chunk = 4 or 8 waves per workgroup)
+  // 3 one DM trial per wave: 64 lanes, chunk = 4 or 8 waves per workgroup);
+  // bit 3 with loader 3: a chunk's LDS reads all issued before its sums
   const int loader = (order >> 1) & 3;
+  const bool pipe = ((order >> 3) & 1) != 0;
   const bool ok = loader == 3
                       ? lanes_per_dm == 64 && (chunk == 4 || chunk == 8)
                       : (lanes_per_dm == 8 || lanes_per_dm == 16 || lanes_per_dm == 32 ||
                          lanes_per_dm == 64) &&
                             (chunk == 8 || chunk == 16 || chunk == 32) && !(loader == 2 && chunk == 32);
-  if (!ok || order < 0 || order > 7) return fail(HELLA_ERR_INVALID, "tile shape out of range");
-  c->tile = DDTile{lanes_per_dm, chunk, loader == 0 ? 1 : loader == 1 ? 0 : loader};
+  if (!ok || order < 0 || order > 15 || (pipe && loader != 3))
+    return fail(HELLA_ERR_INVALID, "tile shape out of range");
+  c->tile = DDTile{lanes_per_dm, chunk, loader == 0 ? 1 : loader == 1 ? 0 : pipe ? 4 : loader};
   c->tile_order = order & 1;
   c->tile_forced = true;
   return HELLA_OK;

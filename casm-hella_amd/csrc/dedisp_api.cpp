@@ -19,8 +19,9 @@ struct dedisp_plan_struct {
   std::vector<dedisp_bool> killmask_pub;
   // device-side copies, rebuilt when the DM list or the killmask change
   bool dirty = true;
-  DevBuf d_delta, d_live, d_delta_t;
+  DevBuf d_delta, d_live, d_delta_t, d_ent;
   int dt_stride = 0;
+  int ent_stride = 0, ent_nd = 0;  // entry tables of the one-trial-per-wave kernel
   int nlive = 0;
   std::vector<int32_t> delta_host, live_host;
   DDTile tile{16, 32, 1};
@@ -85,6 +86,14 @@ dedisp_error refresh_device(dedisp_plan plan)
     delays_transposed(plan->delta_host, ndm, plan->nlive, dt, plan->dt_stride);
     plan->d_delta_t.ensure(dt.size() * 4);
     HAMD_CHECK(hipMemcpy(plan->d_delta_t.p, dt.data(), dt.size() * 4, hipMemcpyHostToDevice));
+    plan->ent_nd = 0;
+    if (plan->tile.dma >= 3) {
+      std::vector<uint32_t> e;
+      plan->ent_nd = dd_tile_ND(plan->tile);
+      dd_entries(plan->delta_host, ndm, plan->nlive, plan->ent_nd, e, plan->ent_stride);
+      plan->d_ent.ensure(e.size() * 4);
+      HAMD_CHECK(hipMemcpy(plan->d_ent.p, e.data(), e.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   plan->dirty = false;
   return DEDISP_NO_ERROR;
@@ -246,6 +255,12 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
     B.live = plan->d_live.as<int32_t>();
     B.delta_t = plan->d_delta_t.as<int32_t>() + first_dm_idx;
     B.dt_stride = plan->dt_stride;
+    // the entry tables hold blocks that start at multiples of ent_nd
+    if (plan->ent_nd > 0 && first_dm_idx % (size_t)plan->ent_nd == 0) {
+      B.ent = plan->d_ent.as<uint32_t>() + first_dm_idx * (size_t)plan->ent_stride * 2;
+      B.ent_stride = plan->ent_stride;
+      B.ent_nd = plan->ent_nd;
+    }
     B.nlive = plan->nlive;
     B.ndm = (int)dm_count;
     B.nout = (int)nout;

@@ -501,10 +501,14 @@ void launch_pf(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
 // of two LDS buffers once the current chunk is summed, so one barrier per
 // chunk suffices.  Channels past the last live one stage zeros, so every
 // chunk is a full NDW channels and the sums are straight-line code.
+// The LDS entry of a (trial, channel) comes from the beam's precomputed
+// table (dd_entries) when it has one for blocks of NDW trials and the tile
+// starts at a ring column that is a multiple of 16; otherwise the scalar
+// unit derives it from the delays.
 // Requirements: ring % 16 == 0 (every beam's ring is a multiple of 32 or
 // 64 columns) and the DM-major delay table readable for DD_DELTA_PAD
 // entries past its end.
-template <int NDW, int NU>
+template <int NDW, int NU, bool PIPE>
 __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restrict__ beams,
                                                      const DDTileDesc *__restrict__ tiles,
                                                      int lrow)
@@ -572,22 +576,38 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
       if (u + 1 < NU || lane16 + u * 1024 < lrow) *reinterpret_cast<u32x4 *>(dst + u * 1024) = P[u];
   };
 
-  // this wave's trial delay and the block's first-trial delay of the CC
-  // channels of a chunk (scalar registers); past the last live channel the
-  // two are made equal, so the zero rows are read from their start
-  int32_t own[CC], bas[CC];
+  // LDS entry of this wave's trial for the CC channels of a chunk (scalar
+  // registers): the trial's dword offset in the staged row and its v_perm
+  // selector of the even samples.  From the precomputed table when the
+  // beam has one for this block size (one scalar load per chunk), else from
+  // this trial's and the block's first trial's delays; past the last live
+  // channel the offset is that of the zero row's start.
+  typedef uint32_t evec __attribute__((ext_vector_type(2 * CC), aligned(4)));
+  typedef __attribute__((address_space(4))) const evec cevec;
+  const bool tab = B.ent != nullptr && B.ent_nd == NDW && (S & 15) == 0;
+  const uint32_t *ent_row = tab ? B.ent + (int64_t)min(d0 + wave, ndm - 1) * B.ent_stride * 2 : nullptr;
+  uint32_t dw[CC], se[CC];
   auto entries = [&](int k0) {
+    if (tab) {
+      const evec e = *(cevec *)(ent_row + 2 * k0);
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc) {
+        dw[cc] = e[2 * cc];
+        se[cc] = e[2 * cc + 1];
+      }
+      return;
+    }
     const dvec o = *(cdvec *)(own_row + k0);
     const dvec f = *(cdvec *)(first_row + k0);
 #pragma unroll
     for (int cc = 0; cc < CC; ++cc) {
-      own[cc] = o[cc];
-      bas[cc] = f[cc];
-    }
-    if (k0 + CC > nlive) {
-#pragma unroll
-      for (int cc = 0; cc < CC; ++cc)
-        if (k0 + cc >= nlive) own[cc] = bas[cc];
+      const int bas = f[cc];
+      const int own = k0 + cc < nlive ? o[cc] : bas;
+      // the row starts at the 16-byte aligned column below the first trial's start
+      const int q = ((S + bas) & 15) + own - bas;
+      const uint32_t s = (uint32_t)(q & 3);
+      dw[cc] = (uint32_t)(q & ~3);
+      se[cc] = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
     }
   };
 
@@ -600,14 +620,10 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
 
   const int lane_off = lane * V;  // byte offset of the lane's first sample
   auto one = [&](int buf, int cc) {
-    // window offset of this trial in the staged row: the row starts at the
-    // 16-byte aligned column below the first trial's start
-    const int q = ((S + bas[cc]) & 15) + own[cc] - bas[cc];
-    const uint32_t s = (uint32_t)(q & 3);
-    const uint32_t sel_e = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
+    const uint32_t sel_e = se[cc];
     const uint32_t sel_o = sel_e + 0x00010001u;
     const uint32_t *wrow = reinterpret_cast<const uint32_t *>(
-        lds + (buf * CC + cc) * lrow + (q & ~3) + lane_off);
+        lds + (buf * CC + cc) * lrow + dw[cc] + lane_off);
     uint32_t w[NW];
 #pragma unroll
     for (int i = 0; i < NW; ++i) w[i] = wrow[i];
@@ -630,10 +646,31 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
     const int buf = ci & 1;
     const bool more = ci + 1 < nchunks;
     if (more) fetch();  // the next chunk's window, in flight during the sums
+    if (PIPE) {
+      // every channel's LDS words of the chunk are requested before the
+      // first v_perm, so the reads of later channels overlap the sums
+      uint32_t w[CC][NW], so[CC];
 #pragma unroll
-    for (int cc = 0; cc < CC; cc += 2) {
-      one(buf, cc);
-      one(buf, cc + 1);
+      for (int cc = 0; cc < CC; ++cc) {
+        so[cc] = se[cc] + 0x00010001u;
+        const uint32_t *wrow = reinterpret_cast<const uint32_t *>(
+            lds + (buf * CC + cc) * lrow + dw[cc] + lane_off);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) w[cc][i] = wrow[i];
+      }
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc)
+#pragma unroll
+        for (int k = 0; k < V / 4; ++k) {
+          acc[2 * k] += perm(w[cc][k + 1], w[cc][k], se[cc]);
+          acc[2 * k + 1] += perm(w[cc][k + 1], w[cc][k], so[cc]);
+        }
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < CC; cc += 2) {
+        one(buf, cc);
+        one(buf, cc + 1);
+      }
     }
     since_flush += CC;
     if (since_flush >= 256 || !more) {
@@ -658,7 +695,7 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
   store_out(B, d0 + wave, t0 + lane * V, tot);
 }
 
-template <int NDW>
+template <int NDW, bool PIPE>
 void launch_w(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int lrow,
               hipStream_t s)
 {
@@ -672,11 +709,11 @@ void launch_w(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, i
                        lrow);
   };
   switch (nu < 2 ? 2 : nu) {
-    case 2: go(k_dedisp_w<NDW, 2>); break;
-    case 3: go(k_dedisp_w<NDW, 3>); break;
-    case 4: go(k_dedisp_w<NDW, 4>); break;
-    case 5: go(k_dedisp_w<NDW, 5>); break;
-    case 6: go(k_dedisp_w<NDW, 6>); break;
+    case 2: go(k_dedisp_w<NDW, 2, PIPE>); break;
+    case 3: go(k_dedisp_w<NDW, 3, PIPE>); break;
+    case 4: go(k_dedisp_w<NDW, 4, PIPE>); break;
+    case 5: go(k_dedisp_w<NDW, 5, PIPE>); break;
+    case 6: go(k_dedisp_w<NDW, 6, PIPE>); break;
     default: throw HipError{hipErrorInvalidValue, "dedispersion window too wide for k_dedisp_w", __LINE__};
   }
 }
@@ -762,7 +799,7 @@ static int lrow_for(const DDTile &t, int max_spread)
 
 size_t dd_lds_bytes(const DDTile &t, int max_spread)
 {
-  if (t.dma == 3) return (size_t)2 * t.chunk * lrow_for(t, max_spread);
+  if (t.dma >= 3) return (size_t)2 * t.chunk * lrow_for(t, max_spread);
   const int ND = NT / t.lanes_per_dm;
   // window rows, the chunk's delays (register loader), a pad word per
   // channel and the (address, selector) table of every (channel, trial)
@@ -798,7 +835,7 @@ void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
 
 size_t dd_lds_budget(const DDTile &t)
 {
-  if (t.dma != 3) return DD_LDS_BUDGET;
+  if (t.dma < 3) return DD_LDS_BUDGET;
   // the one-trial-per-wave kernel may use the device's whole per-workgroup LDS
   static size_t limit = 0;
   if (limit == 0) {
@@ -814,7 +851,7 @@ size_t dd_lds_budget(const DDTile &t)
 
 bool dd_tile_fits(const DDTile &t, int max_spread, int64_t min_ring)
 {
-  if (t.dma != 3) return dd_lds_bytes(t, max_spread) <= dd_lds_budget(t);
+  if (t.dma < 3) return dd_lds_bytes(t, max_spread) <= dd_lds_budget(t);
   const int lrow = lrow_for(t, max_spread);
   (void)min_ring;  // windows wrap any number of times
   return t.lanes_per_dm == 64 && (t.chunk == 4 || t.chunk == 8) &&
@@ -825,15 +862,20 @@ void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
                int max_spread, hipStream_t s, int64_t min_ring)
 {
   if (ntiles <= 0) return;
-  if (t_in.dma == 3) {
+  if (t_in.dma >= 3) {
     // the DM block of the tile table is the workgroup's wave count: no
     // shape can be changed here, so a shape that does not fit is an error
     if (!dd_tile_fits(t_in, max_spread, min_ring))
       throw HipError{hipErrorInvalidValue, "dedispersion tile (one trial per wave) does not fit",
                      __LINE__};
     const int lrow = lrow_for(t_in, max_spread);
-    if (t_in.chunk == 4) launch_w<4>(d_beams, d_tiles, ntiles, lrow, s);
-    else launch_w<8>(d_beams, d_tiles, ntiles, lrow, s);
+    if (t_in.dma == 4) {
+      if (t_in.chunk == 4) launch_w<4, true>(d_beams, d_tiles, ntiles, lrow, s);
+      else launch_w<8, true>(d_beams, d_tiles, ntiles, lrow, s);
+    } else {
+      if (t_in.chunk == 4) launch_w<4, false>(d_beams, d_tiles, ntiles, lrow, s);
+      else launch_w<8, false>(d_beams, d_tiles, ntiles, lrow, s);
+    }
     HAMD_CHECK(hipGetLastError());
     return;
   }

@@ -95,6 +95,12 @@ struct DDBeamDev {
   const int32_t *live;   // [nlive] channel index of each live channel
   const int32_t *delta_t;  // channel-major delays: delta_t[c * dt_stride + d] = delta[d][c]
   int dt_stride;           // (from this beam's first trial; >= ndm + 64, padded)
+  // one-trial-per-wave kernel: per (trial, live channel) LDS entry of the
+  // trial in its block's staged window (dd_entries), for blocks of ent_nd
+  // trials; null when absent (the kernel then derives the entries itself)
+  const uint32_t *ent;     // [ndm][ent_stride][2]
+  int ent_stride;
+  int ent_nd;
   int nlive;
   int ndm;
   int nout;              // output samples per DM row
@@ -102,6 +108,17 @@ struct DDBeamDev {
   void *out;
   int64_t out_stride;    // elements between DM rows
 };
+
+// Per (trial d, live channel c) entry of the one-trial-per-wave kernel for
+// DM blocks of nd trials (block start d0 = d - d % nd), valid when the ring
+// column of every tile start is a multiple of 16: q = (bas & 15) + own - bas
+// with bas = delay(d0, c), own = delay(d, c) is the trial's byte offset in the
+// channel's staged window; the entry holds q rounded down to a dword and the
+// v_perm selector of the even samples at q's byte alignment.  Rows are padded
+// to stride >= nlive + 16 with entries of offset 0 (the zero rows staged for
+// channels past the last live one).
+void dd_entries(const std::vector<int32_t> &delta, int ndm, int nlive, int nd,
+                std::vector<uint32_t> &out, int &stride);
 
 // Channel-major copy of a [ndm][nlive] delay table: [nlive][stride] with
 // stride = ndm + 64, the pad repeating each channel's last delay, so any
@@ -126,14 +143,15 @@ struct DDTile {
   int dma = 1;       // 0: loads through registers; 1: double-buffered global->LDS DMA;
                      // 2: next chunk prefetched in registers (chunk 8 or 16);
                      // 3: one DM trial per wave (lanes_per_dm 64): chunk = 4 or 8 waves,
-                     //    i.e. trials per workgroup, scalar delay arithmetic
+                     //    i.e. trials per workgroup, scalar delay arithmetic;
+                     // 4: as 3, every LDS read of a chunk issued before its sums
 };
 
 int dd_samples_per_thread();
 // LDS bytes a dedispersion workgroup may use (the per-workgroup launch limit)
 constexpr size_t DD_LDS_BUDGET = 64 * 1024;
 inline int dd_tile_T(const DDTile &t) { return t.lanes_per_dm * dd_samples_per_thread(); }
-inline int dd_tile_ND(const DDTile &t) { return t.dma == 3 ? t.chunk : 256 / t.lanes_per_dm; }
+inline int dd_tile_ND(const DDTile &t) { return t.dma >= 3 ? t.chunk : 256 / t.lanes_per_dm; }
 // largest per-channel spread of delays inside any block of nd consecutive DMs
 int dd_max_spread(const int32_t *delta, int ndm, int nlive, int nd);
 size_t dd_lds_bytes(const DDTile &t, int max_spread);

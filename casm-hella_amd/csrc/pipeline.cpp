@@ -293,6 +293,15 @@ void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, in
 void build_plain_tiles(BeamState &B, const DDTile &t, int order)
 {
   B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(t)) : 0;
+  // entry tables of the one-trial-per-wave kernel (its blocks are tile-sized)
+  if (t.dma >= 3 && B.nlive > 0 && B.ent_nd != dd_tile_ND(t)) {
+    std::vector<uint32_t> e;
+    dd_entries(B.delta_host, B.ndm, B.nlive, dd_tile_ND(t), e, B.ent_stride);
+    B.d_ent.ensure(e.size() * 4);
+    HAMD_CHECK(hipMemcpy(B.d_ent.p, e.data(), e.size() * 4, hipMemcpyHostToDevice));
+    B.ent = B.d_ent.as<uint32_t>();
+    B.ent_nd = dd_tile_ND(t);
+  }
   B.ntiles = 0;
   if (B.nlive == 0) return;
   DDBeamDev dd{};
@@ -458,6 +467,12 @@ DDBeamDev make_ddesc(const Config &cfg, int W, BeamState &B, WorkSet &w)
   d.live = B.live;
   d.delta_t = B.delta_t;
   d.dt_stride = B.dt_stride;
+  // the entries assume tile starts at ring columns that are multiples of 16:
+  // window starts advance by the gulp, tile starts by multiples of 1792
+  const bool aligned = cfg.oneshot || (W % 16 == 0 && B.ntime % 16 == 0);
+  d.ent = aligned ? B.ent : nullptr;
+  d.ent_stride = B.ent_stride;
+  d.ent_nd = B.ent_nd;
   d.nlive = B.nlive;
   d.ndm = B.ndm;
   d.nout = B.ntime_dd;
@@ -943,7 +958,8 @@ void ctx_autotune(hella_ctx c, const char *path)
   HAMD_CHECK(hipEventCreate(&e1));
   DevBuf d_tiles;
   std::vector<DDTile> shapes;
-  for (int cc : {8, 4}) shapes.push_back(DDTile{64, cc, 3});
+  for (int dma : {3, 4})
+    for (int cc : {8, 4}) shapes.push_back(DDTile{64, cc, dma});
   for (int dma : {2, 1, 0})
     for (int G : {8, 16, 32, 64})
       for (int cc : {8, 16, 32})

@@ -45,10 +45,12 @@ struct BeamState {
   PlanHost plan;
   std::vector<int> killmask;
   std::vector<int32_t> delta_host, live_host;
-  DevBuf d_delta, d_live, d_delta_t;
+  DevBuf d_delta, d_live, d_delta_t, d_ent;
   const int32_t *delta = nullptr, *live = nullptr;  // device tables (own or shared)
   const int32_t *delta_t = nullptr;                 // channel-major delays
   int dt_stride = 0;
+  const uint32_t *ent = nullptr;                    // dd_entries for blocks of ent_nd trials
+  int ent_stride = 0, ent_nd = 0;
   int nlive = 0, ndm = 0;
   int64_t md = 0;
   int64_t ntime = 0;        // ring columns (NTIME)

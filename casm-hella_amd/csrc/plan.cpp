@@ -79,6 +79,27 @@ void PlanHost::delays_live(size_t d_first, size_t nd, std::vector<int32_t> &delt
       delta[d * live.size() + i] = delay_of(d_first + d, (size_t)live[i]);
 }
 
+void dd_entries(const std::vector<int32_t> &delta, int ndm, int nlive, int nd,
+                std::vector<uint32_t> &out, int &stride)
+{
+  auto sel = [](uint32_t s) { return s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u; };
+  stride = (nlive + 16 + 3) / 4 * 4;
+  out.assign((size_t)ndm * stride * 2, 0);
+  for (int d = 0; d < ndm; ++d) {
+    const int d0 = d - d % nd;
+    uint32_t *row = out.data() + (size_t)d * stride * 2;
+    for (int c = 0; c < stride; ++c) {
+      int q = 0;
+      if (c < nlive) {
+        const int32_t bas = delta[(size_t)d0 * nlive + c], own = delta[(size_t)d * nlive + c];
+        q = (bas & 15) + own - bas;
+      }
+      row[2 * c] = (uint32_t)(q & ~3);
+      row[2 * c + 1] = sel((uint32_t)(q & 3));
+    }
+  }
+}
+
 void delays_transposed(const std::vector<int32_t> &delta, int ndm, int nlive,
                        std::vector<int32_t> &out, int &stride)
 {

@@ -269,6 +269,9 @@ std::unique_ptr<BeamState> new_resident(hella_server s)
     B->live = g.live;
     B->delta_t = g.delta_t;
     B->dt_stride = g.dt_stride;
+    B->ent = g.ent;
+    B->ent_stride = g.ent_stride;
+    B->ent_nd = g.ent_nd;
     B->nlive = g.nlive;
     B->ndm = g.ndm;
     B->md = g.md;

@@ -283,6 +283,11 @@ def _declare_hella(L):
     d("hella_server_get_stats", _I, P, C.POINTER(_ServerStats))
     d("hella_server_footprint", _I, C.c_char_p, C.POINTER(_ServerOpts),
       C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong))
+    d("hella_get_flag_counts", _I, P, _I, C.POINTER(C.c_uint), _I)
+    d("hella_write_candidates", _I, C.c_char_p, C.POINTER(_Cand), _I, _I)
+    d("hella_format_coincidencer", _I, C.c_longlong, C.c_longlong, C.c_char_p, C.POINTER(_Cand),
+      _I, _I, C.c_char_p, _I)
+    d("hella_write_specflags", _I, C.c_char_p, C.POINTER(C.c_uint), _I)
 
 
 _orig_declare = _declare
@@ -431,6 +436,15 @@ class Hella:
         out = np.zeros(n, CLUSTER_DTYPE)
         if n:
             lib().hella_get_clusters(self._c, out.ctypes.data_as(C.POINTER(_Cluster)), n)
+        return out
+
+    def flag_counts(self, beam=0):
+        """Per-channel flagged-sample counts of the beam (accumulated)."""
+        n = lib().hella_get_flag_counts(self._c, beam, None, 0)
+        if n < 0:
+            _hchk(1)
+        out = np.zeros(n, np.uint32)
+        lib().hella_get_flag_counts(self._c, beam, out.ctypes.data_as(C.POINTER(C.c_uint)), n)
         return out
 
     def crossings(self, beam):
@@ -654,3 +668,35 @@ def serve(server, ring, events, pool, time_scale=1.0):
     if "error" in res:
         raise res["error"]
     return res["late"]
+
+
+# ---------------------------------------------------------------- candidate emission
+def _cands_arg(cands):
+    c = np.ascontiguousarray(cands, CAND_DTYPE)
+    return c, (c.ctypes.data_as(C.POINTER(_Cand)) if c.size else None)
+
+
+def write_candidates(path, cands, beam0=0):
+    """Append candidates to a text file in the reference's format (header
+    line when the file is empty, then "%g %d %g %d %d %g %d" lines)."""
+    c, ptr = _cands_arg(cands)
+    _hchk(lib().hella_write_candidates(str(path).encode(), ptr, int(c.size), beam0))
+
+
+def format_coincidencer(samp0, gulp, cands, beam0=0, dada_fields=None):
+    """The coincidencer message of one gulp as text."""
+    c, ptr = _cands_arg(cands)
+    df = dada_fields.encode() if dada_fields else None
+    n = lib().hella_format_coincidencer(samp0, gulp, df, ptr, int(c.size), beam0, None, 0)
+    if n < 0:
+        raise HellaError("hella_format_coincidencer: bad argument")
+    buf = C.create_string_buffer(n + 1)
+    lib().hella_format_coincidencer(samp0, gulp, df, ptr, int(c.size), beam0, buf, n + 1)
+    return buf.value.decode()
+
+
+def write_specflags(path, counts):
+    """Append one line per channel with its flagged-sample count."""
+    a = np.ascontiguousarray(counts, np.uint32)
+    _hchk(lib().hella_write_specflags(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_uint)),
+                                      int(a.size)))

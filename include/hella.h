@@ -162,8 +162,36 @@ hella_status hella_autotune(hella_ctx ctx, const char *path);
  * (8/16/32); order bit 0: tile order (0 DM-block-major, 1 time-major); order
  * bits 1-2: window loader (0 LDS DMA, 1 through registers, 2 next chunk
  * prefetched in registers, chunk 8 or 16; 3 one DM trial per wave: lanes 64,
- * chunk 4 or 8 = DM trials per workgroup) */
+ * chunk 4 or 8 = DM trials per workgroup); order bit 3 (loader 3 only): issue
+ * every LDS read of a chunk before its sums */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
+
+/* ---- candidate emission (src/hella/src/peaks.cu:129-259) ----------------- */
+/* Per-channel count of flagged samples of the beam (the grown-mask counts of
+ * every 1 x 1 / box scrunch, accumulated since the context was created; the
+ * reference's flagSpec, flagger.cu:116-124, casm:705-716).  Returns nchan
+ * (copies min(cap, nchan)), or -1 on error. */
+int hella_get_flag_counts(hella_ctx ctx, int beam, unsigned *out, int cap);
+/* Append candidates to the text file path as output_peaks does
+ * (peaks.cu:129-182): the header "SNR SAMP_START TIME_START WIDTH DM_IDX DM
+ * BEAM_IDX" if the file is empty, then one "%g %d %g %d %d %g %d" line per
+ * candidate: S/N, abs_samp, abs_samp * 1e-3 / 86400 (TIME_RESOLUTION and
+ * SECONDS_PER_DAY, definitions.h:56,59), width index, DM index, DM,
+ * beam + beam0.  The samples print in full (%lld) beyond 2^31. */
+hella_status hella_write_candidates(const char *path, const hella_candidate *cands, int n,
+                                    int beam0);
+/* The coincidencer message of one gulp (peaks.cu:212-231): the line
+ * "(samp0 / gulp) + 1" (samp0: the gulp's first output sample, C division),
+ * followed by " dada_fields" (the DADA header's "UTC_START PICOSECONDS TSAMP";
+ * NULL or "" for other inputs), then the candidate lines of
+ * hella_write_candidates.  Writes at most cap - 1 bytes and a NUL; returns
+ * the full length (as snprintf), or -1 on a bad argument.  Sending it (a TCP
+ * connection per gulp in the reference) is left to the caller. */
+int hella_format_coincidencer(long long samp0, long long gulp, const char *dada_fields,
+                              const hella_candidate *cands, int n, int beam0, char *buf, int cap);
+/* Append one "%u" line per channel (the spectral-flag file of
+ * casm:756-761). */
+hella_status hella_write_specflags(const char *path, const unsigned *counts, int nchan);
 
 /* ---- PSRDADA-style block ring (streaming ingest) -------------------------
  * Replaces the PSRDADA data block casm-hella reads from (src/hella/src/dada.cpp:16-88;
