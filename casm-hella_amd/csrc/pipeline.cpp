@@ -290,18 +290,22 @@ void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, in
   tile = DDTile{64, 8, 0};
 }
 
+// entry tables of the one-trial-per-wave kernel (its blocks are tile-sized)
+static void ensure_entries(BeamState &B, const DDTile &t)
+{
+  if (t.dma < 3 || B.nlive == 0 || B.ent_nd == dd_tile_ND(t)) return;
+  std::vector<uint32_t> e;
+  dd_entries(B.delta_host, B.ndm, B.nlive, dd_tile_ND(t), e, B.ent_stride);
+  B.d_ent.ensure(e.size() * 4);
+  HAMD_CHECK(hipMemcpy(B.d_ent.p, e.data(), e.size() * 4, hipMemcpyHostToDevice));
+  B.ent = B.d_ent.as<uint32_t>();
+  B.ent_nd = dd_tile_ND(t);
+}
+
 void build_plain_tiles(BeamState &B, const DDTile &t, int order)
 {
   B.spread = B.nlive ? dd_max_spread(B.delta_host.data(), B.ndm, B.nlive, dd_tile_ND(t)) : 0;
-  // entry tables of the one-trial-per-wave kernel (its blocks are tile-sized)
-  if (t.dma >= 3 && B.nlive > 0 && B.ent_nd != dd_tile_ND(t)) {
-    std::vector<uint32_t> e;
-    dd_entries(B.delta_host, B.ndm, B.nlive, dd_tile_ND(t), e, B.ent_stride);
-    B.d_ent.ensure(e.size() * 4);
-    HAMD_CHECK(hipMemcpy(B.d_ent.p, e.data(), e.size() * 4, hipMemcpyHostToDevice));
-    B.ent = B.d_ent.as<uint32_t>();
-    B.ent_nd = dd_tile_ND(t);
-  }
+  ensure_entries(B, t);
   B.ntiles = 0;
   if (B.nlive == 0) return;
   DDBeamDev dd{};
@@ -976,6 +980,18 @@ void ctx_autotune(hella_ctx c, const char *path)
           if (!dd_tile_fits(t, sp, min_ring)) fits = false;
         }
         if (!fits) continue;
+        if (dma >= 3) {
+          // this shape's entry tables
+          size_t j = 0;
+          for (size_t b = 0; b < c->beams.size(); ++b) {
+            BeamState &B = c->beams[b];
+            if (!B.nlive) continue;
+            ensure_entries(B, t);
+            dd[j++] = make_ddesc(c->cfg, c->W, B, c->ws[b]);
+          }
+          HAMD_CHECK(hipMemcpy(d_beams.p, dd.data(), dd.size() * sizeof(DDBeamDev),
+                               hipMemcpyHostToDevice));
+        }
         std::vector<DDTileDesc> tiles;
         dd_build_tiles(dd, t, tiles, order);
         d_tiles.ensure(tiles.size() * sizeof(DDTileDesc));
