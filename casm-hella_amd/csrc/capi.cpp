@@ -375,17 +375,21 @@ hella_status hella_set_tile(hella_ctx c, int lanes_per_dm, int chunk, int order)
   if (c->built) return fail(HELLA_ERR_STATE, "set the tile before the first gulp");
   // order bits 1-2: loader (0 LDS DMA, 1 registers, 2 register prefetch,
   // 3 one DM trial per wave: 64 lanes, chunk = 4 or 8 waves per workgroup);
-  // bit 3 with loader 3: a chunk's LDS reads all issued before its sums
+  // bit 3 with loader 3: a chunk's LDS reads all issued before its sums;
+  // bit 4 with loader 3 and chunk 4: windows copied by LDS DMA two chunks ahead
   const int loader = (order >> 1) & 3;
   const bool pipe = ((order >> 3) & 1) != 0;
+  const bool dma2 = ((order >> 4) & 1) != 0;
   const bool ok = loader == 3
                       ? lanes_per_dm == 64 && (chunk == 4 || chunk == 8)
                       : (lanes_per_dm == 8 || lanes_per_dm == 16 || lanes_per_dm == 32 ||
                          lanes_per_dm == 64) &&
                             (chunk == 8 || chunk == 16 || chunk == 32) && !(loader == 2 && chunk == 32);
-  if (!ok || order < 0 || order > 15 || (pipe && loader != 3))
+  if (!ok || order < 0 || order > 31 || (pipe && loader != 3) ||
+      (dma2 && (loader != 3 || pipe || chunk != 4)))
     return fail(HELLA_ERR_INVALID, "tile shape out of range");
-  c->tile = DDTile{lanes_per_dm, chunk, loader == 0 ? 1 : loader == 1 ? 0 : pipe ? 4 : loader};
+  c->tile = DDTile{lanes_per_dm, chunk,
+                   loader == 0 ? 1 : loader == 1 ? 0 : dma2 ? 5 : pipe ? 4 : loader};
   c->tile_order = order & 1;
   c->tile_forced = true;
   return HELLA_OK;

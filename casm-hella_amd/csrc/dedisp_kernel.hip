@@ -718,6 +718,198 @@ void launch_w(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, i
   }
 }
 
+// Zeros for the windows of channels past the last live one (DMA source).
+__device__ __align__(16) uint8_t dd_zero_page[6 * 1024];
+
+// Workgroup barrier that waits for this wave's LDS reads but not for its
+// LDS-DMA copies in flight (the fence of __syncthreads() would wait for
+// those too); the memory clobber keeps the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void barrier_keep_dma()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// k_dedisp_w with the windows copied global -> LDS by the DMA path of
+// global_load_lds_dwordx4 two chunks ahead, into three statically separate
+// LDS buffers: the sums of chunk i read one buffer while the copies of chunk
+// i + 2 land in another, and the compiler, seeing distinct objects, does not
+// make the reads wait for the copies.  No staging registers.  The chunk loop
+// is unrolled by three so every code site names its buffers statically.
+// Every wave issues exactly NU copies per chunk (channels past the last live
+// one copy zeros), so "at most NU copies outstanding" means the previous
+// chunk's have landed.  Rows are NU * 1024 bytes.  NDW = 4 trials per
+// workgroup; the entries come from the beam's table or the delays as in
+// k_dedisp_w.
+template <int NU>
+__global__ __launch_bounds__(256) void k_dedisp_wd(const DDBeamDev *__restrict__ beams,
+                                                 const DDTileDesc *__restrict__ tiles)
+{
+  constexpr int NDW = 4, CC = 4;
+  constexpr int LROW = NU * 1024;
+  typedef int32_t dvec __attribute__((ext_vector_type(CC), aligned(4)));
+  typedef __attribute__((address_space(4))) const dvec cdvec;
+  typedef uint32_t evec __attribute__((ext_vector_type(2 * CC), aligned(4)));
+  typedef __attribute__((address_space(4))) const evec cevec;
+  __shared__ __align__(16) uint8_t buf0[CC * LROW];
+  __shared__ __align__(16) uint8_t buf1[CC * LROW];
+  __shared__ __align__(16) uint8_t buf2[CC * LROW];
+  const DDTileDesc td = tiles[blockIdx.x];
+  if (td.beam < 0) return;
+  const DDBeamDev B = beams[td.beam];
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int d0 = td.d0, t0 = td.t0;
+  const int ndm = B.ndm, nlive = B.nlive;
+  const int ring = (int)B.ring;
+  const int S = (int)B.start + t0;
+  const int32_t *own_row = B.delta + (int64_t)min(d0 + wave, ndm - 1) * nlive;
+  const int32_t *first_row = B.delta + (int64_t)d0 * nlive;
+  const bool tab = B.ent != nullptr && B.ent_nd == NDW && (S & 15) == 0;
+  const uint32_t *ent_row = tab ? B.ent + (int64_t)min(d0 + wave, ndm - 1) * B.ent_stride * 2 : nullptr;
+  const int nchunks = (nlive + CC - 1) / CC;
+
+  // copy this wave's channel of chunk ci into the wave's row of dst; the
+  // copies are issued from one place (after the source addresses are
+  // chosen), so the compiler keeps track of their LDS destination
+  auto stage = [&](int ci, uint8_t *dst) {
+    const int k = ci * CC + wave;
+    const uint8_t *src[NU];
+    if (k < nlive) {
+      const int bas = sload(first_row, k);
+      int first = S + bas;
+      first -= first >= ring ? ring : 0;
+      const int al = first & ~15;
+      const uint8_t *row = B.base + (int64_t)sload(B.live, k) * B.row_stride;
+      const bool wraps = al + LROW > ring;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        int col = al + 16 * (u * 64 + lane);
+        if (wraps)
+          while (col >= ring) col -= ring;
+        src[u] = row + col;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) src[u] = dd_zero_page + 16 * (u * 64 + lane);
+    }
+    uint8_t *row_lds = dst + wave * LROW;
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+      __builtin_amdgcn_global_load_lds((glb_void_t *)src[u], (lds_void_t *)(row_lds + 1024 * u), 16, 0, 0);
+  };
+
+  uint32_t dw[CC], se[CC];
+  auto entries = [&](int k0) {
+    if (tab) {
+      const evec e = *(cevec *)(ent_row + 2 * k0);
+#pragma unroll
+      for (int cc = 0; cc < CC; ++cc) {
+        dw[cc] = e[2 * cc];
+        se[cc] = e[2 * cc + 1];
+      }
+      return;
+    }
+    const dvec o = *(cdvec *)(own_row + k0);
+    const dvec f = *(cdvec *)(first_row + k0);
+#pragma unroll
+    for (int cc = 0; cc < CC; ++cc) {
+      const int bas = f[cc];
+      const int own = k0 + cc < nlive ? o[cc] : bas;
+      const int q = ((S + bas) & 15) + own - bas;
+      const uint32_t s = (uint32_t)(q & 3);
+      dw[cc] = (uint32_t)(q & ~3);
+      se[cc] = s | 0x0c00u | ((s + 2u) << 16) | 0x0c000000u;
+    }
+  };
+
+  uint32_t acc[V / 2];
+  uint32_t tot[V];
+#pragma unroll
+  for (int i = 0; i < V / 2; ++i) acc[i] = 0;
+#pragma unroll
+  for (int i = 0; i < V; ++i) tot[i] = 0;
+  const int lane_off = lane * V;
+  int since_flush = 0;
+
+  // waitcnt immediates: vmcnt(n), no wait on the other counters
+  constexpr int WAIT_VM_NU = (NU & 15) | (7 << 4) | (15 << 8) | ((NU >> 4) << 14);
+  constexpr int WAIT_VM_0 = (7 << 4) | (15 << 8);
+
+  auto step = [&](int ci, const uint8_t *cur, uint8_t *ahead) {
+    const bool copy = ci + 2 < nchunks;
+#pragma unroll
+    for (int cc = 0; cc < CC; cc += 2) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int c = cc + h;
+        const uint32_t sel_e = se[c], sel_o = se[c] + 0x00010001u;
+        const uint32_t *wrow = reinterpret_cast<const uint32_t *>(cur + c * LROW + dw[c] + lane_off);
+        uint32_t w[NW];
+#pragma unroll
+        for (int i = 0; i < NW; ++i) w[i] = wrow[i];
+#pragma unroll
+        for (int k = 0; k < V / 4; ++k) {
+          acc[2 * k] += perm(w[k + 1], w[k], sel_e);
+          acc[2 * k + 1] += perm(w[k + 1], w[k], sel_o);
+        }
+      }
+    }
+    since_flush += CC;
+    if (since_flush >= 256 || ci + 1 >= nchunks) {
+#pragma unroll
+      for (int k = 0; k < V / 4; ++k) {
+        tot[4 * k + 0] += acc[2 * k] & 0xffffu;
+        tot[4 * k + 2] += acc[2 * k] >> 16;
+        tot[4 * k + 1] += acc[2 * k + 1] & 0xffffu;
+        tot[4 * k + 3] += acc[2 * k + 1] >> 16;
+        acc[2 * k] = 0;
+        acc[2 * k + 1] = 0;
+      }
+      since_flush = 0;
+    }
+    if (ci + 1 < nchunks) entries((ci + 1) * CC);
+    // chunk ci + 2 goes to the buffer every wave finished reading before the
+    // previous barrier
+    if (copy) stage(ci + 2, ahead);
+    // the next chunk's copies (issued one step earlier) have landed when at
+    // most this step's NU copies are outstanding
+    if (copy) __builtin_amdgcn_s_waitcnt(WAIT_VM_NU);
+    else __builtin_amdgcn_s_waitcnt(WAIT_VM_0);
+    barrier_keep_dma();
+  };
+
+  stage(0, buf0);
+  if (nchunks > 1) stage(1, buf1);
+  entries(0);
+  if (nchunks > 1) __builtin_amdgcn_s_waitcnt(WAIT_VM_NU);
+  else __builtin_amdgcn_s_waitcnt(WAIT_VM_0);
+  barrier_keep_dma();
+  for (int ci = 0; ci < nchunks; ci += 3) {
+    step(ci, buf0, buf2);
+    if (ci + 1 >= nchunks) break;
+    step(ci + 1, buf1, buf0);
+    if (ci + 2 >= nchunks) break;
+    step(ci + 2, buf2, buf1);
+  }
+  store_out(B, d0 + wave, t0 + lane * V, tot);
+}
+
+void launch_wd(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int nu,
+               hipStream_t s)
+{
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(256), 0, s, d_beams, d_tiles);
+  };
+  switch (nu < 2 ? 2 : nu) {
+    case 2: go(k_dedisp_wd<2>); break;
+    case 3: go(k_dedisp_wd<3>); break;
+    case 4: go(k_dedisp_wd<4>); break;
+    case 5: go(k_dedisp_wd<5>); break;
+    default: throw HipError{hipErrorInvalidValue, "dedispersion window too wide for k_dedisp_wd", __LINE__};
+  }
+}
+
 template <int G>
 void launch_g(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, int chunk,
               int lrow, int dma, hipStream_t s)
@@ -794,11 +986,13 @@ int dd_max_spread(const int32_t *delta, int ndm, int nlive, int nd)
 static int lrow_for(const DDTile &t, int max_spread)
 {
   const int T = t.lanes_per_dm * V;
-  return (int)round_up((size_t)(T + max_spread + 16 + 4 + 16), 16);
+  const int l = (int)round_up((size_t)(T + max_spread + 16 + 4 + 16), 16);
+  return t.dma == 5 ? (int)round_up((size_t)l, 1024) : l;  // DMA rows: whole 1024-byte units
 }
 
 size_t dd_lds_bytes(const DDTile &t, int max_spread)
 {
+  if (t.dma == 5) return (size_t)3 * t.chunk * lrow_for(t, max_spread);
   if (t.dma >= 3) return (size_t)2 * t.chunk * lrow_for(t, max_spread);
   const int ND = NT / t.lanes_per_dm;
   // window rows, the chunk's delays (register loader), a pad word per
@@ -854,6 +1048,8 @@ bool dd_tile_fits(const DDTile &t, int max_spread, int64_t min_ring)
   if (t.dma < 3) return dd_lds_bytes(t, max_spread) <= dd_lds_budget(t);
   const int lrow = lrow_for(t, max_spread);
   (void)min_ring;  // windows wrap any number of times
+  if (t.dma == 5)  // three static buffers of 4 rows of up to 5 KB
+    return t.lanes_per_dm == 64 && t.chunk == 4 && lrow <= 5 * 1024;
   return t.lanes_per_dm == 64 && (t.chunk == 4 || t.chunk == 8) &&
          dd_lds_bytes(t, max_spread) <= dd_lds_budget(t) && lrow <= 6 * 1024;
 }
@@ -869,7 +1065,9 @@ void dd_launch(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
       throw HipError{hipErrorInvalidValue, "dedispersion tile (one trial per wave) does not fit",
                      __LINE__};
     const int lrow = lrow_for(t_in, max_spread);
-    if (t_in.dma == 4) {
+    if (t_in.dma == 5) {
+      launch_wd(d_beams, d_tiles, ntiles, lrow / 1024, s);
+    } else if (t_in.dma == 4) {
       if (t_in.chunk == 4) launch_w<4, true>(d_beams, d_tiles, ntiles, lrow, s);
       else launch_w<8, true>(d_beams, d_tiles, ntiles, lrow, s);
     } else {

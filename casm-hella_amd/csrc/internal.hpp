@@ -144,7 +144,8 @@ struct DDTile {
                      // 2: next chunk prefetched in registers (chunk 8 or 16);
                      // 3: one DM trial per wave (lanes_per_dm 64): chunk = 4 or 8 waves,
                      //    i.e. trials per workgroup, scalar delay arithmetic;
-                     // 4: as 3, every LDS read of a chunk issued before its sums
+                     // 4: as 3, every LDS read of a chunk issued before its sums;
+                     // 5: as 3 (chunk 4), windows copied by LDS DMA two chunks ahead
 };
 
 int dd_samples_per_thread();

@@ -962,6 +962,7 @@ void ctx_autotune(hella_ctx c, const char *path)
   HAMD_CHECK(hipEventCreate(&e1));
   DevBuf d_tiles;
   std::vector<DDTile> shapes;
+  shapes.push_back(DDTile{64, 4, 5});
   for (int dma : {3, 4})
     for (int cc : {8, 4}) shapes.push_back(DDTile{64, cc, dma});
   for (int dma : {2, 1, 0})

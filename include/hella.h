@@ -163,7 +163,8 @@ hella_status hella_autotune(hella_ctx ctx, const char *path);
  * bits 1-2: window loader (0 LDS DMA, 1 through registers, 2 next chunk
  * prefetched in registers, chunk 8 or 16; 3 one DM trial per wave: lanes 64,
  * chunk 4 or 8 = DM trials per workgroup); order bit 3 (loader 3 only): issue
- * every LDS read of a chunk before its sums */
+ * every LDS read of a chunk before its sums; order bit 4 (loader 3, chunk 4):
+ * copy the windows by LDS DMA two chunks ahead */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
 
 /* ---- candidate emission (src/hella/src/peaks.cu:129-259) ----------------- */

@@ -213,10 +213,13 @@ def test_injection_recovery_c1(hamd, orc):
 @pytest.mark.parametrize("chunk,order,kill,W", [(8, 1, (), 4096), (4, 0, (), 4096),
                                                 (8, 0, (3, 11, 40), 4096), (4, 1, (0, 63, 7), 4096),
                                                 (4, 9, (0, 63, 7), 4096), (8, 8, (3, 11, 40), 4096),
-                                                (4, 1, (0, 63, 7), 4000), (8, 9, (), 4008)])
+                                                (4, 1, (0, 63, 7), 4000), (8, 9, (), 4008),
+                                                (4, 17, (), 4096), (4, 16, (0, 63, 7), 4096),
+                                                (4, 17, (3, 11, 40), 4008)])
 def test_wave_per_trial_loader(hamd, orc, chunk, order, kill, W):
     """One DM trial per wave (scalar delay arithmetic, double-buffered
-    staging; order bit 3: all LDS reads of a chunk first): exact sums, also
+    staging; order bit 3: all LDS reads of a chunk first; bit 4: LDS DMA
+    staging two chunks ahead): exact sums, also
     when the live channels do not fill the last chunk and the DM count is not
     a multiple of the trials per workgroup.  Gulps that are multiples of 16
     take the precomputed entry tables, the others derive the entries."""
