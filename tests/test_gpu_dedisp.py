@@ -78,3 +78,45 @@ def test_c1_geometry(hamd, orc):
     got = lp.execute_adv(data)
     ref = orc.dedisperse_tf(data, op.delta, ns - op.max_delay)
     np.testing.assert_array_equal(got, ref.astype(np.float32))
+
+
+def test_stress_geometry_full_size_spot_checks(hamd, orc):
+    """The 4096-channel stress geometry at full size through the dedisp
+    interface with device pointers: 4096 ch over 500->400 MHz, 65.536 us,
+    DM 0-5000 (7511 trials, max delay 712492), 262144 output samples per
+    trial (8.1e12 integer adds).  Random bytes in; 2048 (trial, sample)
+    entries spread over 64 trials (the first and the last among them) are
+    checked against direct sums of the input at the oracle's delays, and the
+    whole row of trial 0 against the channel sums."""
+    nch, dt, nout = 4096, 65.536e-6, 262144
+    lp, op = _plan(hamd, orc, nch, dt, 500.0, -100.0 / nch, 0.0, 5000.0)
+    assert op.dms.size == 7511 and op.max_delay == 712492
+    ns = op.max_delay + nout
+    data = np.frombuffer(np.random.default_rng(5000).bytes(ns * nch), np.uint8).reshape(ns, nch)
+    din = hamd.DeviceBuffer(data.nbytes)
+    din.upload(data)
+    ndm = op.dms.size
+    dout = hamd.DeviceBuffer(ndm * nout * 4)
+    print(f"# stress geometry: ns={ns} nch={nch} ndm={ndm}", flush=True)
+    code = hamd.lib().dedisp_execute_adv(lp._p, ns, din.ptr, 8, nch, dout.ptr, 32, nout * 4, 1 << 2)
+    assert code == 0
+    hamd.device_sync()
+    rng = np.random.default_rng(7)
+    rows = np.unique(np.concatenate([[0, ndm - 1], rng.choice(ndm, 62, replace=False)]))
+    chan = np.arange(nch)
+    for d in rows:
+        row = np.zeros(nout, np.float32)
+        hamd.lib().hella_memcpy(row.ctypes.data, dout.ptr + int(d) * nout * 4, nout * 4, 2)
+        ts = rng.integers(0, nout, 32)
+        for t in ts:
+            want = int(data[t + op.delta[d], chan].astype(np.int64).sum())
+            assert row[t] == want, (d, t)
+        if d == 0:
+            # DM 0: every sample of the row is the channel sum of that sample
+            assert not op.delta[0].any()
+            want_row = np.zeros(nout, np.int64)
+            for t0 in range(0, nout, 16384):
+                want_row[t0:t0 + 16384] = data[t0:t0 + 16384].sum(axis=1, dtype=np.int64)
+            np.testing.assert_array_equal(row, want_row.astype(np.float32))
+    din.free()
+    dout.free()
