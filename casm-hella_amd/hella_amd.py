@@ -215,6 +215,17 @@ class _ServerStats(C.Structure):
 
 RING_LAST, RING_END = 1, 2
 
+class FilHeader(C.Structure):
+    _fields_ = ([(n, _I) for n in ("machine_id", "telescope_id", "data_type", "nchans", "nbits",
+                                   "nifs", "barycentric", "pulsarcentric", "ibeam", "nbeams")] +
+                [(n, C.c_double) for n in ("tstart", "tsamp", "fch1", "foff", "refdm", "az_start",
+                                           "za_start", "src_raj", "src_dej", "period")] +
+                [("isign", C.c_char), ("npuls", C.c_longlong), ("source_name", C.c_char * 256),
+                 ("raw_data_file", C.c_char * 256), ("nfreq", _I),
+                 ("frequency_table", C.c_double * 4096), ("header_bytes", C.c_longlong),
+                 ("nsamples", C.c_longlong)])
+
+
 CAND_DTYPE = np.dtype([("snr", "f4"), ("beam", "i4"), ("samp", "i4"), ("width", "i4"),
                        ("dm_idx", "i4"), ("dm", "f4"), ("abs_samp", "i8")])
 CLUSTER_DTYPE = np.dtype([("rep", "i4"), ("count", "i4"), ("samp_min", "i4"), ("samp_max", "i4"),
@@ -288,6 +299,10 @@ def _declare_hella(L):
     d("hella_format_coincidencer", _I, C.c_longlong, C.c_longlong, C.c_char_p, C.POINTER(_Cand),
       _I, _I, C.c_char_p, _I)
     d("hella_write_specflags", _I, C.c_char_p, C.POINTER(C.c_uint), _I)
+    d("hella_sigproc_read_header", _I, C.c_char_p, C.POINTER(FilHeader))
+    d("hella_sigproc_read", C.c_longlong, C.c_char_p, C.POINTER(FilHeader), C.c_longlong,
+      C.c_longlong, P)
+    d("hella_sigproc_config", _I, C.POINTER(FilHeader), C.c_char_p, _I)
 
 
 _orig_declare = _declare
@@ -700,3 +715,33 @@ def write_specflags(path, counts):
     a = np.ascontiguousarray(counts, np.uint32)
     _hchk(lib().hella_write_specflags(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_uint)),
                                       int(a.size)))
+
+
+# ---------------------------------------------------------------- SIGPROC input
+class FilterbankFile:
+    """A SIGPROC filterbank file: its header and time-major uint8 samples."""
+
+    def __init__(self, path):
+        self.path = str(path)
+        self.hdr = FilHeader()
+        _hchk(lib().hella_sigproc_read_header(self.path.encode(), C.byref(self.hdr)))
+
+    def __getattr__(self, name):
+        if name in dict(FilHeader._fields_):
+            v = getattr(self.hdr, name)
+            return v.decode() if isinstance(v, bytes) else v
+        raise AttributeError(name)
+
+    def read(self, first, nsamp):
+        out = np.zeros((nsamp, self.hdr.nchans), np.uint8)
+        n = lib().hella_sigproc_read(self.path.encode(), C.byref(self.hdr), first, nsamp,
+                                     out.ctypes.data)
+        if n < 0:
+            raise HellaError(lib().hella_last_error().decode())
+        return out[:n]
+
+    def config(self):
+        n = lib().hella_sigproc_config(C.byref(self.hdr), None, 0)
+        buf = C.create_string_buffer(n + 1)
+        lib().hella_sigproc_config(C.byref(self.hdr), buf, n + 1)
+        return buf.value.decode()

@@ -167,6 +167,37 @@ hella_status hella_autotune(hella_ctx ctx, const char *path);
  * copy the windows by LDS DMA two chunks ahead */
 hella_status hella_set_tile(hella_ctx ctx, int lanes_per_dm, int chunk, int order);
 
+/* ---- SIGPROC filterbank input (src/hella/src/sigproc.cpp:26-191) ------------ */
+#define HELLA_FIL_MAXFREQ 4096
+/* the readable header parameters (sigproc.h:17-49), plus where the data start
+ * and how many whole samples the file holds */
+typedef struct {
+  int machine_id, telescope_id, data_type, nchans, nbits, nifs, barycentric, pulsarcentric;
+  int ibeam, nbeams;
+  double tstart, tsamp, fch1, foff, refdm, az_start, za_start, src_raj, src_dej, period;
+  char isign;
+  long long npuls;
+  char source_name[256];
+  char raw_data_file[256];
+  int nfreq;                  /* fchannel entries read (fch1 = foff = 0 then) */
+  double frequency_table[HELLA_FIL_MAXFREQ];
+  long long header_bytes;     /* 0: the file has no SIGPROC header */
+  long long nsamples;         /* whole samples after the header */
+} hella_fil_header;
+/* Parse the keyword header of a SIGPROC file (read_header, sigproc.cpp:58-191):
+ * HELLA_ERR_CONFIG for an unknown keyword or a truncated header, as the
+ * reference throws. */
+hella_status hella_sigproc_read_header(const char *path, hella_fil_header *hdr);
+/* Read nsamp samples from sample `first` on into out ([nsamp][nchans] uint8,
+ * the time-major layout the pipeline takes as FILTERBANK/FILE input).  Only
+ * 8-bit, single-IF files (sigproc.cpp:184-188).  Returns the samples read
+ * (fewer at the end of the file), or -1. */
+long long hella_sigproc_read(const char *path, const hella_fil_header *hdr, long long first,
+                             long long nsamp, void *out);
+/* The geometry keys of the configuration ("NCHAN .. FCH1 .. FOFF .. TSAMP ..")
+ * from a header; returns the length (as snprintf), -1 on a bad argument. */
+int hella_sigproc_config(const hella_fil_header *hdr, char *buf, int cap);
+
 /* ---- candidate emission (src/hella/src/peaks.cu:129-259) ----------------- */
 /* Per-channel count of flagged samples of the beam (the grown-mask counts of
  * every 1 x 1 / box scrunch, accumulated since the context was created; the
