@@ -208,6 +208,56 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
   }
 }
 
+// P_Y0 when the first width is the raw dedispersed sample (BOXCAR, minW 1):
+// the statistics of y = normed(D[c0 + j]) over a run of bpw blocks of one
+// row, read straight from the plane (lane l takes samples l + 256k of a
+// block, as k_stats does), the next block's samples loaded while the
+// current block's are reduced; only the tree uses LDS.
+__global__ __launch_bounds__(NT) void k_y0_direct(const SearchBeamDev *__restrict__ beams,
+                                                  SearchCommon C, int bpw)
+{
+  const SearchBeamDev &B = beams[blockIdx.z];
+  const int r = blockIdx.y;
+  if (r >= B.rows) return;
+  const int b0 = blockIdx.x * bpw, b1 = min(C.nblk, b0 + bpw);
+  if (b0 >= b1) return;
+  const int l = threadIdx.x;
+  __shared__ double shd[2 * 128];
+  const SScal &sc = *B.scal;
+  const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds + C.maxW / 2;
+  constexpr int K = SB / NT;
+  uint32_t nx[K];
+  auto fetch = [&](int b) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = b * SB + l + NT * k;
+      nx[k] = j < C.nout ? drow[j] : 0u;
+    }
+  };
+  fetch(b0);
+  for (int b = b0; b < b1; ++b) {
+    uint32_t cur[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) cur[k] = nx[k];
+    if (b + 1 < b1) fetch(b + 1);
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = b * SB + l + NT * k;
+      if (j >= C.nout || j >= C.wp) break;
+      const double v = (double)normed((double)cur[k], sc, 0);
+      acc[0] += v;
+      acc[1] += v * v;
+    }
+    tree256_multi(acc, shd);
+    if (l == 0) {
+      const int64_t slot = (int64_t)r * C.nblk + b;
+      B.part[slot] = acc[0];
+      B.part2[slot] = acc[1];
+    }
+  }
+}
+
 __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t bhi)
 {
   const uint64_t t = lo + blo;
@@ -795,6 +845,8 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
     if (p == P_Y0) {
       if (hella)
         launch_stats<true>(d_beams, C, p, bpw, gs, s);
+      else if (C.minW == 1)
+        hipLaunchKernelGGL(k_y0_direct, gs, dim3(NT), 0, s, d_beams, C, bpw);
       else
         launch_stats<false>(d_beams, C, p, bpw, gs, s);
     } else if (hella) {
