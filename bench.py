@@ -260,6 +260,8 @@ def main():
                     help="length of the real-time run (0 skips it)")
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--stress", type=int, default=0,
+                    help="only the 4096-channel FP32 stress configuration, this many timed chunks")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has a single MI355X "
@@ -270,6 +272,9 @@ def main():
     if hamd.device_count() < 1:
         raise SystemExit("no GPU visible")
     pulses, rfi = stream_params()
+    if args.stress:
+        print(json.dumps({"stress": stress(hamd, args.stress)}), flush=True)
+        return
     if args.serve_only:
         print(json.dumps({"serving": serving(hamd, args, pulses, rfi)}), flush=True)
         return
@@ -406,6 +411,58 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hamd, host_chunks[0], args.cpu_rows)
     print(json.dumps(out), flush=True)
+
+
+def stress(hamd, gulps):
+    """The 4096-channel FP32 stress configuration, one beam on the plain path:
+    500->400 MHz, 65.536 us, 262144-sample chunks (17.18 s), DM 0-5000, boxcar
+    widths 1..1024, N(0,1) input with 20% of the channels at 30 sigma and
+    broadband bursts every second, a 1x1 scrunch at 6 sigma."""
+    nch, tsamp, W = 4096, 65.536e-6, 262144
+    cfg = dict(INPUT="FILTERBANK", INPUT_DTYPE="F32", NCHAN=nch, FCH1=500.0, FOFF=-100.0 / nch,
+               TSAMP=tsamp, GULP=W, DM_MIN=0, DM_MAX=5000, DM_TOL=1.25, PULSE_WIDTH=64,
+               WIDTH_MIN=1, WIDTH_MAX=1024, SNR=7.0, NBEAM=1, SCRUNCH=[(1, 1, 6.0)])
+    h = hamd.Hella(cfg)
+    info = h.info(0)
+    rng = np.random.default_rng(5)
+    rfi = rng.choice(nch, nch // 5, replace=False)
+    pulses = [(float(rng.uniform(1, 16)), float(rng.uniform(100, 4500)), 2.0, 30.0) for _ in range(5)]
+    t0 = time.perf_counter()
+    bufs = []
+    for i in range(2):
+        c = hamd.synth(nch, W, 500.0, -100.0 / nch, tsamp, seed=77 + i, t_offset=i * W,
+                       dtype="f32", mean=0.0, sigma=1.0, rfi_chan=rfi, rfi_mult=30.0,
+                       pulses=pulses, burst_period=int(round(1.0 / tsamp)), burst_phase=500,
+                       burst_len=10, burst_amp=6.0)
+        b = hamd.DeviceBuffer(c.nbytes)
+        b.upload(c)
+        bufs.append(b)
+        del c
+    print(f"# stress: input ready in {time.perf_counter() - t0:.1f} s; ndm={info['ndm']} "
+          f"max_delay={info['max_delay']} min_gulp={info['min_gulp']}", file=sys.stderr, flush=True)
+    need = info["min_gulp"] + 1
+    for g in range(need):
+        h.process_gulp(bufs[g % 2])
+    hamd.device_sync()
+    times, crossings, ncand = [], [], []
+    for g in range(gulps):
+        ts = time.perf_counter()
+        ncand.append(h.process_gulp(bufs[(need + g) % 2]))
+        times.append((time.perf_counter() - ts) * 1e3)
+        crossings.append(h.crossings(0))
+    h.set_timing(True)
+    h.process_gulp(bufs[0])
+    stages = dict(zip(["flag", "dedisperse", "search", "peaks"], h.stage_times()))
+    h.close()
+    chunk_s = W * tsamp
+    ms = float(np.mean(times))
+    adds = info["ndm"] * info["ntime_dd"] * info["nlive"]
+    return {"workload": "stress: 4096 ch FP32, 262144-sample chunks (17.18 s), DM 0-5000 "
+                        f"({info['ndm']} trials, max delay {info['max_delay']}), widths 1..1024",
+            "ms_per_chunk": ms, "realtime_factor": chunk_s / (ms * 1e-3),
+            "beams_per_gpu": int(chunk_s / (ms * 1e-3)), "stages_ms": stages,
+            "dedisp_Tadd_s": adds / (stages["dedisperse"] * 1e-3) / 1e12,
+            "raw_crossings_per_chunk": crossings, "candidates_per_chunk": ncand}
 
 
 def serving(hamd, args, pulses, rfi):
