@@ -260,6 +260,8 @@ def main():
                     help="length of the real-time run (0 skips it)")
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--uneven", type=int, default=0,
+                    help="only the uneven multi-beam configuration, this many beams per kind")
     ap.add_argument("--stress", type=int, default=0,
                     help="only the 4096-channel FP32 stress configuration, this many timed chunks")
     args = ap.parse_args()
@@ -272,6 +274,9 @@ def main():
     if hamd.device_count() < 1:
         raise SystemExit("no GPU visible")
     pulses, rfi = stream_params()
+    if args.uneven:
+        print(json.dumps({"uneven": uneven(hamd, args.uneven)}), flush=True)
+        return
     if args.stress:
         print(json.dumps({"stress": stress(hamd, args.stress)}), flush=True)
         return
@@ -463,6 +468,82 @@ def stress(hamd, gulps):
             "beams_per_gpu": int(chunk_s / (ms * 1e-3)), "stages_ms": stages,
             "dedisp_Tadd_s": adds / (stages["dedisperse"] * 1e-3) / 1e12,
             "raw_crossings_per_chunk": crossings, "candidates_per_chunk": ncand}
+
+
+def uneven(hamd, per_group, reps=3):
+    """The uneven multi-beam configuration, per_group beams of each kind:
+    512 ch / DM 0-1000, 1024 ch / DM 0-2000, 2048 ch / DM 0-3000
+    (500->400 MHz, 131.072 us), 65536-sample chunks with each beam's own
+    max-DM window, seeded killmasks of 0-10% of the channels, pulses in a
+    quarter of the beams.  Once every window is full, a chunk of all beams
+    through the batched graph-captured chain (after the autotuner) against
+    the plain per-beam sequence on the same chunks."""
+    groups = [(512, 1000.0), (1024, 2000.0), (2048, 3000.0)]
+    beams = [g for g in groups for _ in range(per_group)]
+    nb = len(beams)
+    cfg = dict(INPUT="FILTERBANK", NCHAN=1024, FCH1=500.0, FOFF=-100.0 / 1024, TSAMP=TSAMP,
+               GULP=CHUNK, DM_MIN=0, DM_MAX=2000, DM_TOL=1.25, PULSE_WIDTH=64, WIDTH_MIN=1,
+               WIDTH_MAX=256, SNR=7.0, NBEAM=nb, SCRUNCH=[(1, 1, 6.0)])
+    rng = np.random.default_rng(3)
+    kms = []
+    for nch, _ in beams:
+        km = np.ones(nch, np.int32)
+        km[rng.choice(nch, int(rng.uniform(0, 0.1) * nch), replace=False)] = 0
+        kms.append(km)
+    pulses = [[(float(rng.uniform(1, 8)), float(rng.uniform(50, 0.9 * dmx)), 2.0, 15.0)]
+              if rng.random() < 0.25 else [] for _, dmx in beams]
+    t0 = time.perf_counter()
+    bufs = []
+    for g in range(2):
+        parts = [hamd.synth(nch, CHUNK, 500.0, -100.0 / nch, TSAMP, seed=900 + b,
+                            t_offset=g * CHUNK, pulses=pulses[b]).ravel()
+                 for b, (nch, _) in enumerate(beams)]
+        buf = hamd.DeviceBuffer(sum(p.nbytes for p in parts))
+        buf.upload(np.concatenate(parts))
+        bufs.append(buf)
+    print(f"# uneven: {nb} beams, input ready in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+
+    def ctx():
+        h = hamd.Hella(cfg)
+        for b, (nch, dmx) in enumerate(beams):
+            h.set_beam(b, NCHAN=nch, FOFF=-100.0 / nch, DM_MAX=dmx)
+            h.set_killmask(b, kms[b])
+        return h
+
+    def run(h, batched):
+        need = max(h.info(b)["min_gulp"] for b in range(nb)) + 1
+        for g in range(need):
+            h.process_gulp(bufs[g % 2], batched=batched)
+        if batched:
+            h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_uneven.json"))
+        h.process_gulp(bufs[need % 2], batched=batched)
+        hamd.device_sync()
+        ts = time.perf_counter()
+        nc = []
+        for k in range(reps):
+            nc.append(h.process_gulp(bufs[(need + 1 + k) % 2], batched=batched))
+        hamd.device_sync()
+        return (time.perf_counter() - ts) / reps, nc, h.peaks()
+
+    hb = ctx()
+    t_b, nc_b, pk_b = run(hb, True)
+    info = [hb.info(b) for b in range(nb)]
+    hb.close()
+    hp = ctx()
+    t_p, nc_p, pk_p = run(hp, False)
+    hp.close()
+    for b in bufs:
+        b.free()
+    adds = sum(i["ndm"] * i["ntime_dd"] * i["nlive"] for i in info)
+    return {"workload": f"uneven multi-beam: {nb} beams ({per_group} each of 512 ch/DM 1000, "
+                        "1024 ch/DM 2000, 2048 ch/DM 3000), killmasks 0-10%, 65536-sample chunks",
+            "batched_ms_per_chunk": t_b * 1e3, "plain_ms_per_chunk": t_p * 1e3,
+            "batched_over_plain": t_p / t_b, "dedisp_adds_per_chunk": adds,
+            "batched_Tadd_s_end_to_end": adds / t_b / 1e12,
+            "realtime_factor_batched": CHUNK * TSAMP / t_b,
+            "candidates": {"batched": nc_b, "plain": nc_p},
+            "last_chunk_identical": bool(np.array_equal(pk_b, pk_p))}
 
 
 def serving(hamd, args, pulses, rfi):

@@ -1006,10 +1006,20 @@ void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
                     std::vector<DDTileDesc> &tiles, int order)
 {
   const int T = dd_tile_T(t), ND = dd_tile_ND(t);
+  // the dispatcher hands workgroup i to XCD i % 8.  Every beam's tile list is
+  // cut into 8 contiguous runs, one per XCD, so neighbouring time tiles
+  // (overlapping windows) share that XCD's L2, and every XCD gets the same
+  // share of every beam: beams of different channel counts cost different
+  // amounts per tile, and a split of the whole list by count would leave the
+  // XCDs holding the widest beams with most of the work.  Each beam's runs
+  // are padded to equal length with empty tiles (at most 7 per beam).
+  const size_t nx = 8;
+  tiles.clear();
   std::vector<DDTileDesc> list;
   for (size_t b = 0; b < beams.size(); ++b) {
     const DDBeamDev &B = beams[b];
     if (B.nout <= 0 || B.ndm <= 0 || B.nlive <= 0) continue;
+    list.clear();
     if (order == 0) {
       for (int d0 = 0; d0 < B.ndm; d0 += ND)
         for (int t0 = 0; t0 < B.nout; t0 += T) list.push_back(DDTileDesc{(int)b, d0, t0, 0});
@@ -1017,14 +1027,10 @@ void dd_build_tiles(const std::vector<DDBeamDev> &beams, const DDTile &t,
       for (int t0 = 0; t0 < B.nout; t0 += T)
         for (int d0 = 0; d0 < B.ndm; d0 += ND) list.push_back(DDTileDesc{(int)b, d0, t0, 0});
     }
+    const size_t q = (list.size() + nx - 1) / nx, base = tiles.size();
+    tiles.resize(base + q * nx, DDTileDesc{-1, 0, 0, 0});
+    for (size_t k = 0; k < list.size(); ++k) tiles[base + (k % q) * nx + k / q] = list[k];
   }
-  // the dispatcher hands workgroup i to XCD i % 8: give each XCD a contiguous
-  // run of the list so neighbouring time tiles (overlapping windows) share
-  // that XCD's L2
-  const size_t nx = 8;
-  const size_t q = (list.size() + nx - 1) / nx;
-  tiles.assign(q * nx, DDTileDesc{-1, 0, 0, 0});
-  for (size_t k = 0; k < list.size(); ++k) tiles[(k % q) * nx + k / q] = list[k];
 }
 
 size_t dd_lds_budget(const DDTile &t)
