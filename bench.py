@@ -261,7 +261,8 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--uneven", type=int, default=0,
-                    help="only the uneven multi-beam configuration, this many beams per kind")
+                    help="only the uneven multi-beam configuration with this many beams "
+                         "(split as evenly as possible over the three kinds; 128 = 43/43/42)")
     ap.add_argument("--stress", type=int, default=0,
                     help="only the 4096-channel FP32 stress configuration, this many timed chunks")
     args = ap.parse_args()
@@ -470,16 +471,18 @@ def stress(hamd, gulps):
             "raw_crossings_per_chunk": crossings, "candidates_per_chunk": ncand}
 
 
-def uneven(hamd, per_group, reps=3):
-    """The uneven multi-beam configuration, per_group beams of each kind:
-    512 ch / DM 0-1000, 1024 ch / DM 0-2000, 2048 ch / DM 0-3000
-    (500->400 MHz, 131.072 us), 65536-sample chunks with each beam's own
-    max-DM window, seeded killmasks of 0-10% of the channels, pulses in a
-    quarter of the beams.  Once every window is full, a chunk of all beams
-    through the batched graph-captured chain (after the autotuner) against
-    the plain per-beam sequence on the same chunks."""
+def uneven(hamd, nbeam, reps=3):
+    """The uneven multi-beam configuration: nbeam beams split as evenly as
+    possible (earlier kinds first) over 512 ch / DM 0-1000, 1024 ch / DM
+    0-2000 and 2048 ch / DM 0-3000 (500->400 MHz, 131.072 us), 65536-sample
+    chunks with each beam's own max-DM window, seeded killmasks of 0-10% of
+    the channels, pulses in a quarter of the beams.  Once every window is
+    full, chunks of all beams through the batched graph-captured chain (after
+    the autotuner, for up to 32 beams) against the plain per-beam sequence on
+    the same chunks.  Every chunk carries the same synthetic samples."""
     groups = [(512, 1000.0), (1024, 2000.0), (2048, 3000.0)]
-    beams = [g for g in groups for _ in range(per_group)]
+    counts = [nbeam // 3 + (1 if k < nbeam % 3 else 0) for k in range(3)]
+    beams = [g for g, n in zip(groups, counts) for _ in range(n)]
     nb = len(beams)
     cfg = dict(INPUT="FILTERBANK", NCHAN=1024, FCH1=500.0, FOFF=-100.0 / 1024, TSAMP=TSAMP,
                GULP=CHUNK, DM_MIN=0, DM_MAX=2000, DM_TOL=1.25, PULSE_WIDTH=64, WIDTH_MIN=1,
@@ -493,16 +496,18 @@ def uneven(hamd, per_group, reps=3):
     pulses = [[(float(rng.uniform(1, 8)), float(rng.uniform(50, 0.9 * dmx)), 2.0, 15.0)]
               if rng.random() < 0.25 else [] for _, dmx in beams]
     t0 = time.perf_counter()
-    bufs = []
-    for g in range(2):
-        parts = [hamd.synth(nch, CHUNK, 500.0, -100.0 / nch, TSAMP, seed=900 + b,
-                            t_offset=g * CHUNK, pulses=pulses[b]).ravel()
-                 for b, (nch, _) in enumerate(beams)]
-        buf = hamd.DeviceBuffer(sum(p.nbytes for p in parts))
-        buf.upload(np.concatenate(parts))
-        bufs.append(buf)
-    print(f"# uneven: {nb} beams, input ready in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
+    sizes = [nch * CHUNK for nch, _ in beams]
+    host = np.empty(sum(sizes), np.uint8)
+    off = 0
+    for b, (nch, _) in enumerate(beams):
+        host[off:off + sizes[b]] = hamd.synth(nch, CHUNK, 500.0, -100.0 / nch, TSAMP, seed=900 + b,
+                                              pulses=pulses[b]).ravel()
+        off += sizes[b]
+    buf = hamd.DeviceBuffer(host.nbytes)
+    buf.upload(host)
+    del host
+    print(f"# uneven: {nb} beams ({counts[0]}/{counts[1]}/{counts[2]}), input ready in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     def ctx():
         h = hamd.Hella(cfg)
@@ -514,15 +519,17 @@ def uneven(hamd, per_group, reps=3):
     def run(h, batched):
         need = max(h.info(b)["min_gulp"] for b in range(nb)) + 1
         for g in range(need):
-            h.process_gulp(bufs[g % 2], batched=batched)
-        if batched:
+            h.process_gulp(buf, batched=batched)
+        if batched and nb <= 32:
             h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_uneven.json"))
-        h.process_gulp(bufs[need % 2], batched=batched)
+        h.process_gulp(buf, batched=batched)
         hamd.device_sync()
+        print(f"# uneven: {'batched' if batched else 'plain'} warm after {need + 1} chunks",
+              file=sys.stderr, flush=True)
         ts = time.perf_counter()
         nc = []
         for k in range(reps):
-            nc.append(h.process_gulp(bufs[(need + 1 + k) % 2], batched=batched))
+            nc.append(h.process_gulp(buf, batched=batched))
         hamd.device_sync()
         return (time.perf_counter() - ts) / reps, nc, h.peaks()
 
@@ -533,11 +540,11 @@ def uneven(hamd, per_group, reps=3):
     hp = ctx()
     t_p, nc_p, pk_p = run(hp, False)
     hp.close()
-    for b in bufs:
-        b.free()
+    buf.free()
     adds = sum(i["ndm"] * i["ntime_dd"] * i["nlive"] for i in info)
-    return {"workload": f"uneven multi-beam: {nb} beams ({per_group} each of 512 ch/DM 1000, "
-                        "1024 ch/DM 2000, 2048 ch/DM 3000), killmasks 0-10%, 65536-sample chunks",
+    return {"workload": f"uneven multi-beam: {nb} beams ({counts[0]} x 512 ch/DM 1000, "
+                        f"{counts[1]} x 1024 ch/DM 2000, {counts[2]} x 2048 ch/DM 3000), "
+                        "killmasks 0-10%, 65536-sample chunks",
             "batched_ms_per_chunk": t_b * 1e3, "plain_ms_per_chunk": t_p * 1e3,
             "batched_over_plain": t_p / t_b, "dedisp_adds_per_chunk": adds,
             "batched_Tadd_s_end_to_end": adds / t_b / 1e12,
