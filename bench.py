@@ -267,8 +267,9 @@ def main():
                     help="only the 4096-channel FP32 stress configuration, this many timed chunks")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this build has a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU only: beams shard across GPUs with no exchange, "
+              "so the multi-GPU figure is the per-GPU one times N; it is not run here",
+              file=sys.stderr)
         sys.exit(2)
 
     hamd = load_pkg()

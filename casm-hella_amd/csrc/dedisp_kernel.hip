@@ -543,31 +543,35 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
   (void)sl;
   u32x4 P[NU];
   const int lane16 = lane * 16;
+  // One load site for every case, so the loads stay in flight until the
+  // store after the sums (loads issued on several paths make the compiler
+  // merge their results with register moves, which wait for the data).  A
+  // channel past the last live one gets a resource of zero records: every
+  // load of it is out of range and returns zeros.
   auto fetch = [&]() {
-    if (sch >= 0) {
-      int first = S + sb;
-      first -= first >= ring ? ring : 0;
-      const int al = first & ~15;
-      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-          (void *)(B.base + (int64_t)sch * B.row_stride), (short)0, ring, 0x00020000);
-      if (al + NU * 1024 <= ring) {
+    const bool live = sch >= 0;
+    int first = S + sb;
+    first -= first >= ring ? ring : 0;
+    const int al = first & ~15;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(B.base + (live ? (int64_t)sch * B.row_stride : 0)), (short)0, live ? ring : 0,
+        0x00020000);
+    int col[NU];
+    if (al + NU * 1024 <= ring) {
 #pragma unroll
-        for (int u = 0; u < NU; ++u)
-          P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, al + lane16 + u * 1024, 0, 0);
-      } else {
-        // the staged units run past the ring end (short rings may wrap more
-        // than once)
-#pragma unroll
-        for (int u = 0; u < NU; ++u) {
-          int col = al + lane16 + u * 1024;
-          while (col >= ring) col -= ring;
-          P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col, 0, 0);
-        }
-      }
+      for (int u = 0; u < NU; ++u) col[u] = al + lane16 + u * 1024;
     } else {
+      // the staged units run past the ring end (short rings may wrap more
+      // than once)
 #pragma unroll
-      for (int u = 0; u < NU; ++u) P[u] = u32x4{0u, 0u, 0u, 0u};
+      for (int u = 0; u < NU; ++u) {
+        int c = al + lane16 + u * 1024;
+        while (c >= ring) c -= ring;
+        col[u] = c;
+      }
     }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col[u], 0, 0);
   };
   auto store = [&](int buf) {
     uint8_t *dst = lds + (size_t)(buf * CC + wave) * lrow + lane16;

@@ -794,7 +794,7 @@ static void launch_stats_box(const SearchBeamDev *d_beams, const SearchCommon &C
 // per CU, the rest of a row's blocks walked with register prefetch
 static int stats_bpw(const SearchCommon &C, int max_rows, int nbeam)
 {
-  const long long wgs_wanted = 256LL * 12;
+  const long long wgs_wanted = 256LL * 24;
   const long long per = (long long)C.nblk * max_rows * nbeam / wgs_wanted;
   return (int)std::max(1LL, std::min((long long)C.nblk, per));
 }
