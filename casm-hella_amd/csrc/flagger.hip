@@ -460,9 +460,48 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
   double acc[2 * NB];
 #pragma unroll
   for (int j = 0; j < 2 * NB; ++j) acc[j] = 0.0;
+  const bool sums = part != FPART_NONE;
+  const bool to_ring = TSD && op == FOP_NORM_QUANT;
+  constexpr int KB = FLAG_BLOCK / NT;
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int base = (blk0 + j) * FLAG_BLOCK;
+    if (base + FLAG_BLOCK <= W && (!sums || base + FLAG_BLOCK <= npart)) {
+      // a block with every sample inside the gulp (and the statistics
+      // columns): the same operations in the same order, with every LDS
+      // read of the block issued before the arithmetic
+      int v[KB];
+      float x[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) v[k] = bytes[j][l + NT * k];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        x[k] = tab[v[k]];
+        cnt += flg[v[k]];
+      }
+      if (TSD) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) x[k] = x[k] / tsb[l + NT * k];
+      }
+      if (to_ring) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          x[k] = (float)(((double)x[k] - sc.mu) * sc.inv);
+          int64_t col = B.col0 + base + l + NT * k;
+          if (col >= B.ring_len) col -= B.ring_len;
+          B.ring[(int64_t)c * B.ring_stride + col] = quant(x[k]);
+        }
+      }
+      if (sums) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          const double dd = (double)x[k];
+          acc[2 * j] += dd;
+          acc[2 * j + 1] += dd * dd;
+        }
+      }
+      continue;
+    }
 #pragma unroll 4
     for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
       const int t = base + l + NT * k;
@@ -471,7 +510,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       float x = tab[v];
       cnt += flg[v];
       if (TSD) x = x / tsb[l + NT * k];
-      if (op == FOP_NORM_QUANT) {
+      if (to_ring) {
         x = (float)(((double)x - sc.mu) * sc.inv);
         int64_t col = B.col0 + t;
         if (col >= B.ring_len) col -= B.ring_len;
