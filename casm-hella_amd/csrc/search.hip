@@ -69,19 +69,29 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // the four waves through wtot.
 __device__ void block_prefix(uint32_t *P, int L, uint32_t *wtot)
 {
+  // thread l owns the seg consecutive words from 1 + l * seg; the loops run
+  // over the largest segment with the tail predicated off, so every LDS read
+  // of a segment is issued before the first addition (u32 sums wrap, and any
+  // order of the additions gives the same words)
+  constexpr int MAXSEG = (SB + MAXBOX + NT - 1) / NT;
   const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
   const int seg = (L + NT - 1) / NT;
-  const int i0 = 1 + l * seg, i1 = min(i0 + seg, L + 1);
+  const int i0 = 1 + l * seg, n = max(0, min(seg, L + 1 - i0));
+  uint32_t v[MAXSEG];
+#pragma unroll
+  for (int i = 0; i < MAXSEG; ++i) v[i] = i < n ? P[i0 + i] : 0u;
   uint32_t run = 0;
-  for (int i = i0; i < i1; ++i) run += P[i];
+#pragma unroll
+  for (int i = 0; i < MAXSEG; ++i) run += v[i];
   const uint32_t incl = wave_incl_scan(run);
   if (lane == 63) wtot[wv] = incl;
   __syncthreads();
   uint32_t acc = incl - run;
   for (int w = 0; w < wv; ++w) acc += wtot[w];
-  for (int i = i0; i < i1; ++i) {
-    acc += P[i];
-    P[i] = acc;
+#pragma unroll
+  for (int i = 0; i < MAXSEG; ++i) {
+    acc += v[i];
+    if (i < n) P[i0 + i] = acc;
   }
   __syncthreads();
 }
@@ -91,11 +101,20 @@ __device__ __forceinline__ float normed(double raw, const SScal &sc, int k)
   return sc.sigf[k] > 0.f ? (float)((raw - sc.mu[k]) * sc.inv[k]) : (float)raw;
 }
 
+// largest value of the wave, in every lane: the maximum within each 16-lane
+// row by four DPP steps (lane pairs, quads, half rows, rows), then the four
+// row maxima read into scalar registers
 __device__ __forceinline__ uint32_t wave_max(uint32_t x)
 {
-#pragma unroll
-  for (int s = 32; s >= 1; s >>= 1) x = max(x, (uint32_t)__shfl_down(x, s, 64));
-  return x;
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));  // row_mirror
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
+  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
+  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
+  const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+  return max(max(a, b), max(c, d));
 }
 
 constexpr int MAXL = SB + MAXBOX;        // samples a block's boxes read
