@@ -117,6 +117,18 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t x)
   return max(max(a, b), max(c, d));
 }
 
+// sum of the wave (u32, wrapping), in every lane: DPP steps within each
+// 16-lane row, then the four row sums read into scalar registers
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 0) + (uint32_t)__builtin_amdgcn_readlane((int)x, 16) +
+         (uint32_t)__builtin_amdgcn_readlane((int)x, 32) + (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+}
+
 constexpr int MAXL = SB + MAXBOX;        // samples a block's boxes read
 constexpr int PF = (MAXL + NT - 1) / NT; // prefetch registers per thread
 constexpr int PN = MAXL + 1;
@@ -494,9 +506,7 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
 #pragma unroll
     for (int k = 0; k < SEARCH_MAXW; ++k) {
       if (k < C.nw) {
-        uint32_t x = cnt[k];
-#pragma unroll
-        for (int s = 32; s >= 1; s >>= 1) x += __shfl_down(x, s, 64);
+        const uint32_t x = wave_sum(cnt[k]);
         if ((l & 63) == 0 && x) atomicAdd(&shc[k], x);
       }
     }
@@ -865,7 +875,9 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
       if (hella)
         launch_stats<true>(d_beams, C, p, bpw, gs, s);
       else if (C.minW == 1)
-        hipLaunchKernelGGL(k_y0_direct, gs, dim3(NT), 0, s, d_beams, C, bpw);
+        // one block per workgroup: a plain stream over the plane with every
+        // block's loads in flight at once (its partials are per block)
+        hipLaunchKernelGGL(k_y0_direct, gp, dim3(NT), 0, s, d_beams, C, 1);
       else
         launch_stats<false>(d_beams, C, p, bpw, gs, s);
     } else if (hella) {
