@@ -334,6 +334,7 @@ SearchCommon search_common(const Config &cfg, const BeamState &B0, int nw)
   C.nblk = (C.nout + SEARCH_BLOCK - 1) / SEARCH_BLOCK;
   C.wp = (C.nout / 256) * 256;
   if (C.wp == 0) C.wp = C.nout;
+  C.box_max = (int64_t)cfg.width_max * 255 * B0.nch;
   return C;
 }
 
@@ -398,6 +399,7 @@ void ctx_build(hella_ctx c)
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
   c->d_in.ensure((size_t)c->gulp_bytes);
   c->common = search_common(cfg, c->beams[0], c->nw);
+  c->common.box_max = (int64_t)cfg.width_max * 255 * c->max_nch;
   for (auto &B : c->beams) build_plain_tiles(B, c->tile, c->tile_order);
   HAMD_CHECK(hipStreamSynchronize(s));
   c->built = true;

@@ -304,7 +304,11 @@ __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo,
 // Every box sum is below 2^30 (checked on the host), so a thread's 16
 // squares of one block fit a u64 before they are folded into the u128.
 // W1: widths 1, 2, 4, ... (WIDTH_MIN 1), compile-time LDS offsets.
-template <int NW, bool W1>
+// NARROW: every box sum is so small that a thread's squares over all its
+// blocks fit a u64 (box_max^2 x 16 x bpw < 2^64, checked on the host); the
+// thread then keeps one u64 per width and only the reduction widens to u128,
+// which frees the registers of the per-thread u128 sums.
+template <int NW, bool W1, bool NARROW>
 __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C, int bpw)
 {
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   uint64_t s1[NW], q[NW], s2lo[NW], s2hi[NW];
   uint32_t mx[NW];
 #pragma unroll
-  for (int k = 0; k < NW; ++k) s1[k] = s2lo[k] = s2hi[k] = 0ull;
+  for (int k = 0; k < NW; ++k) s1[k] = s2lo[k] = s2hi[k] = q[k] = 0ull;
   uint32_t nx[PF];
   auto fetch = [&](int b) {
     const int base = b * SB;
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
     block_prefix(P, L, wtot);
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
-      q[k] = 0ull;
+      if (!NARROW) q[k] = 0ull;
       mx[k] = 0u;
     }
     // iterations whose 256 samples are all searched / all in the statistics
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
     }
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
-      add128(s2lo[k], s2hi[k], q[k], 0ull);
+      if (!NARROW) add128(s2lo[k], s2hi[k], q[k], 0ull);
       const uint32_t m = wave_max(mx[k]);
       if (lane == 0) B.bmax[(((int64_t)k * B.rows + r) * C.nblk + b) * 4 + wv] = m;
     }
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   // integer totals: wave sums by shuffles, then the four waves
 #pragma unroll
   for (int k = 0; k < NW; ++k) {
-    uint64_t a = s1[k], lo = s2lo[k], hi = s2hi[k];
+    uint64_t a = s1[k], lo = NARROW ? q[k] : s2lo[k], hi = NARROW ? 0ull : s2hi[k];
 #pragma unroll
     for (int sft = 32; sft >= 1; sft >>= 1) {
       a += __shfl_down(a, sft, 64);
@@ -802,13 +806,20 @@ static void launch_stats(const SearchBeamDev *d_beams, const SearchCommon &C, in
 static void launch_stats_box(const SearchBeamDev *d_beams, const SearchCommon &C, int bpw,
                              dim3 grid, hipStream_t s)
 {
+  // a thread adds at most 16 squares per block over bpw blocks
+  const unsigned __int128 bm = (unsigned __int128)(uint64_t)std::max<int64_t>(C.box_max, 0);
+  const bool narrow = C.box_max > 0 && bm * bm * 16u * (unsigned)bpw < ((unsigned __int128)1 << 64);
   switch (C.nw) {
-#define HAMD_BOX_CASE(N)                                                                  \
-  case N:                                                                                 \
-    if (C.minW == 1)                                                                      \
-      hipLaunchKernelGGL((k_stats_box<N, true>), grid, dim3(NT), 0, s, d_beams, C, bpw);   \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_stats_box<N, false>), grid, dim3(NT), 0, s, d_beams, C, bpw);  \
+#define HAMD_BOX_CASE(N)                                                                        \
+  case N:                                                                                       \
+    if (C.minW == 1 && narrow)                                                                  \
+      hipLaunchKernelGGL((k_stats_box<N, true, true>), grid, dim3(NT), 0, s, d_beams, C, bpw);   \
+    else if (C.minW == 1)                                                                       \
+      hipLaunchKernelGGL((k_stats_box<N, true, false>), grid, dim3(NT), 0, s, d_beams, C, bpw);  \
+    else if (narrow)                                                                            \
+      hipLaunchKernelGGL((k_stats_box<N, false, true>), grid, dim3(NT), 0, s, d_beams, C, bpw);  \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_stats_box<N, false, false>), grid, dim3(NT), 0, s, d_beams, C, bpw); \
     break;
     HAMD_BOX_CASE(1) HAMD_BOX_CASE(2) HAMD_BOX_CASE(3) HAMD_BOX_CASE(4)
     HAMD_BOX_CASE(5) HAMD_BOX_CASE(6) HAMD_BOX_CASE(7) HAMD_BOX_CASE(8)

@@ -64,6 +64,7 @@ struct SearchCommon {
   int cap;       // candidate buffer capacity per beam
   int nblk;      // ceil(nout / SEARCH_BLOCK)
   int wp;        // statistics columns
+  int64_t box_max;  // bound on every BOXCAR box sum: WIDTH_MAX x 255 x channels
 };
 
 int search_nwidth(int mode, int minW, int maxW);
