@@ -34,22 +34,27 @@ constexpr int MAXBOX = 1024;
 
 enum Pass { P_STATS = 0, P_Y0 = 1, P_COUNT = 2, P_WRITE = 3 };
 
-// exclusive scan of one u32 per thread over the workgroup
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v);
+
+// exclusive scan of one u32 per thread over the 256-thread workgroup: wave
+// scans, then the four wave totals through LDS (u32 sums: any order gives the
+// same words)
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total)
 {
-  const int l = threadIdx.x;
-  sh[l] = v;
+  const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) sh[wv] = incl;
   __syncthreads();
-  for (int off = 1; off < NT; off <<= 1) {
-    const uint32_t t = (l >= off) ? sh[l - off] : 0u;
-    __syncthreads();
-    sh[l] += t;
-    __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const uint32_t c = sh[w];
+    tot += c;
+    before += w < wv ? c : 0u;
   }
-  const uint32_t incl = sh[l];
-  *total = sh[NT - 1];
+  *total = tot;
   __syncthreads();
-  return incl - v;
+  return before + incl - v;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
@@ -519,32 +524,53 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
     return;
   }
 
-  // P_WRITE: ordered by (width, row, sample)
+  // P_WRITE: ordered by (width, row, sample).  Per width, every wave keeps
+  // the ballot of each 256-sample step and publishes its hit count; after
+  // one barrier a hit's rank is the hits of the earlier steps, of the
+  // earlier waves of its step, and of the lower lanes of its wave.
+  constexpr int KS = SB / NT;
+  const int lane = l & 63, wv = l >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
   for (int k = 0; k < C.nw; ++k) {
     if (!((can >> k) & 1u)) continue;
-    uint32_t pos = B.offs[slot(k)];
-    for (int kk = 0; kk < SB / NT; ++kk) {
+    uint64_t m[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) {
       const int j = l + NT * kk;
-      const bool in = base + j < C.nout;
-      float y = 0.f;
       bool hit = false;
-      if (in) {
+      if (base + j < C.nout) {
         const double v = raw(k, j);
         hit = HELLA ? normed(v, sc, k) > thr : (uint64_t)v >= sc.rthr[k];
-        if (hit) y = normed(v, sc, k);
       }
-      uint32_t tot;
-      const uint32_t rank = block_excl_scan(hit ? 1u : 0u, shu, &tot);
-      if (hit && pos + rank < (uint32_t)C.cap) {
-        Cand c;
-        c.snr = y;
-        c.samp = base + j;
-        c.width = k;
-        c.dm_idx = r + B.row0;
-        B.cands[pos + rank] = c;
+      m[kk] = __ballot(hit);
+      if (lane == 0) shu[kk * (NT / 64) + wv] = (uint32_t)__popcll(m[kk]);
+    }
+    __syncthreads();
+    uint32_t pos = B.offs[slot(k)];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) {
+      uint32_t tot = 0, before = 0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) {
+        const uint32_t cw = shu[kk * (NT / 64) + w];
+        tot += cw;
+        before += w < wv ? cw : 0u;
+      }
+      if ((m[kk] >> lane) & 1ull) {
+        const uint32_t rank = before + (uint32_t)__popcll(m[kk] & below);
+        if (pos + rank < (uint32_t)C.cap) {
+          const int j = l + NT * kk;
+          Cand c;
+          c.snr = normed(raw(k, j), sc, k);
+          c.samp = base + j;
+          c.width = k;
+          c.dm_idx = r + B.row0;
+          B.cands[pos + rank] = c;
+        }
       }
       pos += tot;
     }
+    __syncthreads();  // shu is rewritten by the next width
   }
 }
 
