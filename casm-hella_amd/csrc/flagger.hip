@@ -705,14 +705,23 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
   const dim3 gbeam((unsigned)nbeam);
   int cur = -1;  // table slot holding the current values (-1: the input bytes)
   const dim3 gplane4((unsigned)((nblk + 3) / 4), (unsigned)max_nch, (unsigned)nbeam);
+  const dim3 gplane2((unsigned)((nblk + 1) / 2), (unsigned)max_nch, (unsigned)nbeam);
   auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
     const bool same = op == FOP_NONE || op == FOP_NORM_QUANT || op == FOP_TSDIV;
     const int out = same ? cur : (cur == 0 ? 1 : 0);
+    // NB blocks per workgroup: 4, or all of a short gulp's blocks (each
+    // block's partial sums are its own, whatever the grouping)
     if (op == FOP_TSDIV || op == FOP_NORM_QUANT)
       hipLaunchKernelGGL((k_apply_lut<1, true>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
                          npart, a0, a1, cur, out);
-    else
+    else if (nblk >= 4)
       hipLaunchKernelGGL((k_apply_lut<4, false>), gplane4, dim3(NT), 0, s, d_beams, W, op, part,
+                         npart, a0, a1, cur, out);
+    else if (nblk >= 2)
+      hipLaunchKernelGGL((k_apply_lut<2, false>), gplane2, dim3(NT), 0, s, d_beams, W, op, part,
+                         npart, a0, a1, cur, out);
+    else
+      hipLaunchKernelGGL((k_apply_lut<1, false>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
                          npart, a0, a1, cur, out);
     cur = out;
   };
