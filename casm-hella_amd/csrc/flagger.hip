@@ -34,7 +34,7 @@ __device__ __forceinline__ uint8_t quant(float x)
 // ---- ingest: input gulp -> X (transpose_input, src/hella/src/transpose.cu:23-42)
 __global__ __launch_bounds__(256) void k_ingest(const FlagBeamDev *__restrict__ beams, int W)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c0 = blockIdx.y * 64;
   const int t0 = blockIdx.x * 64;
   if (c0 >= B.nch) return;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_ingest(const FlagBeamDev *__restrict__ 
 __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ beams, int W,
                                               int op, int part, int npart, float a0, float a1)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int blk = blockIdx.x;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
 // bandpass from partials: bp[c] = (float)(sum_b part[c][b] / W)
 __global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict__ beams, int W)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.x * NT + threadIdx.x;
   if (c >= B.nch) return;
   double s = 0.0;
@@ -205,7 +205,7 @@ __device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool va
 
 __global__ __launch_bounds__(FS_NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams, int n)
 {
-  const FlagBeamDev &B = beams[blockIdx.x];
+  const FlagBeamDev B = beams[blockIdx.x];
   __shared__ double rowv[FLAG_BLOCK];
   __shared__ double sh[136];
   const double mu = flag_rows_csum(B, n, 0.0, false, rowv, sh);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(NT) void k_box_thresh(const FlagBeamDev *__restrict
                                                    int ts, int fs, float k, float thr,
                                                    float nthr)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int ax = ts / 2, ay = fs / 2;
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_box_thresh(const FlagBeamDev *__restrict
 __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restrict__ beams, int W,
                                                      int ts, int fs, int npart)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int ax = ts / 2, ay = fs / 2;
@@ -296,7 +296,7 @@ constexpr int TS_PARTS = 16;
 __global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__restrict__ beams,
                                                            int W)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   __shared__ double sp[TS_PARTS][64];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int t = blockIdx.x * 64 + lane;
@@ -356,7 +356,7 @@ typedef uint32_t u32x4f __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict__ beams, int W)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c0 = blockIdx.y * 64;
   const int t0 = blockIdx.x * 64;
   if (c0 >= B.nch) return;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
                                                   int op, int part, int npart, float a0, float a1,
                                                   int lin, int lout)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int blk0 = blockIdx.x * NB;
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
 __global__ __launch_bounds__(NT) void k_table(const FlagBeamDev *__restrict__ beams, int op,
                                               float a0, float a1, int lin, int lout)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int l = threadIdx.x;
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ 
                                                 int lin)
 {
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int q = blockIdx.y;
   const int l = threadIdx.x;
   const int t0 = blockIdx.x * TS_BLOCK + TS_SPT * l;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ 
 
 __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict__ beams, int W)
 {
-  const FlagBeamDev &B = beams[blockIdx.z];
+  const FlagBeamDev B = beams[blockIdx.z];
   const int t = blockIdx.x * NT + threadIdx.x;
   if (t >= W) return;
   double p[TS_PARTS];

@@ -150,7 +150,7 @@ template <bool HELLA, int NW>
 __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ beams,
                                               SearchCommon C, int pass, int bpw)
 {
-  const SearchBeamDev &B = beams[blockIdx.z];
+  const SearchBeamDev B = beams[blockIdx.z];
   const int r = blockIdx.y;
   if (r >= B.rows) return;
   const int b0 = blockIdx.x * bpw, b1 = min(C.nblk, b0 + bpw);
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
 __global__ __launch_bounds__(NT) void k_y0_direct(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C, int bpw)
 {
-  const SearchBeamDev &B = beams[blockIdx.z];
+  const SearchBeamDev B = beams[blockIdx.z];
   const int r = blockIdx.y;
   if (r >= B.rows) return;
   const int b0 = blockIdx.x * bpw, b1 = min(C.nblk, b0 + bpw);
@@ -317,7 +317,7 @@ template <int NW, bool W1, bool NARROW>
 __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C, int bpw)
 {
-  const SearchBeamDev &B = beams[blockIdx.z];
+  const SearchBeamDev B = beams[blockIdx.z];
   const int r = blockIdx.y;
   if (r >= B.rows) return;
   const int g = blockIdx.x;
@@ -440,7 +440,7 @@ template <bool HELLA>
 __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ beams,
                                              SearchCommon C, int pass)
 {
-  const SearchBeamDev &B = beams[blockIdx.z];
+  const SearchBeamDev B = beams[blockIdx.z];
   const int r = blockIdx.y;
   if (r >= B.rows) return;
   const int b = blockIdx.x;
@@ -456,16 +456,23 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   const float thr = sc.thr;
   auto slot = [&](int k) { return ((int64_t)k * B.rows + r) * C.nblk + b; };
 
-  // widths of this block that can have crossings
+  // widths of this block that can have crossings (every width's words are
+  // requested before the first is used)
   uint32_t can = 0;
   if (pass == P_COUNT) {
     if (!sc.gated) {
-      for (int k = 0; k < C.nw; ++k) {
-        const uint32_t *bm = B.bmax + slot(k) * 4;
-        const uint32_t m = max(max(bm[0], bm[1]), max(bm[2], bm[3]));
-        const bool c = HELLA ? normed((double)__uint_as_float(m), sc, k) > thr
-                             : (uint64_t)m >= sc.rthr[k];
-        can |= c ? (1u << k) : 0u;
+      uint4 bm[SEARCH_MAXW];
+#pragma unroll
+      for (int k = 0; k < SEARCH_MAXW; ++k)
+        if (k < C.nw) bm[k] = *reinterpret_cast<const uint4 *>(B.bmax + slot(k) * 4);
+#pragma unroll
+      for (int k = 0; k < SEARCH_MAXW; ++k) {
+        if (k < C.nw) {
+          const uint32_t m = max(max(bm[k].x, bm[k].y), max(bm[k].z, bm[k].w));
+          const bool c = HELLA ? normed((double)__uint_as_float(m), sc, k) > thr
+                               : (uint64_t)m >= sc.rthr[k];
+          can |= c ? (1u << k) : 0u;
+        }
       }
     }
     if (!can) {
@@ -473,14 +480,28 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
       return;
     }
   } else {
-    for (int k = 0; k < C.nw; ++k) can |= B.counts[slot(k)] ? (1u << k) : 0u;
+    uint32_t cn[SEARCH_MAXW];
+#pragma unroll
+    for (int k = 0; k < SEARCH_MAXW; ++k)
+      if (k < C.nw) cn[k] = B.counts[slot(k)];
+#pragma unroll
+    for (int k = 0; k < SEARCH_MAXW; ++k)
+      if (k < C.nw) can |= cn[k] ? (1u << k) : 0u;
     if (!can) return;
   }
   if (!HELLA) {
+    // the block and its halo: every load issued before the first LDS store
     const int L = SB + C.maxW;
-    for (int i = l; i < L; i += NT) {
-      const int col = base + i;
-      P[i + 1] = col < C.ntime_dd ? drow[col] : 0u;
+    uint32_t v[PF];
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      const int i = l + NT * t, col = base + i;
+      v[t] = (i < L && col < C.ntime_dd) ? drow[col] : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < PF; ++t) {
+      const int i = l + NT * t;
+      if (i < L) P[i + 1] = v[t];
     }
     if (l == 0) P[0] = 0u;
     __syncthreads();
@@ -648,7 +669,7 @@ __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k
 __global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restrict__ beams,
                                                    SearchCommon C, int which, int ngroups)
 {
-  const SearchBeamDev &B = beams[blockIdx.y];
+  const SearchBeamDev B = beams[blockIdx.y];
   const int k = blockIdx.x;
   __shared__ double rowv[SB];
   __shared__ double sh[260];  // 128 tree slots + the broadcast slot 256
@@ -693,7 +714,7 @@ __device__ __forceinline__ int64_t scan_len(const SearchBeamDev &B, const Search
 __global__ __launch_bounds__(NT) void k_scan_tiles(const SearchBeamDev *__restrict__ beams,
                                                    SearchCommon C)
 {
-  const SearchBeamDev &B = beams[blockIdx.y];
+  const SearchBeamDev B = beams[blockIdx.y];
   const int64_t n = scan_len(B, C);
   const int64_t t0 = (int64_t)blockIdx.x * SB;
   if (t0 >= n) return;
@@ -709,7 +730,7 @@ __global__ __launch_bounds__(NT) void k_scan_tiles(const SearchBeamDev *__restri
 __global__ __launch_bounds__(NT) void k_scan_top(const SearchBeamDev *__restrict__ beams,
                                                  SearchCommon C)
 {
-  const SearchBeamDev &B = beams[blockIdx.x];
+  const SearchBeamDev B = beams[blockIdx.x];
   const int64_t n = scan_len(B, C);
   const int nt = (int)((n + SB - 1) / SB);
   __shared__ uint32_t sh[NT];
@@ -730,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_scan_top(const SearchBeamDev *__restrict
 __global__ __launch_bounds__(NT) void k_scan_down(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C)
 {
-  const SearchBeamDev &B = beams[blockIdx.y];
+  const SearchBeamDev B = beams[blockIdx.y];
   const int64_t n = scan_len(B, C);
   const int64_t t0 = (int64_t)blockIdx.x * SB;
   if (t0 >= n) return;
@@ -759,7 +780,7 @@ __global__ __launch_bounds__(NT) void k_hella_fir(const SearchBeamDev *__restric
                                                   SearchCommon C, int k, int sm,
                                                   const float *__restrict__ kern)
 {
-  const SearchBeamDev &B = beams[blockIdx.z];
+  const SearchBeamDev B = beams[blockIdx.z];
   const int r = blockIdx.y;
   const int s = blockIdx.x * NT + threadIdx.x;
   if (r >= B.rows || s >= C.nout) return;

@@ -245,14 +245,17 @@ def test_wave_per_trial_loader(hamd, orc, chunk, order, kill, W):
                                                        i["ntime_dd"], killmask=km))
 
 
-@pytest.mark.parametrize("lanes,chunk,order", [(16, 8, 0), (32, 8, 1), (64, 16, 1), (8, 32, 0)])
-@pytest.mark.parametrize("loader", [0, 1, 2])
+_SHAPES = [(16, 8, 0), (32, 8, 1), (64, 16, 1), (8, 32, 0)]
+
+
+# the register-prefetch loader (2) takes chunks of 8 or 16 channels only
+@pytest.mark.parametrize("lanes,chunk,order,loader",
+                         [s + (ld,) for ld in (0, 1, 2) for s in _SHAPES
+                          if not (ld == 2 and s[1] > 16)])
 def test_tile_shapes_and_loaders(hamd, orc, lanes, chunk, order, loader):
     """Every dedispersion tile shape / window loader gives the same exact sums
     (loader 0: LDS DMA double buffer, 1: through registers, 2: register
-    prefetch; 2 needs chunk <= 16)."""
-    if loader == 2 and chunk > 16:
-        pytest.skip("the register-prefetch loader takes chunks of 8 or 16")
+    prefetch)."""
     W = 4096
     cfg = _cfg(INPUT="FILTERBANK", GULP=W, DM_MAX=150, WIDTH_MAX=8)
     h = hamd.Hella(cfg)
