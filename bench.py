@@ -280,7 +280,7 @@ def main():
         print(json.dumps({"uneven": uneven(hamd, args.uneven)}), flush=True)
         return
     if args.stress:
-        print(json.dumps({"stress": stress(hamd, args.stress)}), flush=True)
+        print(json.dumps({"stress": stress(hamd, args.stress, args.tune)}), flush=True)
         return
     if args.serve_only:
         print(json.dumps({"serving": serving(hamd, args, pulses, rfi)}), flush=True)
@@ -420,7 +420,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stress(hamd, gulps):
+def stress(hamd, gulps, tune=False):
     """The 4096-channel FP32 stress configuration, one beam on the plain path:
     500->400 MHz, 65.536 us, 262144-sample chunks (17.18 s), DM 0-5000, boxcar
     widths 1..1024, N(0,1) input with 20% of the channels at 30 sigma and
@@ -450,6 +450,10 @@ def stress(hamd, gulps):
     need = info["min_gulp"] + 1
     for g in range(need):
         h.process_gulp(bufs[g % 2])
+    if tune:
+        h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_stress.json"))
+        h.process_gulp(bufs[need % 2])
+        need += 1
     hamd.device_sync()
     times, crossings, ncand = [], [], []
     for g in range(gulps):

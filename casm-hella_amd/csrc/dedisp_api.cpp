@@ -39,10 +39,11 @@ namespace hamd {
 DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
                       int lanes_pref, size_t lds_budget)
 {
-  // measured on MI355X: one DM trial per wave in workgroups of 4 trials is
-  // fastest, then 32 lanes per trial with 8-channel chunks staged through
-  // registers; other lane counts when the spread does not fit
-  const DDTile w{64, 4, 3};
+  // measured on MI355X: one DM trial per wave in workgroups of 4 trials,
+  // every LDS read of a chunk issued before its sums, is fastest, then 32
+  // lanes per trial with 8-channel chunks staged through registers; other
+  // lane counts when the spread does not fit
+  const DDTile w{64, 4, 4};
   if (lanes_pref <= 64) {
     const int spread = dd_max_spread(delta, ndm, nlive, dd_tile_ND(w));
     if (dd_tile_fits(w, spread, INT64_MAX)) {

@@ -267,15 +267,16 @@ void WorkSet::ensure(const WorkDims &d)
 
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
 {
-  // measured on MI355X (1024 ch, DM 0-2000, 28 samples per thread): one DM
-  // trial per wave in workgroups of 4 trials, time-major tile order, 4.92 ms;
-  // then 32 lanes per trial with 8-channel chunks staged through registers
-  // (5.10 ms); wider lane groups take over when the delay spread of 8 DM
-  // trials does not fit
+  // measured on MI355X: one DM trial per wave in workgroups of 4 trials,
+  // time-major tile order, every LDS read of a chunk issued before its sums
+  // (1024 ch, DM 0-2000: 4.07 ms, the same as reading channel by channel;
+  // 4096 ch, DM 0-5000: 284 ms against 328 ms); then 32 lanes per trial with
+  // 8-channel chunks staged through registers; wider lane groups take over
+  // when the delay spread of 8 DM trials does not fit
   int64_t min_ring = INT64_MAX;
   for (const BeamState *B : beams) min_ring = std::min(min_ring, B->ntime);
   order = 1;
-  for (const DDTile t : {DDTile{64, 4, 3}, DDTile{32, 8, 0}, DDTile{64, 8, 0}}) {
+  for (const DDTile t : {DDTile{64, 4, 4}, DDTile{32, 8, 0}, DDTile{64, 8, 0}}) {
     bool fits = true;
     for (const BeamState *B : beams) {
       if (B->nlive == 0) continue;
