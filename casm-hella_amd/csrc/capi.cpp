@@ -200,10 +200,7 @@ hella_status hella_smooth(hella_ctx c, int scale)
 hella_status hella_find_peaks(hella_ctx c, int beam)
 {
   if (!valid_beam(c, beam) || beam != c->cur_beam) return fail(HELLA_ERR_STATE, "beam not smoothed");
-  return guard([&] {
-    ctx_find_peaks(c, beam);
-    ctx_cluster(c);
-  });
+  return guard([&] { ctx_find_peaks(c, beam); });
 }
 
 hella_status hella_clear_peaks(hella_ctx c)

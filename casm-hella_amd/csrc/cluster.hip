@@ -1,0 +1,220 @@
+// Candidate clustering on the device: the crossings of a beam's gulp fall into
+// cells of CLUSTER_DT samples x CLUSTER_DDM DM trials; 8-connected occupied
+// cells form one cluster, represented by its highest-S/N member (lowest list
+// index on ties); clusters are listed by representative index.  One
+// workgroup per beam:
+//   1. every candidate marks its cell with the lowest candidate index in it
+//      (atomic minimum into the beam's cell grid);
+//   2. the first candidate of each cell joins the cells around it (union-find
+//      over candidate indices, the larger root hooked under the smaller by
+//      compare-and-swap, so a component's root is its lowest cell index);
+//   3. every candidate folds itself into its root: the (S/N, -index) key by
+//      atomic maximum, the count, and the sample and DM bounds;
+//   4. the candidates that represent their cluster are compacted in index
+//      order (wave ballots, workgroup scan).
+// The candidates taken are those the host keeps: the first min(crossings,
+// CAND_CAP, MAX_GIANTS) of the beam, and within a gulp's list no more than
+// MAX_GIANTS in all (the beams before this one counted in order).
+// Reads of words other waves change by atomics use device-scope atomic loads,
+// which bypass the CU's L1.
+#include <hip/hip_runtime.h>
+
+#include "search.hpp"
+
+namespace hamd {
+namespace {
+
+constexpr int CT = 256;  // threads per workgroup
+constexpr uint32_t NONE = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t ld(const uint32_t *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long ld64(const unsigned long long *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root of x: parents only ever point to lower indices
+__device__ uint32_t find(uint32_t *par, uint32_t x)
+{
+  uint32_t p = ld(par + x);
+  while (p != x) {
+    x = p;
+    p = ld(par + x);
+  }
+  return x;
+}
+
+__device__ void unite(uint32_t *par, uint32_t a, uint32_t b)
+{
+  while (true) {
+    a = find(par, a);
+    b = find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    // hook the larger root under the smaller; retry if a was hooked meanwhile
+    if (atomicCAS(par + a, a, b) == a) return;
+  }
+}
+
+// float -> u32 with the same order (every S/N here is finite)
+__device__ __forceinline__ uint32_t ordered(float f)
+{
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ uint32_t scan_excl(uint32_t v, uint32_t *sh, uint32_t *total)
+{
+  const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) sh[wv] = incl;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < CT / 64; ++w) {
+    const uint32_t c = sh[w];
+    tot += c;
+    before += w < wv ? c : 0u;
+  }
+  *total = tot;
+  __syncthreads();
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict__ beams,
+                                                ClusterArgs A)
+{
+  const int b = blockIdx.x;
+  const SearchBeamDev B = beams[b];
+  const int l = threadIdx.x;
+  __shared__ uint32_t sh[CT / 64];
+  // candidates taken for this beam
+  auto taken = [&](const SearchBeamDev &X) {
+    return min(min(*X.ncand, (uint32_t)A.cand_cap), (uint32_t)A.max_giants);
+  };
+  uint32_t before = (uint32_t)A.prefix;
+  if (A.cumulative)
+    for (int q = 0; q < b; ++q) {
+      const uint32_t t = taken(beams[q]);
+      const uint32_t left = before < (uint32_t)A.max_giants ? (uint32_t)A.max_giants - before : 0u;
+      before += min(t, left);
+    }
+  const uint32_t left = before < (uint32_t)A.max_giants ? (uint32_t)A.max_giants - before : 0u;
+  const int n = (int)min(taken(B), left);
+  if (l == 0) B.cl_n[1] = (uint32_t)n;
+  if (n == 0) {
+    if (l == 0) B.cl_n[0] = 0u;
+    return;
+  }
+  uint32_t *par = B.cl_work;
+  uint32_t *cnt = par + B.cl_cap;
+  uint32_t *smin = cnt + B.cl_cap;
+  uint32_t *smax = smin + B.cl_cap;
+  uint32_t *dmin = smax + B.cl_cap;
+  uint32_t *dmax = dmin + B.cl_cap;
+  unsigned long long *key = B.cl_key;
+  auto cell = [&](const Cand &c) {
+    return (c.dm_idx / A.cd) * B.cl_cols + c.samp / A.ct;
+  };
+
+  // 1. cells
+  for (int i = l; i < n; i += CT) atomicMin(B.cl_grid + cell(B.cands[i]), (uint32_t)i);
+  __threadfence();
+  __syncthreads();
+  // 2. cell heads: own parent, then joined with occupied neighbours
+  for (int i = l; i < n; i += CT) {
+    const Cand c = B.cands[i];
+    if (ld(B.cl_grid + cell(c)) != (uint32_t)i) continue;
+    __hip_atomic_store(par + i, (uint32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(key + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(cnt + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(smin + i, NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(smax + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(dmin + i, NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(dmax + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __threadfence();
+  __syncthreads();
+  for (int i = l; i < n; i += CT) {
+    const Cand c = B.cands[i];
+    if (ld(B.cl_grid + cell(c)) != (uint32_t)i) continue;
+    const int r = c.dm_idx / A.cd, t = c.samp / A.ct;
+    for (int dr = -1; dr <= 1; ++dr)
+      for (int dt = -1; dt <= 1; ++dt) {
+        if (!dr && !dt) continue;
+        const int rr = r + dr, tt = t + dt;
+        if (rr < 0 || rr >= B.cl_rows || tt < 0 || tt >= B.cl_cols) continue;
+        const uint32_t g = ld(B.cl_grid + rr * B.cl_cols + tt);
+        if (g != NONE) unite(par, (uint32_t)i, g);
+      }
+  }
+  __threadfence();
+  __syncthreads();
+  // 3. every candidate into its root
+  for (int i = l; i < n; i += CT) {
+    const Cand c = B.cands[i];
+    const uint32_t root = find(par, ld(B.cl_grid + cell(c)));
+    atomicMax(key + root, ((unsigned long long)ordered(c.snr) << 32) | (unsigned long long)(NONE - (uint32_t)i));
+    atomicAdd(cnt + root, 1u);
+    atomicMin(smin + root, (uint32_t)c.samp);
+    atomicMax(smax + root, (uint32_t)c.samp);
+    atomicMin(dmin + root, (uint32_t)c.dm_idx);
+    atomicMax(dmax + root, (uint32_t)c.dm_idx);
+  }
+  __threadfence();
+  __syncthreads();
+  // 4. representatives in index order
+  uint32_t run = 0;
+  for (int base = 0; base < n; base += CT) {
+    const int i = base + l;
+    uint32_t root = 0;
+    bool rep = false;
+    if (i < n) {
+      root = find(par, ld(B.cl_grid + cell(B.cands[i])));
+      rep = NONE - (uint32_t)(ld64(key + root) & 0xffffffffull) == (uint32_t)i;
+    }
+    uint32_t tot;
+    const uint32_t slot = run + scan_excl(rep ? 1u : 0u, sh, &tot);
+    if (rep) {
+      ClusterRec o;
+      o.rep = i;
+      o.count = (int)ld(cnt + root);
+      o.samp_min = (int)ld(smin + root);
+      o.samp_max = (int)ld(smax + root);
+      o.dm_min = (int)ld(dmin + root);
+      o.dm_max = (int)ld(dmax + root);
+      o.beam = 0;
+      B.clusters[slot] = o;
+    }
+    run += tot;
+  }
+  if (l == 0) B.cl_n[0] = run;
+  // the grid is left empty for the next gulp
+  __syncthreads();
+  for (int i = l; i < n; i += CT) B.cl_grid[cell(B.cands[i])] = NONE;
+}
+
+}  // namespace
+
+void SearchRunner::cluster(const SearchBeamDev *d_beams, int nbeam, const ClusterArgs &A,
+                           hipStream_t s)
+{
+  if (nbeam <= 0) return;
+  hipLaunchKernelGGL(k_cluster, dim3((unsigned)nbeam), dim3(CT), 0, s, d_beams, A);
+  HAMD_CHECK(hipGetLastError());
+}
+
+}  // namespace hamd

@@ -194,6 +194,9 @@ WorkDims beam_dims(const BeamState &B, const Config &cfg, int W, int nw)
   d.mask = flag_needs_mask(cfg.flag);
   d.tables = flag_value_tables(cfg.input_fmt, cfg.flag);
   d.planes = cfg.search_mode == SEARCH_HELLA;
+  d.cl_rows = (B.ndm + std::max(1, cfg.cluster_ddm) - 1) / std::max(1, cfg.cluster_ddm);
+  d.cl_cols = (B.nout + std::max(1, cfg.cluster_dt) - 1) / std::max(1, cfg.cluster_dt);
+  d.cl_cap = std::max(1, std::min(cfg.cand_cap, cfg.max_giants));
   return d;
 }
 
@@ -211,10 +214,14 @@ void WorkDims::merge(const WorkDims &o)
   mask = mask || o.mask;
   planes = planes || o.planes;
   tables = tables || o.tables;
+  cl_rows = std::max(cl_rows, o.cl_rows);
+  cl_cols = std::max(cl_cols, o.cl_cols);
+  cl_cap = std::max(cl_cap, o.cl_cap);
 }
 
 struct WorkSizes {
   size_t X, mask, D, planes, fpart, fts, spart, scnt, stiles, scands;
+  size_t cgrid, cwork, ckey, cout;
 };
 
 static WorkSizes work_sizes(const WorkDims &d)
@@ -235,6 +242,10 @@ static WorkSizes work_sizes(const WorkDims &d)
   z.scnt = ncnt * 4;
   z.stiles = ((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4;
   z.scands = (size_t)d.cand_cap * sizeof(Cand);
+  z.cgrid = (size_t)d.cl_rows * d.cl_cols * 4;
+  z.cwork = (size_t)6 * d.cl_cap * 4;
+  z.ckey = (size_t)d.cl_cap * 8;
+  z.cout = (size_t)d.cl_cap * sizeof(ClusterRec);
   return z;
 }
 
@@ -242,7 +253,7 @@ size_t WorkSet::bytes_for(const WorkDims &d)
 {
   const WorkSizes z = work_sizes(d);
   return z.X + z.mask + z.D + z.planes + 2 * z.fpart + z.fts + 3 * z.spart + 6 * z.scnt +
-         z.stiles + z.scands + 16;
+         z.stiles + z.scands + 16 + z.cgrid + z.cwork + z.ckey + z.cout;
 }
 
 void WorkSet::ensure(const WorkDims &d)
@@ -262,7 +273,15 @@ void WorkSet::ensure(const WorkDims &d)
   s_bmax.ensure(4 * z.scnt);    // one maximum per wave of each block
   s_tiles.ensure(z.stiles);
   s_cands.ensure(z.scands);
-  s_ncand.ensure(16);
+  s_ncand.ensure(16);  // crossings, clusters, candidates taken
+  // the clustering grid starts empty (every cell 0xffffffff) and each launch
+  // empties the cells it used
+  const void *g0 = c_grid.p;
+  c_grid.ensure(std::max<size_t>(z.cgrid, 4));
+  if (c_grid.p != g0) HAMD_CHECK(hipMemset(c_grid.p, 0xff, c_grid.bytes));
+  c_work.ensure(z.cwork);
+  c_key.ensure(z.ckey);
+  c_out.ensure(z.cout);
 }
 
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order)
@@ -394,7 +413,7 @@ void ctx_build(hella_ctx c)
   HAMD_CHECK(hipHostMalloc((void **)&c->h_fdesc, nb * sizeof(FlagBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
-  HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, nb * sizeof(uint32_t), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
   c->d_fdesc.ensure(nb * sizeof(FlagBeamDev));
   c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
@@ -507,7 +526,45 @@ SearchBeamDev make_sdesc(const Config &cfg, BeamState &B, WorkSet &w)
   s.tiles = w.s_tiles.as<uint32_t>();
   s.cands = w.s_cands.as<Cand>();
   s.ncand = w.s_ncand.as<uint32_t>();
+  const WorkDims d = beam_dims(B, cfg, (int)cfg.gulp, 1);
+  s.cl_grid = w.c_grid.as<uint32_t>();
+  s.cl_work = w.c_work.as<uint32_t>();
+  s.cl_key = w.c_key.as<unsigned long long>();
+  s.clusters = w.c_out.as<ClusterRec>();
+  s.cl_n = s.ncand + 1;
+  s.cl_rows = d.cl_rows;
+  s.cl_cols = d.cl_cols;
+  s.cl_cap = d.cl_cap;
   return s;
+}
+
+ClusterArgs cluster_args(const Config &cfg, int prefix, bool cumulative)
+{
+  ClusterArgs a{};
+  a.cand_cap = cfg.cand_cap;
+  a.max_giants = cfg.max_giants;
+  a.prefix = prefix;
+  a.cumulative = cumulative ? 1 : 0;
+  a.ct = std::max(1, cfg.cluster_dt);
+  a.cd = std::max(1, cfg.cluster_ddm);
+  return a;
+}
+
+void append_clusters(const std::vector<ClusterRec> &cl, int rep0, int beam_id,
+                     std::vector<hella_cluster> &out)
+{
+  static_assert(sizeof(ClusterRec) == sizeof(hella_cluster), "cluster record layout");
+  for (const ClusterRec &x : cl) {
+    hella_cluster h;
+    h.rep = rep0 + x.rep;
+    h.count = x.count;
+    h.samp_min = x.samp_min;
+    h.samp_max = x.samp_max;
+    h.dm_min = x.dm_min;
+    h.dm_max = x.dm_max;
+    h.beam = beam_id;
+    out.push_back(h);
+  }
 }
 
 // ------------------------------------------------------------------ timing
@@ -616,22 +673,30 @@ int ctx_find_peaks(hella_ctx c, int b)
   WorkSet &w = c->ws[0];
   uint32_t nc = 0;
   std::vector<Cand> cs;
+  std::vector<ClusterRec> cl;
+  const int rep0 = (int)c->peaks.size();
   {
     StageTimer t(c, 3);
     SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
     SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
-    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[b], w.s_ncand.p, 4, hipMemcpyDeviceToHost, c->stream));
+    SearchRunner::cluster(sd, 1, cluster_args(c->cfg, rep0, false), c->stream);
+    uint32_t *hn = c->h_ncand + 4 * (size_t)b;
+    HAMD_CHECK(hipMemcpyAsync(hn, w.s_ncand.p, 12, hipMemcpyDeviceToHost, c->stream));
     HAMD_CHECK(hipStreamSynchronize(c->stream));
-    nc = c->h_ncand[b];
+    nc = hn[0];
     cs.resize(beam_take(c->cfg, nc));
-    if (!cs.empty()) {
+    cl.resize(hn[1]);
+    if (!cs.empty())
       HAMD_CHECK(hipMemcpyAsync(cs.data(), w.s_cands.p, cs.size() * sizeof(Cand),
                                 hipMemcpyDeviceToHost, c->stream));
-      HAMD_CHECK(hipStreamSynchronize(c->stream));
-    }
+    if (!cl.empty())
+      HAMD_CHECK(hipMemcpyAsync(cl.data(), w.c_out.p, cl.size() * sizeof(ClusterRec),
+                                hipMemcpyDeviceToHost, c->stream));
+    if (!cs.empty() || !cl.empty()) HAMD_CHECK(hipStreamSynchronize(c->stream));
   }
   B.crossings = nc;
   append_candidates(c->cfg, c->W, B, b, cs, (size_t)c->cfg.max_giants, c->peaks);
+  append_clusters(cl, rep0, b, c->clusters);
   return (int)cs.size();
 }
 
@@ -662,7 +727,6 @@ void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr)
     ctx_find_peaks(c, (int)b);
   }
   HAMD_CHECK(hipStreamSynchronize(c->stream));
-  ctx_cluster(c);
   ctx_end_gulp(c);
 }
 
@@ -739,9 +803,10 @@ static void batch_record(hella_ctx c)
   for (int b : bt.beams) max_rows = std::max(max_rows, c->beams[(size_t)b].rows);
   SearchRunner::run(c->d_sdesc.as<SearchBeamDev>(), nr, max_rows, c->common,
                     c->d_kern.as<float>(), c->kofs, s);
+  SearchRunner::cluster(c->d_sdesc.as<SearchBeamDev>(), nr, cluster_args(c->cfg, 0, true), s);
   for (int i = 0; i < nr; ++i) {
     const int b = bt.beams[(size_t)i];
-    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[i], c->ws[(size_t)b].s_ncand.p, 4,
+    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[4 * (size_t)i], c->ws[(size_t)b].s_ncand.p, 12,
                               hipMemcpyDeviceToHost, s));
   }
 }
@@ -796,117 +861,36 @@ void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  // candidates: only beams with crossings are copied back
+  // candidates and clusters: only beams with crossings are copied back
   std::vector<std::vector<Cand>> per(bt.beams.size());
+  std::vector<std::vector<ClusterRec>> pcl(bt.beams.size());
   for (size_t i = 0; i < bt.beams.size(); ++i) {
-    const size_t n = beam_take(c->cfg, c->h_ncand[i]);
-    if (!n) continue;
-    per[i].resize(n);
-    HAMD_CHECK(hipMemcpyAsync(per[i].data(), c->ws[(size_t)bt.beams[i]].s_cands.p,
-                              n * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
+    const uint32_t *hn = c->h_ncand + 4 * i;
+    const WorkSet &w = c->ws[(size_t)bt.beams[i]];
+    const size_t n = beam_take(c->cfg, hn[0]);
+    if (n) {
+      per[i].resize(n);
+      HAMD_CHECK(hipMemcpyAsync(per[i].data(), w.s_cands.p, n * sizeof(Cand),
+                                hipMemcpyDeviceToHost, c->stream));
+    }
+    if (hn[1]) {
+      pcl[i].resize(hn[1]);
+      HAMD_CHECK(hipMemcpyAsync(pcl[i].data(), w.c_out.p, hn[1] * sizeof(ClusterRec),
+                                hipMemcpyDeviceToHost, c->stream));
+    }
   }
   HAMD_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < bt.beams.size(); ++i) {
     if ((int)c->peaks.size() >= c->cfg.max_giants) break;
     BeamState &B = c->beams[(size_t)bt.beams[i]];
-    B.crossings = c->h_ncand[i];
+    B.crossings = c->h_ncand[4 * i];
+    const int rep0 = (int)c->peaks.size();
     append_candidates(c->cfg, c->W, B, bt.beams[i], per[i], (size_t)c->cfg.max_giants, c->peaks);
+    append_clusters(pcl[i], rep0, bt.beams[i], c->clusters);
   }
-  ctx_cluster(c);
   ctx_end_gulp(c);
 }
 
-// ------------------------------------------------------------------ clustering
-// Cells of CLUSTER_DT samples x CLUSTER_DDM DM trials per beam; 8-connected
-// occupied cells form one cluster, represented by its highest-S/N member
-// (lowest list index on ties); clusters ordered by representative index.
-void cluster_candidates(const std::vector<hella_candidate> &peaks, int ct_, int cd_,
-                        std::vector<hella_cluster> &clusters)
-{
-  clusters.clear();
-  const int n = (int)peaks.size();
-  if (!n) return;
-  const int ct = std::max(1, ct_), cd = std::max(1, cd_);
-  struct KI { int64_t beam; int64_t key; int idx; };
-  auto key = [&](int64_t cdm, int64_t cts) { return (cdm << 32) | (int64_t)(uint32_t)cts; };
-  std::vector<KI> v((size_t)n);
-  for (int i = 0; i < n; ++i) {
-    const hella_candidate &h = peaks[(size_t)i];
-    v[(size_t)i] = KI{h.beam, key(h.dm_idx / cd, h.samp / ct), i};
-  }
-  std::sort(v.begin(), v.end(), [](const KI &a, const KI &b) {
-    if (a.beam != b.beam) return a.beam < b.beam;
-    if (a.key != b.key) return a.key < b.key;
-    return a.idx < b.idx;
-  });
-  std::vector<std::pair<int64_t, int64_t>> cells;  // (beam, key)
-  std::vector<int> cell_of((size_t)n);
-  for (int i = 0; i < n; ++i) {
-    if (i == 0 || v[(size_t)i].beam != v[(size_t)i - 1].beam || v[(size_t)i].key != v[(size_t)i - 1].key)
-      cells.emplace_back(v[(size_t)i].beam, v[(size_t)i].key);
-    cell_of[(size_t)v[(size_t)i].idx] = (int)cells.size() - 1;
-  }
-  const int nc = (int)cells.size();
-  std::vector<int> par((size_t)nc);
-  for (int i = 0; i < nc; ++i) par[(size_t)i] = i;
-  std::function<int(int)> find = [&](int x) {
-    while (par[(size_t)x] != x) {
-      par[(size_t)x] = par[(size_t)par[(size_t)x]];
-      x = par[(size_t)x];
-    }
-    return x;
-  };
-  for (int i = 0; i < nc; ++i) {
-    const int64_t bm = cells[(size_t)i].first, k = cells[(size_t)i].second;
-    const int64_t cdm = k >> 32, cts = (int32_t)(uint32_t)k;
-    for (int dd = -1; dd <= 1; ++dd)
-      for (int dt = -1; dt <= 1; ++dt) {
-        if (!dd && !dt) continue;
-        const auto it = std::lower_bound(cells.begin(), cells.end(),
-                                         std::make_pair(bm, key(cdm + dd, cts + dt)));
-        if (it != cells.end() && it->first == bm && it->second == key(cdm + dd, cts + dt)) {
-          int a = find(i), b = find((int)(it - cells.begin()));
-          if (a != b) par[(size_t)std::max(a, b)] = std::min(a, b);
-        }
-      }
-  }
-  std::vector<int> rep((size_t)nc, -1);
-  for (int i = 0; i < n; ++i) {
-    const int r = find(cell_of[(size_t)i]);
-    const int cur = rep[(size_t)r];
-    if (cur < 0 || peaks[(size_t)i].snr > peaks[(size_t)cur].snr) rep[(size_t)r] = i;
-  }
-  std::vector<int> reps;
-  for (int i = 0; i < nc; ++i)
-    if (find(i) == i && rep[(size_t)i] >= 0) reps.push_back(rep[(size_t)i]);
-  std::sort(reps.begin(), reps.end());
-  std::vector<int> slot((size_t)nc, -1);
-  for (size_t q = 0; q < reps.size(); ++q) {
-    slot[(size_t)find(cell_of[(size_t)reps[q]])] = (int)q;
-    hella_cluster h{};
-    h.rep = reps[q];
-    h.samp_min = h.dm_min = 0x7fffffff;
-    h.samp_max = h.dm_max = -1;
-    h.beam = peaks[(size_t)reps[q]].beam;
-    clusters.push_back(h);
-  }
-  for (int i = 0; i < n; ++i) {
-    hella_cluster &h = clusters[(size_t)slot[(size_t)find(cell_of[(size_t)i])]];
-    const hella_candidate &p = peaks[(size_t)i];
-    h.count++;
-    h.samp_min = std::min(h.samp_min, p.samp);
-    h.samp_max = std::max(h.samp_max, p.samp);
-    h.dm_min = std::min(h.dm_min, p.dm_idx);
-    h.dm_max = std::max(h.dm_max, p.dm_idx);
-  }
-}
-
-void ctx_cluster(hella_ctx c)
-{
-  cluster_candidates(c->peaks, c->cfg.cluster_dt, c->cfg.cluster_ddm, c->clusters);
-}
-
-// ------------------------------------------------------------------ stage timing
 float ctx_bench_stage(hella_ctx c, int b, int which, int reps)
 {
   BeamState &B = c->beams[(size_t)b];

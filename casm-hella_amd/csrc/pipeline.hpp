@@ -74,6 +74,7 @@ struct BeamState {
 // Sizes a WorkSet must cover (maximum over the beams that use it).
 struct WorkDims {
   int nch = 0, W = 0, ndm = 0, rows = 0, nout = 0, nw = 0, cand_cap = 0;
+  int cl_rows = 0, cl_cols = 0, cl_cap = 0;  // clustering grid and capacity
   int64_t xs = 0, dstride = 0;
   bool mask = false, planes = false;
   bool tables = false;  // flagger value-table path: uint8 plane + tables instead of FP32
@@ -87,6 +88,7 @@ struct WorkSet {
   DevBuf X, mask, D, planes;
   DevBuf f_part, f_part2, f_ts;
   DevBuf s_part, s_part2, s_counts, s_offs, s_bmax, s_tiles, s_cands, s_ncand;
+  DevBuf c_grid, c_work, c_key, c_out, c_n;
   void ensure(const WorkDims &d);
   static size_t bytes_for(const WorkDims &d);
 };
@@ -155,10 +157,13 @@ SearchCommon search_common(const Config &cfg, const BeamState &B0, int nw);
 // candidate list holding at most cap entries
 void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id,
                        const std::vector<Cand> &cs, size_t cap, std::vector<hella_candidate> &out);
-// clusters of a candidate list (8-connected cells of ct samples x cd trials,
-// per beam), ordered by representative index
-void cluster_candidates(const std::vector<hella_candidate> &peaks, int ct, int cd,
-                        std::vector<hella_cluster> &out);
+// clustering arguments (SearchRunner::cluster): prefix candidates already in
+// the list; cumulative: the launch's beams share one list, in order
+ClusterArgs cluster_args(const Config &cfg, int prefix, bool cumulative);
+// append a beam's clusters (device records; rep0 = the beam's first index in
+// the candidate list)
+void append_clusters(const std::vector<ClusterRec> &cl, int rep0, int beam_id,
+                     std::vector<hella_cluster> &out);
 // default dedispersion tile shape for a set of beams
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order);
 void build_plain_tiles(BeamState &B, const DDTile &t, int order);
@@ -170,7 +175,6 @@ int ctx_find_peaks(hella_ctx c, int b);
 void ctx_end_gulp(hella_ctx c);
 void ctx_process_plain(hella_ctx c, const void *gulp, bool host_ptr);
 void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr);
-void ctx_cluster(hella_ctx c);
 void ctx_autotune(hella_ctx c, const char *path);
 float ctx_bench_stage(hella_ctx c, int b, int which, int reps);
 const void *ctx_stage_input(hella_ctx c, const void *gulp, bool host_ptr);

@@ -52,6 +52,27 @@ struct SearchBeamDev {
   uint32_t *tiles;    // scan scratch [ceil(n/SEARCH_BLOCK)]
   Cand *cands;        // ordered candidates (first cap stored)
   uint32_t *ncand;    // total crossings (uncapped)
+  // clustering (cluster.hip)
+  uint32_t *cl_grid;  // [cl_rows][cl_cols] lowest candidate index of each cell; all
+                      // 0xffffffff between launches
+  uint32_t *cl_work;  // [6][cl_cap] union-find parents, count, sample and DM bounds
+  unsigned long long *cl_key;  // [cl_cap] best (S/N, -index) of each component root
+  struct ClusterRec *clusters;  // [cl_cap] clusters by representative index
+  uint32_t *cl_n;     // [0] clusters, [1] candidates taken
+  int cl_rows, cl_cols, cl_cap;
+};
+
+// one cluster (the layout of hella_cluster; rep is the index in the beam's
+// taken candidates, beam is filled in by the host)
+struct ClusterRec {
+  int rep, count, samp_min, samp_max, dm_min, dm_max, beam;
+};
+
+struct ClusterArgs {
+  int cand_cap, max_giants;  // a beam keeps min(crossings, cand_cap, max_giants)
+  int prefix;                // candidates already in the gulp's list before the first beam
+  int cumulative;            // beams share the list (each counts the ones before it)
+  int ct, cd;                // cell size: samples x DM trials
 };
 
 struct SearchCommon {
@@ -83,6 +104,8 @@ struct SearchRunner {
                      hipStream_t s);
   static void peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows, const SearchCommon &C,
                     hipStream_t s);
+  // clusters of every beam's taken candidates (one workgroup per beam)
+  static void cluster(const SearchBeamDev *d_beams, int nbeam, const ClusterArgs &A, hipStream_t s);
 };
 
 }  // namespace hamd

@@ -238,7 +238,7 @@ void setup_lanes(hella_server s)
     HAMD_CHECK(hipHostMalloc((void **)&L.h_fdesc, nb * sizeof(FlagBeamDev), hipHostMallocDefault));
     HAMD_CHECK(hipHostMalloc((void **)&L.h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
     HAMD_CHECK(hipHostMalloc((void **)&L.h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
-    HAMD_CHECK(hipHostMalloc((void **)&L.h_ncand, nb * sizeof(uint32_t), hipHostMallocDefault));
+    HAMD_CHECK(hipHostMalloc((void **)&L.h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
     L.d_fdesc.ensure(nb * sizeof(FlagBeamDev));
     L.d_ddesc.ensure(nb * sizeof(DDBeamDev));
     L.d_sdesc.ensure(nb * sizeof(SearchBeamDev));
@@ -378,9 +378,10 @@ void record_search(hella_server s, Lane &L, int nr, const DDTileDesc *tiles, int
   }
   SearchRunner::run(L.d_sdesc.as<SearchBeamDev>(), nr, g.rows, s->common, s->d_kern.as<float>(),
                     s->kofs, L.stream);
+  SearchRunner::cluster(L.d_sdesc.as<SearchBeamDev>(), nr, cluster_args(s->cfg, 0, false), L.stream);
   for (int i = 0; i < nr; ++i)
-    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[i], L.ws[(size_t)i].s_ncand.p, 4, hipMemcpyDeviceToHost,
-                              L.stream));
+    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[4 * (size_t)i], L.ws[(size_t)i].s_ncand.p, 12,
+                              hipMemcpyDeviceToHost, L.stream));
 }
 
 template <class Rec> Graph &get_graph(Lane &L, std::map<int, Graph> &m, int key, Rec &&rec)
@@ -479,7 +480,8 @@ void launch_plain(hella_server s, Lane &L)
     SearchRunner::smooth(L.d_sdesc.as<SearchBeamDev>(), 1, B.rows, s->common, s->d_kern.as<float>(),
                          s->kofs, L.stream);
     SearchRunner::peaks(L.d_sdesc.as<SearchBeamDev>(), 1, B.rows, s->common, L.stream);
-    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[0], w.s_ncand.p, 4, hipMemcpyDeviceToHost, L.stream));
+    SearchRunner::cluster(L.d_sdesc.as<SearchBeamDev>(), 1, cluster_args(s->cfg, 0, false), L.stream);
+    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[0], w.s_ncand.p, 12, hipMemcpyDeviceToHost, L.stream));
   }
   HAMD_CHECK(hipEventRecord(L.done, L.stream));
 }
@@ -499,18 +501,22 @@ void harvest(hella_server s, hella_ring ring, Lane &L)
     res.r.batch = (int)nb;
     res.r.t_arrival_ns = j.ck.t_arr;
     if (j.searched) {
-      const uint32_t nc = L.h_ncand[i];
+      const uint32_t nc = L.h_ncand[4 * i], ncl = L.h_ncand[4 * i + 1];
       size_t take = std::min<size_t>(nc, (size_t)s->cfg.cand_cap);
       take = std::min<size_t>(take, (size_t)s->cfg.max_giants);
       std::vector<Cand> cs(take);
+      std::vector<ClusterRec> cl(ncl);
       if (take)
         HAMD_CHECK(hipMemcpyAsync(cs.data(), L.ws[i].s_cands.p, take * sizeof(Cand),
+                                  hipMemcpyDeviceToHost, L.stream));
+      if (ncl)
+        HAMD_CHECK(hipMemcpyAsync(cl.data(), L.ws[i].c_out.p, ncl * sizeof(ClusterRec),
                                   hipMemcpyDeviceToHost, L.stream));
       HAMD_CHECK(hipStreamSynchronize(L.stream));
       B.crossings = nc;
       res.r.crossings = nc;
       append_candidates(s->cfg, s->W, B, sb.id, cs, (size_t)s->cfg.max_giants, res.cands);
-      cluster_candidates(res.cands, s->cfg.cluster_dt, s->cfg.cluster_ddm, res.clusters);
+      append_clusters(cl, 0, sb.id, res.clusters);
     }
     res.r.ncand = (int)res.cands.size();
     res.r.nclusters = (int)res.clusters.size();
