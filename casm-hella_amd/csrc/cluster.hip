@@ -9,14 +9,20 @@
 //      over candidate indices, the larger root hooked under the smaller by
 //      compare-and-swap, so a component's root is its lowest cell index);
 //   3. every candidate folds itself into its root: the (S/N, -index) key by
-//      atomic maximum, the count, and the sample and DM bounds;
+//      atomic maximum, the count, and the sample and DM bounds; lanes of a
+//      wave that share a root (neighbouring candidates mostly do) first
+//      combine their values by shuffles, so a large cluster does not
+//      serialise one atomic per candidate on its root;
 //   4. the candidates that represent their cluster are compacted in index
 //      order (wave ballots, workgroup scan).
 // The candidates taken are those the host keeps: the first min(crossings,
 // CAND_CAP, MAX_GIANTS) of the beam, and within a gulp's list no more than
 // MAX_GIANTS in all (the beams before this one counted in order).
-// Reads of words other waves change by atomics use device-scope atomic loads,
-// which bypass the CU's L1.
+// The union-find parents live in LDS for lists of up to PCAP candidates, the
+// other per-candidate words (keys, counts, bounds) for up to WCAP; beyond,
+// the work set's buffers hold them; the same generic pointers serve both.  Reads of words other waves change by
+// atomics use device-scope atomic loads (for global words they bypass the
+// CU's L1).
 #include <hip/hip_runtime.h>
 
 #include "search.hpp"
@@ -24,7 +30,10 @@
 namespace hamd {
 namespace {
 
-constexpr int CT = 256;  // threads per workgroup
+constexpr int CT = 1024;   // threads per workgroup
+constexpr int PCAP = 16384;  // candidates whose parents fit in LDS (4 B each)
+constexpr int WCAP = 3072;   // candidates whose other words fit in LDS (28 B each)
+constexpr size_t CLUSTER_LDS = (size_t)WCAP * 8 + (size_t)PCAP * 4 + (size_t)5 * WCAP * 4;
 constexpr uint32_t NONE = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t ld(const uint32_t *p)
@@ -100,6 +109,7 @@ __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict_
   const int b = blockIdx.x;
   const SearchBeamDev B = beams[b];
   const int l = threadIdx.x;
+  extern __shared__ __align__(16) unsigned long long lds_key[];  // [WCAP] keys, [PCAP] parents, 5 x [WCAP]
   __shared__ uint32_t sh[CT / 64];
   // candidates taken for this beam
   auto taken = [&](const SearchBeamDev &X) {
@@ -119,60 +129,109 @@ __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict_
     if (l == 0) B.cl_n[0] = 0u;
     return;
   }
-  uint32_t *par = B.cl_work;
-  uint32_t *cnt = par + B.cl_cap;
-  uint32_t *smin = cnt + B.cl_cap;
-  uint32_t *smax = smin + B.cl_cap;
-  uint32_t *dmin = smax + B.cl_cap;
-  uint32_t *dmax = dmin + B.cl_cap;
-  unsigned long long *key = B.cl_key;
+  const bool lpar = n <= PCAP, lw = n <= WCAP;
+  uint32_t *const lds_par = reinterpret_cast<uint32_t *>(lds_key + WCAP);
+  uint32_t *par = lpar ? lds_par : B.cl_work;
+  unsigned long long *key = lw ? lds_key : B.cl_key;
+  uint32_t *cnt = lw ? lds_par + PCAP : B.cl_work + B.cl_cap;
+  const int cap = lw ? WCAP : B.cl_cap;
+  uint32_t *smin = cnt + cap;
+  uint32_t *smax = smin + cap;
+  uint32_t *dmin = smax + cap;
+  uint32_t *dmax = dmin + cap;
   auto cell = [&](const Cand &c) {
     return (c.dm_idx / A.cd) * B.cl_cols + c.samp / A.ct;
   };
 
-  // 1. cells
-  for (int i = l; i < n; i += CT) atomicMin(B.cl_grid + cell(B.cands[i]), (uint32_t)i);
-  __threadfence();
-  __syncthreads();
-  // 2. cell heads: own parent, then joined with occupied neighbours
+  // 1. cells, and every candidate's words (only the cell heads' are used)
   for (int i = l; i < n; i += CT) {
-    const Cand c = B.cands[i];
-    if (ld(B.cl_grid + cell(c)) != (uint32_t)i) continue;
-    __hip_atomic_store(par + i, (uint32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(key + i, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(cnt + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(smin + i, NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(smax + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(dmin + i, NONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(dmax + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicMin(B.cl_grid + cell(B.cands[i]), (uint32_t)i);
+    par[i] = (uint32_t)i;
+    key[i] = 0ull;
+    cnt[i] = 0u;
+    smin[i] = NONE;
+    smax[i] = 0u;
+    dmin[i] = NONE;
+    dmax[i] = 0u;
   }
   __threadfence();
   __syncthreads();
+  // 2. cell heads join the occupied cells around them
   for (int i = l; i < n; i += CT) {
     const Cand c = B.cands[i];
-    if (ld(B.cl_grid + cell(c)) != (uint32_t)i) continue;
     const int r = c.dm_idx / A.cd, t = c.samp / A.ct;
+    if (ld(B.cl_grid + r * B.cl_cols + t) != (uint32_t)i) continue;
+    uint32_t g[8];
+    int k = 0;
+#pragma unroll
     for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
       for (int dt = -1; dt <= 1; ++dt) {
         if (!dr && !dt) continue;
         const int rr = r + dr, tt = t + dt;
-        if (rr < 0 || rr >= B.cl_rows || tt < 0 || tt >= B.cl_cols) continue;
-        const uint32_t g = ld(B.cl_grid + rr * B.cl_cols + tt);
-        if (g != NONE) unite(par, (uint32_t)i, g);
+        g[k++] = (rr < 0 || rr >= B.cl_rows || tt < 0 || tt >= B.cl_cols)
+                     ? NONE
+                     : ld(B.cl_grid + rr * B.cl_cols + tt);
       }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (g[q] != NONE) unite(par, (uint32_t)i, g[q]);
   }
   __threadfence();
   __syncthreads();
   // 3. every candidate into its root
-  for (int i = l; i < n; i += CT) {
-    const Cand c = B.cands[i];
-    const uint32_t root = find(par, ld(B.cl_grid + cell(c)));
-    atomicMax(key + root, ((unsigned long long)ordered(c.snr) << 32) | (unsigned long long)(NONE - (uint32_t)i));
-    atomicAdd(cnt + root, 1u);
-    atomicMin(smin + root, (uint32_t)c.samp);
-    atomicMax(smax + root, (uint32_t)c.samp);
-    atomicMin(dmin + root, (uint32_t)c.dm_idx);
-    atomicMax(dmax + root, (uint32_t)c.dm_idx);
+  const int lane = l & 63;
+  for (int base = 0; base < n; base += CT) {
+    const int i = base + l;
+    bool act = i < n;
+    uint32_t root = 0, sm = 0, dm = 0;
+    unsigned long long k = 0ull;
+    if (act) {
+      const Cand c = B.cands[i];
+      root = find(par, ld(B.cl_grid + cell(c)));
+      k = ((unsigned long long)ordered(c.snr) << 32) | (unsigned long long)(NONE - (uint32_t)i);
+      sm = (uint32_t)c.samp;
+      dm = (uint32_t)c.dm_idx;
+    }
+    // up to four groups of lanes with a common root, combined by shuffles
+    // (identities from the lanes outside the group); the rest add directly
+    for (int g = 0; g < 4; ++g) {
+      const unsigned long long pend = __ballot(act);
+      if (!pend) break;
+      const int lead = __ffsll((long long)pend) - 1;
+      const uint32_t r = (uint32_t)__shfl((int)root, lead, 64);
+      const bool mine = act && root == r;
+      unsigned long long kk = mine ? k : 0ull;
+      uint32_t cc = mine ? 1u : 0u, s0 = mine ? sm : NONE, s1 = mine ? sm : 0u;
+      uint32_t d0 = mine ? dm : NONE, d1 = mine ? dm : 0u;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long ko = __shfl_xor(kk, o, 64);
+        kk = ko > kk ? ko : kk;
+        cc += (uint32_t)__shfl_xor((int)cc, o, 64);
+        s0 = min(s0, (uint32_t)__shfl_xor((int)s0, o, 64));
+        s1 = max(s1, (uint32_t)__shfl_xor((int)s1, o, 64));
+        d0 = min(d0, (uint32_t)__shfl_xor((int)d0, o, 64));
+        d1 = max(d1, (uint32_t)__shfl_xor((int)d1, o, 64));
+      }
+      if (lane == lead) {
+        atomicMax(key + r, kk);
+        atomicAdd(cnt + r, cc);
+        atomicMin(smin + r, s0);
+        atomicMax(smax + r, s1);
+        atomicMin(dmin + r, d0);
+        atomicMax(dmax + r, d1);
+      }
+      act = act && !mine;
+    }
+    if (act) {
+      atomicMax(key + root, k);
+      atomicAdd(cnt + root, 1u);
+      atomicMin(smin + root, sm);
+      atomicMax(smax + root, sm);
+      atomicMin(dmin + root, dm);
+      atomicMax(dmax + root, dm);
+    }
   }
   __threadfence();
   __syncthreads();
@@ -213,7 +272,14 @@ void SearchRunner::cluster(const SearchBeamDev *d_beams, int nbeam, const Cluste
                            hipStream_t s)
 {
   if (nbeam <= 0) return;
-  hipLaunchKernelGGL(k_cluster, dim3((unsigned)nbeam), dim3(CT), 0, s, d_beams, A);
+  const size_t lds = CLUSTER_LDS;
+  static const bool attr = [&] {
+    HAMD_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_cluster),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return true;
+  }();
+  (void)attr;
+  hipLaunchKernelGGL(k_cluster, dim3((unsigned)nbeam), dim3(CT), lds, s, d_beams, A);
   HAMD_CHECK(hipGetLastError());
 }
 

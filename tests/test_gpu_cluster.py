@@ -124,3 +124,17 @@ def test_server_clusters_match_oracle(hamd, orc):
         srv.close()
         ring.close()
     assert checked > 0
+
+
+@pytest.mark.parametrize("snr,lo,hi", [(2.5, 2049, 16384), (2.0, 16385, 10 ** 9)])
+def test_large_lists(hamd, orc, snr, lo, hi):
+    """lists beyond the LDS capacities (parents in LDS and the other words
+    in the work set's buffers; then everything in the buffers)"""
+    cfg = _cfg(GULP=2048, NBEAM=3, SNR=snr, MAX_GIANTS=60000, CAND_CAP=60000, CLUSTER_DT=8,
+               CLUSTER_DDM=2)
+    seen = 0
+    for peaks, cl in _run(hamd, cfg, False, gulps=4):
+        assert _device_clusters(cl) == _oracle_clusters(orc, peaks, 8, 2)
+        for b in set(peaks["beam"].tolist()):
+            seen += int(lo <= int(np.sum(peaks["beam"] == b)) <= hi)
+    assert seen > 0
