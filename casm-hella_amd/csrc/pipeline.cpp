@@ -278,7 +278,12 @@ void WorkSet::ensure(const WorkDims &d)
   // empties the cells it used
   const void *g0 = c_grid.p;
   c_grid.ensure(std::max<size_t>(z.cgrid, 4));
-  if (c_grid.p != g0) HAMD_CHECK(hipMemset(c_grid.p, 0xff, c_grid.bytes));
+  if (c_grid.p != g0) {
+    // on the null stream, which the contexts' non-blocking streams do not
+    // wait for: complete it before any launch can read the grid
+    HAMD_CHECK(hipMemset(c_grid.p, 0xff, c_grid.bytes));
+    HAMD_CHECK(hipDeviceSynchronize());
+  }
   c_work.ensure(z.cwork);
   c_key.ensure(z.ckey);
   c_out.ensure(z.cout);
