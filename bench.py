@@ -26,6 +26,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+OUT_DIR = os.path.join(ROOT, "bench_out")
 
 CHUNK = 65536
 NCH = 1024
@@ -265,7 +266,12 @@ def main():
                          "(split as evenly as possible over the three kinds; 128 = 43/43/42)")
     ap.add_argument("--stress", type=int, default=0,
                     help="only the 4096-channel FP32 stress configuration, this many timed chunks")
+    ap.add_argument("--out-dir", type=str, default=os.path.join(ROOT, "bench_out"),
+                    help="where the autotuner records are written")
     args = ap.parse_args()
+    global OUT_DIR
+    OUT_DIR = args.out_dir
+    os.makedirs(OUT_DIR, exist_ok=True)
     if args.gpus != 1:
         print("bench.py measures one GPU only: beams shard across GPUs with no exchange, "
               "so the multi-GPU figure is the per-GPU one times N; it is not run here",
@@ -308,7 +314,7 @@ def main():
     for g in range(need):
         h.process_gulp(dev[g % ndistinct])
     if args.tune:
-        h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_plain.json"))
+        h.autotune(os.path.join(OUT_DIR, "autotune_plain.json"))
         h.process_gulp(dev[0])
     lat = []
     hamd.device_sync()
@@ -360,7 +366,7 @@ def main():
     for g in range(need):
         hb.process_gulp(gulp_dev[g % ndistinct], batched=True)
     if args.tune:
-        hb.autotune(os.path.join(ROOT, "gpurun_out", "autotune_c2.json"))
+        hb.autotune(os.path.join(OUT_DIR, "autotune_c2.json"))
         hb.process_gulp(gulp_dev[0], batched=True)
     blat = []
     hamd.device_sync()
@@ -451,7 +457,7 @@ def stress(hamd, gulps, tune=False):
     for g in range(need):
         h.process_gulp(bufs[g % 2])
     if tune:
-        h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_stress.json"))
+        h.autotune(os.path.join(OUT_DIR, "autotune_stress.json"))
         h.process_gulp(bufs[need % 2])
         need += 1
     hamd.device_sync()
@@ -526,7 +532,7 @@ def uneven(hamd, nbeam, reps=3):
         for g in range(need):
             h.process_gulp(buf, batched=batched)
         if batched and nb <= 32:
-            h.autotune(os.path.join(ROOT, "gpurun_out", "autotune_uneven.json"))
+            h.autotune(os.path.join(OUT_DIR, "autotune_uneven.json"))
         h.process_gulp(buf, batched=batched)
         hamd.device_sync()
         print(f"# uneven: {'batched' if batched else 'plain'} warm after {need + 1} chunks",

@@ -48,7 +48,7 @@ def test_dm_list_invariants(orc, name):
     assert np.all(np.abs(x - root) <= ulp)
 
 
-def test_survey_scale_numbers(orc):
+def test_production_scale_numbers(orc):
     """Order-of-magnitude pins quoted for the 256- and 1024-channel workloads."""
     p1 = orc.Plan(*GEOMS["c1_256ch"])
     p2 = orc.Plan(*GEOMS["c2_1024ch"])
