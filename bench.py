@@ -113,12 +113,28 @@ def serve_min_gulp(hamd):
     return ntime // SERVE_CHUNK
 
 
-def serve_pool(hamd, n, pulses, rfi):
-    """distinct chunks of the streaming configuration (reused across beams)"""
+def serve_pool(hamd, n, rfi, seed=20261204):
+    """n consecutive chunks of the streaming configuration, which the beams
+    cycle through: pulses by a Poisson process at 0.05/s with the production
+    single-beam distributions (DM 50-1800, FWHM log-uniform 0.2-10 ms, S/N
+    8-50), and a bandpass slope that drifts chunk by chunk over a triangle
+    between -0.6 and +0.6 (channel gains 0.7-1.3 at the extremes), besides the
+    persistent RFI channels and the broadband bursts"""
+    rng = np.random.default_rng(seed)
+    span = n * SERVE_CHUNK * TSAMP
+    t, pulses = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / 0.05))
+        if t >= span:
+            break
+        pulses.append((t, float(rng.uniform(50, 1800)),
+                       float(np.exp(rng.uniform(np.log(0.2), np.log(10.0)))),
+                       float(rng.uniform(8, 50))))
+    slopes = [0.6 * (1.0 - abs(4.0 * k / n - 2.0)) for k in range(n)]
     return [hamd.synth(NCH, SERVE_CHUNK, 500.0, FOFF, TSAMP, seed=777, t_offset=k * SERVE_CHUNK,
                        rfi_chan=rfi, rfi_mult=10.0, pulses=pulses,
                        burst_period=int(round(2.0 / TSAMP)), burst_phase=1000, burst_len=10,
-                       burst_amp=6.0) for k in range(n)]
+                       burst_amp=6.0, bp_slope=slopes[k]) for k in range(n)]
 
 
 def serve_saturation(hamd, pool, mg, nbeam, nchunk, **opts):
@@ -289,7 +305,7 @@ def main():
         print(json.dumps({"stress": stress(hamd, args.stress, args.tune)}), flush=True)
         return
     if args.serve_only:
-        print(json.dumps({"serving": serving(hamd, args, pulses, rfi)}), flush=True)
+        print(json.dumps({"serving": serving(hamd, args, rfi)}), flush=True)
         return
     h = hamd.Hella(config(1))
     if args.tile:
@@ -417,7 +433,7 @@ def main():
                              "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
     }
     if not args.no_serve:
-        out["serving"] = serving(hamd, args, pulses, rfi)
+        out["serving"] = serving(hamd, args, rfi)
         out["realtime_beams"]["serving_plain"] = out["serving"]["saturation"]["plain"]["realtime_beams"]
         out["realtime_beams"]["serving_batched"] = \
             out["serving"]["saturation"]["batched"]["realtime_beams"]
@@ -564,10 +580,10 @@ def uneven(hamd, nbeam, reps=3):
             "last_chunk_identical": bool(np.array_equal(pk_b, pk_p))}
 
 
-def serving(hamd, args, pulses, rfi):
+def serving(hamd, args, rfi):
     mg = serve_min_gulp(hamd)
     t0 = time.perf_counter()
-    pool = serve_pool(hamd, 24, pulses, rfi)
+    pool = serve_pool(hamd, 24, rfi)
     print(f"# serving: pool in {time.perf_counter() - t0:.1f} s, windows fill in {mg} chunks",
           file=sys.stderr, flush=True)
     sat = {}
@@ -584,7 +600,8 @@ def serving(hamd, args, pulses, rfi):
               f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
     return {"workload": f"streaming: 1024 ch, {SERVE_CHUNK}-sample chunks "
                         f"({SERVE_CHUNK * TSAMP:.3f} s), DM 0-2000, widths 1..256, "
-                        f"HBM budget {SERVE_BUDGET >> 30} GB",
+                        f"HBM budget {SERVE_BUDGET >> 30} GB, pulses 0.05/s per beam, "
+                        "bandpass slope drifting between -0.6 and +0.6",
             "chunk_s": SERVE_CHUNK * TSAMP, "window_fill_chunks": mg,
             "saturation": sat,
             "batched_over_plain": sat["batched"]["chunks_per_s"] / sat["plain"]["chunks_per_s"],
