@@ -166,11 +166,12 @@ def serve_saturation(hamd, pool, mg, nbeam, nchunk, **opts):
             "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)}
 
 
-def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, **opts):
+def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opts):
     """nbeam warm beams stream chunks on the real-time cadence (random phase);
     beams leave after exponential dwell (mean 20 s) and a new beam joins in
     their place, filling its window before it is searched.  Latency = chunk
-    arrival to candidates on the host."""
+    arrival to candidates on the host.  The stream is replayed in segments of
+    segment_s seconds through the same server (a progress line after each)."""
     rng = np.random.default_rng(seed)
     chunk_s = SERVE_CHUNK * TSAMP
     srv = hamd.Server(serve_config(), hbm_budget=SERVE_BUDGET, **opts)
@@ -198,11 +199,18 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, **opts):
                 next_id += 1
                 leave = t + float(rng.exponential(20.0))
     ev.sort(key=lambda e: (e[0], e[1]))
-    ring = hamd.Ring(256, pool[0].nbytes, f"NCHAN {NCH}\n")
     t0 = time.perf_counter()
-    late = hamd.serve(srv, ring, ev, pool, time_scale=1.0)
+    late, seg0 = 0.0, 0.0
+    while seg0 < seconds:
+        seg = [(t - seg0,) + e[1:] for e in ev for t in (e[0],) if seg0 <= t < seg0 + segment_s]
+        ring = hamd.Ring(256, pool[0].nbytes, f"NCHAN {NCH}\n")
+        late = max(late, hamd.serve(srv, ring, seg, pool, time_scale=1.0))
+        ring.close()
+        seg0 += segment_s
+        if seg0 < seconds:
+            print(f"# serving real time: {min(seg0, seconds):.0f} of {seconds:.0f} s streamed, "
+                  f"{srv.stats()['chunks'] - n0} chunks", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    ring.close()
     res = srv.results(with_candidates=False)[n0:]
     st = srv.stats()
     srv.close()
@@ -275,6 +283,8 @@ def main():
     ap.add_argument("--serve-rt-beams", type=int, default=256)
     ap.add_argument("--serve-rt-seconds", type=float, default=12.0,
                     help="length of the real-time run (0 skips it)")
+    ap.add_argument("--serve-rt-segment", type=float, default=60.0,
+                    help="the real-time stream is replayed in segments of this length")
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--uneven", type=int, default=0,
@@ -595,7 +605,8 @@ def serving(hamd, args, rfi):
     rt = None
     if args.serve_rt_seconds > 0:
         rt = serve_realtime(hamd, pool, mg, args.serve_rt_beams, args.serve_rt_seconds,
-                            max_batch=args.serve_batch, lanes=args.serve_lanes)
+                            segment_s=args.serve_rt_segment, max_batch=args.serve_batch,
+                            lanes=args.serve_lanes)
         print(f"# serving real time: {rt['beams']} beams, p50 {rt['latency_ms_p50']} ms, "
               f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
     return {"workload": f"streaming: 1024 ch, {SERVE_CHUNK}-sample chunks "
