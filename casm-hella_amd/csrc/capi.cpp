@@ -97,6 +97,8 @@ void hella_destroy(hella_ctx c)
   if (c->h_ddesc) (void)hipHostFree(c->h_ddesc);
   if (c->h_sdesc) (void)hipHostFree(c->h_sdesc);
   if (c->h_ncand) (void)hipHostFree(c->h_ncand);
+  if (c->h_cstage) (void)hipHostFree(c->h_cstage);
+  if (c->h_kstage) (void)hipHostFree(c->h_kstage);
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);

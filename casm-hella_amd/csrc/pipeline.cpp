@@ -102,6 +102,9 @@ bool Config::parse(const std::string &text, std::string &err, bool beam_only)
 }
 
 // ------------------------------------------------------------------ build
+// candidates and clusters staged in pinned memory by the plain path per beam
+constexpr size_t STAGE_N = 2048;
+
 static void alloc_zero(DevBuf &b, size_t bytes, hipStream_t s)
 {
   b.ensure(std::max<size_t>(bytes, 16));
@@ -419,6 +422,8 @@ void ctx_build(hella_ctx c)
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_cstage, STAGE_N * sizeof(Cand), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_kstage, STAGE_N * sizeof(ClusterRec), hipHostMallocDefault));
   c->d_fdesc.ensure(nb * sizeof(FlagBeamDev));
   c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
@@ -685,19 +690,31 @@ int ctx_find_peaks(hella_ctx c, int b)
     SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
     SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
     SearchRunner::cluster(sd, 1, cluster_args(c->cfg, rep0, false), c->stream);
+    // the counts and the first STAGE_N candidates and clusters in one round
+    // trip (pinned staging); longer lists fetch the rest after the counts
     uint32_t *hn = c->h_ncand + 4 * (size_t)b;
+    const size_t cap = std::min<size_t>(STAGE_N, (size_t)w.s_cands.bytes / sizeof(Cand));
+    const size_t kcap = std::min<size_t>(STAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
     HAMD_CHECK(hipMemcpyAsync(hn, w.s_ncand.p, 12, hipMemcpyDeviceToHost, c->stream));
+    HAMD_CHECK(hipMemcpyAsync(c->h_cstage, w.s_cands.p, cap * sizeof(Cand), hipMemcpyDeviceToHost,
+                              c->stream));
+    HAMD_CHECK(hipMemcpyAsync(c->h_kstage, w.c_out.p, kcap * sizeof(ClusterRec),
+                              hipMemcpyDeviceToHost, c->stream));
     HAMD_CHECK(hipStreamSynchronize(c->stream));
     nc = hn[0];
     cs.resize(beam_take(c->cfg, nc));
     cl.resize(hn[1]);
-    if (!cs.empty())
-      HAMD_CHECK(hipMemcpyAsync(cs.data(), w.s_cands.p, cs.size() * sizeof(Cand),
-                                hipMemcpyDeviceToHost, c->stream));
-    if (!cl.empty())
-      HAMD_CHECK(hipMemcpyAsync(cl.data(), w.c_out.p, cl.size() * sizeof(ClusterRec),
-                                hipMemcpyDeviceToHost, c->stream));
-    if (!cs.empty() || !cl.empty()) HAMD_CHECK(hipStreamSynchronize(c->stream));
+    const size_t c0 = std::min(cs.size(), cap), k0 = std::min(cl.size(), kcap);
+    std::copy(c->h_cstage, c->h_cstage + c0, cs.begin());
+    std::copy(c->h_kstage, c->h_kstage + k0, cl.begin());
+    if (cs.size() > c0)
+      HAMD_CHECK(hipMemcpyAsync(cs.data() + c0, w.s_cands.as<Cand>() + c0,
+                                (cs.size() - c0) * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
+    if (cl.size() > k0)
+      HAMD_CHECK(hipMemcpyAsync(cl.data() + k0, w.c_out.as<ClusterRec>() + k0,
+                                (cl.size() - k0) * sizeof(ClusterRec), hipMemcpyDeviceToHost,
+                                c->stream));
+    if (cs.size() > c0 || cl.size() > k0) HAMD_CHECK(hipStreamSynchronize(c->stream));
   }
   B.crossings = nc;
   append_candidates(c->cfg, c->W, B, b, cs, (size_t)c->cfg.max_giants, c->peaks);

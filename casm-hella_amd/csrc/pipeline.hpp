@@ -123,6 +123,10 @@ struct hella_ctx_s {
   hamd::DDBeamDev *h_ddesc = nullptr;
   hamd::SearchBeamDev *h_sdesc = nullptr;
   uint32_t *h_ncand = nullptr;
+  // pinned staging of the plain path's first candidates and clusters (one
+  // round trip when a beam's lists fit)
+  hamd::Cand *h_cstage = nullptr;
+  hamd::ClusterRec *h_kstage = nullptr;
   hamd::DevBuf d_fdesc, d_ddesc, d_sdesc;
   hamd::DevBuf d_in;           // staged gulp
   uint8_t *h_in = nullptr;     // pinned staging for host gulps
