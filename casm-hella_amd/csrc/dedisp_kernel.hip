@@ -508,6 +508,10 @@ void launch_pf(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, 
 // Requirements: ring % 16 == 0 (every beam's ring is a multiple of 32 or
 // 64 columns) and the DM-major delay table readable for DD_DELTA_PAD
 // entries past its end.
+// s_waitcnt immediate (gfx9 encoding): at most 8 LDS/scalar requests
+// outstanding, vector memory and export counters not waited on
+constexpr int WAIT_LGKM_8 = 15 | (7 << 4) | (8 << 8) | (3 << 14);
+
 template <int NDW, int NU, bool PIPE>
 __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restrict__ beams,
                                                      const DDTileDesc *__restrict__ tiles,
@@ -556,10 +560,13 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(B.base + (live ? (int64_t)sch * B.row_stride : 0)), (short)0, live ? ring : 0,
         0x00020000);
+    // no wrap: one lane offset, the units 1024 bytes apart by the scalar
+    // offset (no per-unit vector additions)
+    const bool nowrap = al + NU * 1024 <= ring;
     int col[NU];
-    if (al + NU * 1024 <= ring) {
+    if (nowrap) {
 #pragma unroll
-      for (int u = 0; u < NU; ++u) col[u] = al + lane16 + u * 1024;
+      for (int u = 0; u < NU; ++u) col[u] = al + lane16;
     } else {
       // the staged units run past the ring end (short rings may wrap more
       // than once)
@@ -571,7 +578,8 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
       }
     }
 #pragma unroll
-    for (int u = 0; u < NU; ++u) P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col[u], 0, 0);
+    for (int u = 0; u < NU; ++u)
+      P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col[u], nowrap ? u * 1024 : 0, 0);
   };
   auto store = [&](int buf) {
     uint8_t *dst = lds + (size_t)(buf * CC + wave) * lrow + lane16;
@@ -654,21 +662,42 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
       // every channel's LDS words of the chunk are requested before the
       // first v_perm, so the reads of later channels overlap the sums
       uint32_t w[CC][NW], so[CC];
-#pragma unroll
-      for (int cc = 0; cc < CC; ++cc) {
+      auto rd = [&](int cc) {
         so[cc] = se[cc] + 0x00010001u;
         const uint32_t *wrow = reinterpret_cast<const uint32_t *>(
             lds + (buf * CC + cc) * lrow + dw[cc] + lane_off);
 #pragma unroll
         for (int i = 0; i < NW; ++i) w[cc][i] = wrow[i];
-      }
-#pragma unroll
-      for (int cc = 0; cc < CC; ++cc)
+      };
+      // two channels per accumulator update (three-operand additions)
+      auto sum2 = [&](int cc) {
 #pragma unroll
         for (int k = 0; k < V / 4; ++k) {
-          acc[2 * k] += perm(w[cc][k + 1], w[cc][k], se[cc]);
-          acc[2 * k + 1] += perm(w[cc][k + 1], w[cc][k], so[cc]);
+          acc[2 * k] += perm(w[cc][k + 1], w[cc][k], se[cc]) +
+                        perm(w[cc + 1][k + 1], w[cc + 1][k], se[cc + 1]);
+          acc[2 * k + 1] += perm(w[cc][k + 1], w[cc][k], so[cc]) +
+                            perm(w[cc + 1][k + 1], w[cc + 1][k], so[cc + 1]);
         }
+      };
+      if constexpr (CC == 4) {
+        // three channels' reads in flight (12 of the 15 LDS requests the
+        // counter tracks); the fourth is issued once the first has landed,
+        // so the sums of each pair overlap the reads of the next
+        rd(0);
+        rd(1);
+        rd(2);
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM_8);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(3);
+        __builtin_amdgcn_sched_barrier(0);
+        sum2(0);
+        sum2(2);
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < CC; ++cc) rd(cc);
+#pragma unroll
+        for (int cc = 0; cc < CC; cc += 2) sum2(cc);
+      }
     } else {
 #pragma unroll
       for (int cc = 0; cc < CC; cc += 2) {
