@@ -120,3 +120,93 @@ def test_stress_geometry_full_size_spot_checks(hamd, orc):
             np.testing.assert_array_equal(row, want_row.astype(np.float32))
     din.free()
     dout.free()
+
+
+ERR = ["NO_ERROR", "MEM_ALLOC_FAILED", "MEM_COPY_FAILED", "NCHANS_EXCEEDS_LIMIT", "INVALID_PLAN",
+       "INVALID_POINTER", "INVALID_STRIDE", "NO_DM_LIST_SET", "TOO_FEW_NSAMPS",
+       "INVALID_FLAG_COMBINATION", "UNSUPPORTED_IN_NBITS", "UNSUPPORTED_OUT_NBITS",
+       "INVALID_DEVICE_INDEX", "DEVICE_ALREADY_SET", "PRIOR_GPU_ERROR", "INTERNAL_GPU_ERROR",
+       "UNKNOWN_ERROR"]
+E = {n: i for i, n in enumerate(ERR)}
+
+
+def test_error_codes(hamd, orc):
+    """invalid calls return dedisp's error values (hella turns any non-zero
+    value into an exception, src/hella/src/dedisperse.cpp:40-46) and leave
+    the plan usable"""
+    import ctypes
+    L = hamd.lib()
+    nch, nout = 32, 100
+    lp, op = _plan(hamd, orc, nch, 256e-6, 500.0, -3.125, 0.0, 100.0)
+    md, ndm = op.max_delay, op.dms.size
+    ns = md + nout
+    rng = np.random.default_rng(11)
+    data = rng.integers(0, 256, size=(ns, nch), dtype=np.uint8)
+    din = hamd.DeviceBuffer(data.nbytes)
+    din.upload(data)
+    dout = hamd.DeviceBuffer(ndm * nout * 4)
+    DP, HP, WAIT, ASYNC = 1 << 2, 1 << 1, 1 << 3, 1 << 4
+
+    def ex(plan=lp._p, n=ns, i=None, ib=8, ist=nch, o=None, ob=32, ost=nout * 4, fl=DP):
+        return L.dedisp_execute_adv(plan, n, din.ptr if i is None else i, ib, ist,
+                                    dout.ptr if o is None else o, ob, ost, fl)
+
+    assert ex(plan=None) == E["INVALID_PLAN"]
+    assert ex(i=0) == E["INVALID_POINTER"] and ex(o=0) == E["INVALID_POINTER"]
+    assert ex(fl=DP | HP) == E["INVALID_FLAG_COMBINATION"]
+    assert ex(fl=DP | WAIT | ASYNC) == E["INVALID_FLAG_COMBINATION"]
+    assert ex(ib=4) == E["UNSUPPORTED_IN_NBITS"] and ex(ib=32) == E["UNSUPPORTED_IN_NBITS"]
+    assert ex(ob=16) == E["UNSUPPORTED_OUT_NBITS"]
+    assert ex(n=md) == E["TOO_FEW_NSAMPS"] and ex(n=0) == E["TOO_FEW_NSAMPS"]
+    assert ex(ist=nch - 1) == E["INVALID_STRIDE"]
+    assert ex(ost=nout * 4 - 4) == E["INVALID_STRIDE"] and ex(ost=nout * 4 + 2) == E["INVALID_STRIDE"]
+    L.dedisp_get_error_string.restype = ctypes.c_char_p
+    for i in range(len(ERR)):
+        assert L.dedisp_get_error_string(i)
+    # after every rejected call the plan still computes the right sums
+    assert ex() == 0
+    hamd.device_sync()
+    got = np.zeros((ndm, nout), np.float32)
+    L.hella_memcpy(got.ctypes.data, dout.ptr, got.nbytes, 2)
+    ref = orc.dedisperse_tf(data, op.delta, nout)
+    np.testing.assert_array_equal(got, ref.astype(np.float32))
+    lp.close()
+    # a plan without a DM list; invalid plans and DM lists
+    fresh = hamd.DedispPlan(nch, 256e-6, 500.0, -3.125)
+    assert ex(plan=fresh._p) == E["NO_DM_LIST_SET"]
+    bad = (ctypes.c_float * 3)(0.0, 20.0, 10.0)
+    assert L.dedisp_set_dm_list(fresh._p, bad, 3) == E["UNKNOWN_ERROR"]
+    assert L.dedisp_set_dm_list(fresh._p, bad, 0) == E["INVALID_POINTER"]
+    assert L.dedisp_generate_dm_list(fresh._p, ctypes.c_float(50.0), ctypes.c_float(10.0),
+                                     ctypes.c_float(64.0), ctypes.c_float(1.25)) == E["UNKNOWN_ERROR"]
+    fresh.close()
+    p = ctypes.c_void_p()
+    assert L.dedisp_create_plan(ctypes.byref(p), 0, ctypes.c_float(256e-6), ctypes.c_float(500.0),
+                                ctypes.c_float(-1.0)) == E["UNKNOWN_ERROR"]
+    assert L.dedisp_create_plan(ctypes.byref(p), 1 << 17, ctypes.c_float(256e-6),
+                                ctypes.c_float(500.0), ctypes.c_float(-1e-4)) == E["NCHANS_EXCEEDS_LIMIT"]
+
+
+def test_custom_dm_lists(hamd, orc):
+    """dedisp_set_dm_list: a single zero DM (every sample's channel sum, no
+    delay) and the first trials of a generated list (the oracle's delays of
+    those trials)"""
+    nch = 48
+    lp, op = _plan(hamd, orc, nch, 256e-6, 500.0, -2.0, 0.0, 300.0)
+    rng = np.random.default_rng(12)
+    ns = op.max_delay + 1234
+    data = rng.integers(0, 256, size=(ns, nch), dtype=np.uint8)
+    sub = hamd.DedispPlan(nch, 256e-6, 500.0, -2.0)
+    sub.set_dm_list(np.array([0.0], np.float32))
+    assert sub.dm_count == 1 and sub.max_delay == 0
+    got = sub.execute_adv(data)
+    np.testing.assert_array_equal(got[0], data.sum(axis=1).astype(np.float32))
+    k = 7
+    sub.set_dm_list(op.dms[:k].astype(np.float32))
+    md = int(op.delta[:k].max())
+    assert sub.dm_count == k and sub.max_delay == md
+    got = sub.execute_adv(data)
+    ref = orc.dedisperse_tf(data, op.delta[:k], ns - md)
+    np.testing.assert_array_equal(got, ref.astype(np.float32))
+    sub.close()
+    lp.close()
