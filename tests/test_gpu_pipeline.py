@@ -302,3 +302,41 @@ def test_sigproc_file_to_candidates(hamd, tmp_path):
     p = h.peaks()
     best = p[np.argmax(p["snr"])]
     assert abs(best["dm"] - 250.0) < 2.0
+
+
+@pytest.mark.parametrize("case", ["constant", "zeros", "one_live_channel", "all_killed"])
+def test_degenerate_inputs(hamd, orc, case):
+    """inputs whose statistics degenerate (zero variance, every channel
+    flagged or killed): the flagger bytes, the dedispersed sums, the
+    per-width statistics (NaN where the oracle's are NaN), the S/N gate and
+    the candidate list still follow the oracle"""
+    cfg = _cfg(INPUT="FILE", GULP=4096)
+    nch = cfg["NCHAN"]
+    if case == "constant":
+        gulp = np.full((4096, nch), 100, np.uint8)
+    elif case == "zeros":
+        gulp = np.zeros((4096, nch), np.uint8)
+    else:
+        gulp = _synth(hamd, cfg, 4096, seed=21, pulses=[(0.5, 40.0, 1.0, 30.0)])
+    km = np.ones(nch, np.int32)
+    if case == "one_live_channel":
+        km[:] = 0
+        km[17] = 1
+    elif case == "all_killed":
+        km[:] = 0
+    h = hamd.Hella(cfg)
+    if case in ("one_live_channel", "all_killed"):
+        h.set_killmask(0, km)
+    h.process_gulp(gulp)
+    win = h.window(0)[:, :4096]
+    ref_flag = _oracle_flag(orc, cfg, gulp)
+    np.testing.assert_array_equal(win, ref_flag)
+    op = _oracle_plan(orc, cfg)
+    i = h.info(0)
+    D = h.dedisp_plane(0)
+    np.testing.assert_array_equal(
+        D, orc.dedisperse_ft(ref_flag, op.delta, i["ntime_dd"], killmask=km))
+    if case == "all_killed":
+        assert not D.any()
+    _check_search(orc, h, cfg, D)
+    h.close()
