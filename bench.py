@@ -35,25 +35,39 @@ FOFF = -100.0 / NCH
 MAXW = 256
 SNR = 7.0
 PEAK_HBM_GBS = 8000.0
+PEAK_LDS_GBS = 128 * 256 * 2.4  # LDS read bandwidth: 128 B per CU per cycle, 256 CUs, 2.4 GHz
 PEAK_ADD_T = 78.6  # packed u16 integer adds/s (256 CU x 64 lanes x 2 x 2.4 GHz), 1e12
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_1024ch_dm2000_counters.md")
 
 
-def profiled_traffic(name):
+def profiled_traffic(name, key="traffic_bytes_per_launch"):
     """HBM bytes per launch of a kernel (or per chunk of a stage) from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload, or
-    None when the summary does not list it."""
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (or,
+    with key, another per-launch figure the summary lists), or None when the
+    summary does not list it."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 parts = line.split()
-                if line.startswith("machine-readable: traffic_bytes_per_launch") and parts[2] == name:
+                if line.startswith("machine-readable: " + key) and parts[2] == name:
                     return float(parts[3])
     except OSError:
         pass
     return None
+
+
+def lds_roof(ms):
+    """LDS read throughput of the dedispersion kernel: the bytes one launch
+    reads from LDS (SQ_INSTS_LDS_LOAD_BANDWIDTH x 64 B in the committed
+    counter summary) over the launch time measured in this run"""
+    b = profiled_traffic("k_dedisp", "lds_load_bytes_per_launch")
+    if b is None or not ms:
+        return None
+    a = b / (ms * 1e-3) / 1e9
+    return {"achieved": a, "peak": PEAK_LDS_GBS, "unit": "GB/s", "frac": a / PEAK_LDS_GBS,
+            "bytes": b}
 
 
 def load_pkg():
@@ -435,7 +449,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_dedisp", "achieved": achieved_add,
                      "peak": PEAK_ADD_T, "unit": "Tadd/s", "frac": achieved_add / PEAK_ADD_T,
                      "traffic": profiled_traffic("k_dedisp"),
-                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+                     "lds": lds_roof(ms_dd)},
         "roofline_flagger": {"bound": "hbm", "achieved": flag_bytes / (ms_flag * 1e-3) / 1e9,
                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": flag_bytes / (ms_flag * 1e-3) / 1e9 / PEAK_HBM_GBS,
