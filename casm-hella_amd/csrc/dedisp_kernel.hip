@@ -692,6 +692,33 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
         __builtin_amdgcn_sched_barrier(0);
         sum2(0);
         sum2(2);
+      } else if constexpr (CC == 8) {
+        // the same rolling window over eight channels: at most three
+        // channels' reads in flight, the next one issued as soon as the
+        // oldest has landed, each pair summed once both are in
+        rd(0);
+        rd(1);
+        rd(2);
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM_8);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(3);
+        __builtin_amdgcn_sched_barrier(0);
+        sum2(0);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(4);
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM_8);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(5);
+        __builtin_amdgcn_sched_barrier(0);
+        sum2(2);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(6);
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM_8);
+        __builtin_amdgcn_sched_barrier(0);
+        rd(7);
+        __builtin_amdgcn_sched_barrier(0);
+        sum2(4);
+        sum2(6);
       } else {
 #pragma unroll
         for (int cc = 0; cc < CC; ++cc) rd(cc);
