@@ -85,11 +85,18 @@ def config(nbeam=1):
                 SNR=SNR, NBEAM=nbeam, SCRUNCH=[(1, 1, 6.0)])
 
 
-def stream_params(seed=20261205, duration_s=60.0):
+CYCLE = 8  # distinct chunks; the timed stream repeats them
+
+
+def stream_params(seed=20261205):
+    """20 pulses per chunk (DM U(50,1800), FWHM log-uniform 0.2-10 ms, S/N
+    U(8,50)) over one cycle of CYCLE chunks, so every searched window holds
+    about 20 arrivals; the 5% RFI channels"""
     rng = np.random.default_rng(seed)
+    period = CYCLE * CHUNK * TSAMP
     pulses = []
-    for _ in range(20):
-        pulses.append((float(rng.uniform(0.5, duration_s - 0.5)), float(rng.uniform(50, 1800)),
+    for _ in range(20 * CYCLE):
+        pulses.append((float(rng.uniform(0.0, period)), float(rng.uniform(50, 1800)),
                        float(np.exp(rng.uniform(np.log(0.2), np.log(10.0)))),
                        float(rng.uniform(8, 50))))
     rfi = rng.choice(NCH, int(0.05 * NCH), replace=False)
@@ -97,8 +104,14 @@ def stream_params(seed=20261205, duration_s=60.0):
 
 
 def make_chunk(hamd, g, pulses, rfi, seed=20261205):
+    """chunk g of the cyclic stream: the pulses repeat every CYCLE chunks (the
+    copies one and two cycles earlier carry the sweeps that continue into the
+    first chunks), so a stream that replays chunks 0..CYCLE-1 in a loop is
+    continuous, sweeps included"""
+    period = CYCLE * CHUNK * TSAMP
+    per = [(t - k * period,) + p[1:] for k in (0, 1, 2) for p in pulses for t in (p[0],)]
     return hamd.synth(NCH, CHUNK, 500.0, FOFF, TSAMP, seed=seed, t_offset=g * CHUNK,
-                      rfi_chan=rfi, rfi_mult=10.0, pulses=pulses,
+                      rfi_chan=rfi, rfi_mult=10.0, pulses=per,
                       burst_period=int(round(2.0 / TSAMP)), burst_phase=1000, burst_len=10,
                       burst_amp=6.0)
 
@@ -240,9 +253,23 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opt
             "work_set_GB": st["work_bytes"] / 2**30}
 
 
-def cpu_baseline(hamd, chunk_tf, sample_rows):
-    """Oracle flag on the whole chunk plus dedisperse+search on a sample of DM
-    rows, scaled to the whole chunk (DM rows cost the same)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hamd, chunk_tf, sample_rows=0):
+    """The oracle (C restatement of the reference algorithm, OpenMP over
+    channels, samples and DM rows) on one whole production chunk: flagger on
+    the chunk, dedispersion of all trials over the chunk plus its tail, and
+    the boxcar search of the whole plane.  sample_rows > 0 times that many DM
+    trials instead and scales them to all trials."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as orc
 
@@ -252,25 +279,33 @@ def cpu_baseline(hamd, chunk_tf, sample_rows):
     x = orc.ingest_u8_tf(chunk_tf)
     flagged, _ = orc.flag(x, -0.05, 0.15, -1, -1, ((1, 1, 6.0),))
     t_flag = time.perf_counter() - t0
-    # window: the flagged chunk followed by itself stands in for chunk + tail
+    del x
+    # window: the flagged chunk repeated stands in for chunk + tail (the cost
+    # does not depend on the bytes)
     ntime_dd = CHUNK + MAXW
     reps = (ntime_dd + plan.max_delay + CHUNK - 1) // CHUNK
     win = np.tile(flagged, (1, reps))
-    rows = np.linspace(0, ndm - 1, sample_rows).astype(int)
+    rows = np.arange(ndm) if sample_rows <= 0 else np.linspace(0, ndm - 1, sample_rows).astype(int)
     t1 = time.perf_counter()
     D = orc.dedisperse_ft(win, plan.delta[rows], ntime_dd)
     t_dd = time.perf_counter() - t1
+    del win
     t2 = time.perf_counter()
     orc.search(D, CHUNK, MAXW, 1, 0, SNR)
     t_s = time.perf_counter() - t2
-    t_chunk = t_flag + (t_dd + t_s) * ndm / sample_rows
+    scale = ndm / len(rows)
+    t_chunk = t_flag + (t_dd + t_s) * scale
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    what = ("the whole chunk" if scale == 1 else
+            f"{len(rows)} of {ndm} DM trials, scaled to all trials")
     return {"value": ndm * CHUNK / t_chunk, "unit": "DM-trial·samples/s", "cores": cores,
-            "kind": "port",
-            "sample": (f"oracle (C, -O3, OpenMP over DM rows; flagger single-threaded): flagger on "
-                       f"one full chunk ({t_flag:.2f} s), dedispersion + boxcar search on "
-                       f"{sample_rows} of {ndm} DM trials ({t_dd + t_s:.2f} s), scaled to all "
-                       f"trials: {t_chunk:.1f} s per chunk")}
+            "kind": "port", "cpu_model": cpu_model(),
+            "stages_s": {"flag": t_flag, "dedisperse": t_dd * scale, "search": t_s * scale},
+            "sample": (f"oracle (C, -O3 -march=x86-64-v3, OpenMP, {cores} threads) on one "
+                       f"production chunk: flagger ({t_flag:.2f} s), dedispersion of "
+                       f"{len(rows)} trials x {ntime_dd} samples x {NCH} channels "
+                       f"({t_dd:.2f} s), boxcar search ({t_s:.2f} s); {what}: "
+                       f"{t_chunk:.1f} s per chunk")}
 
 
 def main():
@@ -279,7 +314,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--batch-beams", type=int, default=4)
-    ap.add_argument("--cpu-rows", type=int, default=96)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="time this many DM trials of the CPU baseline and scale (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true")
     ap.add_argument("--tile", type=str, default=None,
@@ -338,7 +374,7 @@ def main():
     info = h.info(0)
     need = max(args.warmup, info["min_gulp"] + 1)
     nchunks = need + args.steps
-    ndistinct = min(nchunks, 8)
+    ndistinct = CYCLE
     t_gen = time.perf_counter()
     host_chunks = [make_chunk(hamd, g, pulses, rfi) for g in range(ndistinct)]
     dev = []
@@ -356,18 +392,19 @@ def main():
     if args.tune:
         h.autotune(os.path.join(OUT_DIR, "autotune_plain.json"))
         h.process_gulp(dev[0])
-    lat = []
+    lat, ncands, nclust = [], [], []
     hamd.device_sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
-        h.process_gulp(dev[(need + k) % ndistinct])
+        ncands.append(h.process_gulp(dev[(need + k) % ndistinct]))
         lat.append((time.perf_counter() - ts) * 1e3)
+        nclust.append(len(h.clusters()))
     hamd.device_sync()
     t_plain = (time.perf_counter() - t0) / args.steps
-    ncand = len(h.peaks())
-    print(f"# plain: {t_plain * 1e3:.2f} ms/chunk, {ncand} candidates", file=sys.stderr,
-          flush=True)
+    crossings = h.crossings(0)
+    print(f"# plain: {t_plain * 1e3:.2f} ms/chunk, candidates per step {ncands}, "
+          f"clusters per step {nclust}", file=sys.stderr, flush=True)
     # per-stage device times (one extra step with events between stages)
     h.set_timing(True)
     h.process_gulp(dev[0])
@@ -380,6 +417,19 @@ def main():
     adds = info["ndm"] * info["ntime_dd"] * info["nlive"]
     achieved_add = adds / (ms_dd * 1e-3) / 1e12
     flag_bytes = 3 * NCH * CHUNK * 1  # one stats read, one read+write apply (u8 equivalents)
+    # two reads of the u32 plane (statistics, then the sigma0 / threshold pass)
+    search_bytes = 2 * info["ndm"] * CHUNK * 4
+    # PCIe-inclusive rate: the same steps from host (pageable) chunks, the
+    # per-chunk host-to-device copy inside the timed region
+    hamd.device_sync()
+    tp = time.perf_counter()
+    npc = min(args.steps, 5)
+    for k in range(npc):
+        h.process_gulp(host_chunks[(need + args.steps + k) % ndistinct])
+    hamd.device_sync()
+    t_pcie = (time.perf_counter() - tp) / npc
+    pcie = {"ms_per_step": t_pcie * 1e3, "value": info["ndm"] * CHUNK / t_pcie, "steps": npc,
+            "note": "host (pageable) chunk copied to the device inside each step"}
     h.close()
     if args.calib_copy_mb:
         nbytes = args.calib_copy_mb << 20
@@ -439,6 +489,9 @@ def main():
                                f"({info['ndm']} trials), boxcar widths 1..256",
                    "ndm": info["ndm"], "max_delay": info["max_delay"], "chunk": CHUNK,
                    "nchan": NCH, "scrunch": "1 1 6"},
+        "candidates_per_step": ncands, "clusters_per_step": nclust,
+        "crossings_last_step": crossings,
+        "pcie_inclusive": pcie,
         "realtime_factor": chunk_s / t_plain,
         "realtime_beams": {"plain": int(chunk_s / t_plain),
                            "batched": int(nb * chunk_s / t_batch), "batched_beams_run": nb},
@@ -451,6 +504,14 @@ def main():
                      "traffic": profiled_traffic("k_dedisp"),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                      "lds": lds_roof(ms_dd)},
+        "roofline_search": {"bound": "hbm", "kernels": "boxcar statistics, sigma0 pass, "
+                                                       "threshold count and compaction",
+                            "achieved": search_bytes / (ms_search * 1e-3) / 1e9,
+                            "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": search_bytes / (ms_search * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                            "algorithmic_bytes": search_bytes,
+                            "traffic": profiled_traffic("search_chunk"),
+                            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "roofline_flagger": {"bound": "hbm", "achieved": flag_bytes / (ms_flag * 1e-3) / 1e9,
                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": flag_bytes / (ms_flag * 1e-3) / 1e9 / PEAK_HBM_GBS,

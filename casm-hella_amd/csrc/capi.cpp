@@ -78,6 +78,7 @@ hella_status hella_create(const char *config, hella_ctx *out)
     HAMD_CHECK(hipGetDeviceCount(&n));
     if (cfg.gpu < 0 || cfg.gpu >= n) throw std::runtime_error("GPU index out of range");
     HAMD_CHECK(hipSetDevice(cfg.gpu));
+    kernels_setup();
     HAMD_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   });
   if (st != HELLA_OK) {
@@ -319,6 +320,22 @@ hella_status hella_copy_dedisp(hella_ctx c, int beam, void *host, size_t bytes)
     HAMD_CHECK(hipStreamSynchronize(c->stream));
     HAMD_CHECK(hipMemcpy2D(host, (size_t)B.ntime_dd * 4, src, (size_t)B.dstride * 4,
                            (size_t)B.ntime_dd * 4, (size_t)B.ndm, hipMemcpyDeviceToHost));
+  });
+}
+
+hella_status hella_set_dedisp(hella_ctx c, int beam, const void *host, size_t bytes)
+{
+  if (!valid_beam(c, beam) || !host) return fail(HELLA_ERR_INVALID, "bad argument");
+  return guard([&] {
+    ctx_build(c);
+    BeamState &B = c->beams[(size_t)beam];
+    const size_t need = (size_t)B.ndm * B.ntime_dd * 4;
+    if (bytes < need) throw std::invalid_argument("buffer smaller than ndm x ntime_dd u32");
+    c->batch_mode = false;
+    HAMD_CHECK(hipStreamSynchronize(c->stream));
+    HAMD_CHECK(hipMemcpy2D(c->ws[0].D.p, (size_t)B.dstride * 4, host, (size_t)B.ntime_dd * 4,
+                           (size_t)B.ntime_dd * 4, (size_t)B.ndm, hipMemcpyHostToDevice));
+    c->cur_beam = beam;
   });
 }
 

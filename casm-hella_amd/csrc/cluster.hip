@@ -268,17 +268,18 @@ __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict_
 
 }  // namespace
 
+void cluster_kernel_setup()
+{
+  HAMD_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_cluster),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)CLUSTER_LDS));
+}
+
 void SearchRunner::cluster(const SearchBeamDev *d_beams, int nbeam, const ClusterArgs &A,
                            hipStream_t s)
 {
   if (nbeam <= 0) return;
+  // the 148 KiB LDS limit was raised by kernels_setup() at set-up
   const size_t lds = CLUSTER_LDS;
-  static const bool attr = [&] {
-    HAMD_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(k_cluster),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    return true;
-  }();
-  (void)attr;
   hipLaunchKernelGGL(k_cluster, dim3((unsigned)nbeam), dim3(CT), lds, s, d_beams, A);
   HAMD_CHECK(hipGetLastError());
 }

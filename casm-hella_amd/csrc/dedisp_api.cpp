@@ -71,6 +71,9 @@ namespace {
 dedisp_error refresh_device(dedisp_plan plan)
 {
   if (!plan->dirty) return DEDISP_NO_ERROR;
+  // the tile table depends on the tile shape chosen below (DM block, time
+  // step), which a new DM list or killmask may change: rebuild it
+  plan->tiles.clear();
   plan->h.delays_live(0, plan->h.dm_list.size(), plan->delta_host, plan->live_host);
   plan->nlive = (int)plan->live_host.size();
   const int ndm = (int)plan->h.dm_list.size();
@@ -226,6 +229,7 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps, con
   if (nout > (size_t)INT32_MAX || dm_count > (size_t)INT32_MAX) return DEDISP_TOO_FEW_NSAMPS;
   const bool dev = (flags & DEDISP_DEVICE_POINTERS) != 0;
   try {
+    kernels_setup();  // first call on a device: raised LDS limits (not a stream operation)
     refresh_device(plan);
     hipStream_t s = plan->stream;
     const uint8_t *d_in = in;

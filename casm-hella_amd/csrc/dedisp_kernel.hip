@@ -19,7 +19,9 @@
 // two sample additions.  u16 lanes hold at most 257*255, so they are widened
 // into u32 totals every 256 channels.  V/4 = 7 is odd, which spreads the
 // threads of one DM over all 32 LDS banks.
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -761,10 +763,9 @@ void launch_w(const DDBeamDev *d_beams, const DDTileDesc *d_tiles, int ntiles, i
 {
   const size_t lds = (size_t)2 * NDW * lrow;
   const int nu = (lrow + 1023) / 1024;
+  // the LDS limit above 64 KiB was raised by kernels_setup() at set-up, so
+  // nothing here is a non-stream runtime call (launches may be captured)
   auto go = [&](auto kern) {
-    if (lds > 64 * 1024)
-      HAMD_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(64 * NDW), lds, s, d_beams, d_tiles,
                        lrow);
   };
@@ -1024,7 +1025,46 @@ __global__ __launch_bounds__(256) void k_transpose_tf_ft(const uint8_t *__restri
   }
 }
 
+template <int NDW, bool PIPE>
+void raise_lds_limit_w(int bytes)
+{
+  const void *k[] = {reinterpret_cast<const void *>(k_dedisp_w<NDW, 2, PIPE>),
+                     reinterpret_cast<const void *>(k_dedisp_w<NDW, 3, PIPE>),
+                     reinterpret_cast<const void *>(k_dedisp_w<NDW, 4, PIPE>),
+                     reinterpret_cast<const void *>(k_dedisp_w<NDW, 5, PIPE>),
+                     reinterpret_cast<const void *>(k_dedisp_w<NDW, 6, PIPE>)};
+  for (const void *f : k)
+    HAMD_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
 }  // namespace
+
+void dd_kernel_setup()
+{
+  // every one-trial-per-wave instance may take the device's whole
+  // per-workgroup LDS (launches above 64 KiB need the raised limit)
+  const int bytes = (int)dd_lds_budget(DDTile{64, 4, 3});
+  raise_lds_limit_w<4, false>(bytes);
+  raise_lds_limit_w<8, false>(bytes);
+  raise_lds_limit_w<4, true>(bytes);
+  raise_lds_limit_w<8, true>(bytes);
+}
+
+void kernels_setup()
+{
+  // once per device: kernel attributes are per device, and setting them is
+  // not a stream operation, so it must not happen inside a stream capture
+  static std::mutex mu;
+  static std::vector<char> done;
+  int dev = 0;
+  HAMD_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((size_t)dev < done.size() && done[(size_t)dev]) return;
+  dd_kernel_setup();
+  cluster_kernel_setup();
+  if (done.size() <= (size_t)dev) done.resize((size_t)dev + 1, 0);
+  done[(size_t)dev] = 1;
+}
 
 int dd_samples_per_thread() { return V; }
 

@@ -177,6 +177,12 @@ void transpose_tf_to_ft(const uint8_t *in, int64_t nsamps, int64_t in_stride,
                         hipStream_t s);
 
 // zero rows [rows][n] of 32-bit words with a kernel (graph-capturable)
+// Per-device kernel set-up (raised LDS limits of the kernels that take more
+// than 64 KiB).  Called once per device by every entry point that creates a
+// context, server or plan launch, before any stream capture.
+void kernels_setup();
+void dd_kernel_setup();
+void cluster_kernel_setup();
 void fill_zero_rows(void *out, int64_t stride, int n, int rows, hipStream_t s);
 
 DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,

@@ -700,6 +700,7 @@ hella_status hella_server_create(const char *config, const hella_server_opts *op
     if (!probe.parse(config ? config : "", err)) throw std::runtime_error(err);
     if (probe.gpu < 0 || probe.gpu >= n) throw std::runtime_error("GPU index out of range");
     HAMD_CHECK(hipSetDevice(probe.gpu));
+    kernels_setup();
     setup_geometry(s, config);
     footprint(s->geom, s->dims, s->chunk_bytes, s->opts, s->beam_bytes, s->work_bytes);
     if (s->opts.hbm_budget == 0) {

@@ -260,6 +260,7 @@ def _declare_hella(L):
     d("hella_last_crossings", C.c_longlong, P, _I)
     d("hella_copy_window", _I, P, _I, P, C.c_size_t)
     d("hella_copy_dedisp", _I, P, _I, P, C.c_size_t)
+    d("hella_set_dedisp", _I, P, _I, P, C.c_size_t)
     d("hella_get_search_stats", _I, P, _I, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F),
       C.POINTER(_I))
     d("hella_stage_times", _I, P, C.POINTER(_F))
@@ -299,6 +300,8 @@ def _declare_hella(L):
     d("hella_format_coincidencer", _I, C.c_longlong, C.c_longlong, C.c_char_p, C.POINTER(_Cand),
       _I, _I, C.c_char_p, _I)
     d("hella_write_specflags", _I, C.c_char_p, C.POINTER(C.c_uint), _I)
+    d("hella_dada_header_value", _I, C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, _I)
+    d("hella_dada_coincidencer_fields", _I, C.c_char_p, C.c_size_t, C.c_char_p, _I)
     d("hella_sigproc_read_header", _I, C.c_char_p, C.POINTER(FilHeader))
     d("hella_sigproc_read", C.c_longlong, C.c_char_p, C.POINTER(FilHeader), C.c_longlong,
       C.c_longlong, P)
@@ -408,6 +411,11 @@ class Hella:
 
     @staticmethod
     def _ptr(gulp):
+        """(pointer, host flag, keep-alive) of a gulp: a DeviceBuffer, or a host
+        array in the configured INPUT format -- uint8 [gulp][nchan] (FILTERBANK,
+        FILE), float32 [gulp][nchan] (INPUT_DTYPE F32) or float16 [nchan][gulp]
+        (DADA: the fp16 channel-major blocks of the PSRDADA ring), beams
+        concatenated"""
         if isinstance(gulp, DeviceBuffer):
             return gulp.ptr, 0
         a = np.ascontiguousarray(gulp)
@@ -476,6 +484,12 @@ class Hella:
         out = np.zeros((i["ndm"], i["ntime_dd"]), np.uint32)
         _hchk(lib().hella_copy_dedisp(self._c, beam, out.ctypes.data, out.nbytes))
         return out
+
+    def set_dedisp(self, beam, D):
+        """Load a dedispersed plane (u32 [ndm][ntime_dd]) as the beam's current
+        plane; smooth() and find_peaks() then search it."""
+        a = np.ascontiguousarray(D, np.uint32)
+        _hchk(lib().hella_set_dedisp(self._c, beam, a.ctypes.data, a.nbytes))
 
     def search_stats(self, beam=0):
         nw = self.info(beam)["nwidth"]
@@ -707,6 +721,33 @@ def format_coincidencer(samp0, gulp, cands, beam0=0, dada_fields=None):
         raise HellaError("hella_format_coincidencer: bad argument")
     buf = C.create_string_buffer(n + 1)
     lib().hella_format_coincidencer(samp0, gulp, df, ptr, int(c.size), beam0, buf, n + 1)
+    return buf.value.decode()
+
+
+def _header_bytes(header):
+    return header if isinstance(header, bytes) else header.encode()
+
+
+def dada_header_value(header, key):
+    """The value of key in a PSRDADA header text (None if absent)."""
+    h = _header_bytes(header)
+    n = lib().hella_dada_header_value(h, len(h), key.encode(), None, 0)
+    if n < 0:
+        return None
+    buf = C.create_string_buffer(n + 1)
+    lib().hella_dada_header_value(h, len(h), key.encode(), buf, n + 1)
+    return buf.value.decode()
+
+
+def dada_coincidencer_fields(header):
+    """"UTC_START PICOSECONDS TSAMP" of a PSRDADA header (the coincidencer's
+    first-line fields); raises if one is missing."""
+    h = _header_bytes(header)
+    n = lib().hella_dada_coincidencer_fields(h, len(h), None, 0)
+    if n < 0:
+        raise HellaError(lib().hella_last_error().decode())
+    buf = C.create_string_buffer(n + 1)
+    lib().hella_dada_coincidencer_fields(h, len(h), buf, n + 1)
     return buf.value.decode()
 
 

@@ -139,6 +139,11 @@ long long hella_last_crossings(hella_ctx ctx, int beam);
 hella_status hella_copy_window(hella_ctx ctx, int beam, void *host, size_t bytes);
 /* copy the last dedispersed plane of a beam ([ndm][ntime_dd] u32) */
 hella_status hella_copy_dedisp(hella_ctx ctx, int beam, void *host, size_t bytes);
+/* load a dedispersed plane ([ndm][ntime_dd] u32, host memory) as the beam's
+ * current plane, in place of hella_dedisperse: hella_smooth and
+ * hella_find_peaks then search it (the reference's smooth/find_peaks read
+ * whatever p->d_dedisp holds, smooth.cpp:21-127, peaks.cu:31-117) */
+hella_status hella_set_dedisp(hella_ctx ctx, int beam, const void *host, size_t bytes);
 /* per-width mean/sigma (nwidth each), sigma0, gate flag of the beam's last search */
 hella_status hella_get_search_stats(hella_ctx ctx, int beam, float *mu, float *sigma,
                                     float *sigma0, int *gated);
@@ -221,6 +226,21 @@ hella_status hella_write_candidates(const char *path, const hella_candidate *can
  * connection per gulp in the reference) is left to the caller. */
 int hella_format_coincidencer(long long samp0, long long gulp, const char *dada_fields,
                               const hella_candidate *cands, int n, int beam0, char *buf, int cap);
+/* ---- PSRDADA header (src/hella/src/dada.cpp:50-79) ----------------------
+ * The header text as key/value pairs, parsed as parse_dada_header does: the
+ * text ends at the first NUL (or after len bytes), one "KEY VALUE" per line,
+ * the key up to the first space, the value from the first non-space after it
+ * to the end of the line, empty lines skipped, a line without a space ends
+ * the parse, a repeated key keeps its last value.
+ * hella_dada_header_value: the value of key (writes at most cap - 1 bytes and
+ * a NUL; returns its full length as snprintf does), -1 if the key is absent. */
+int hella_dada_header_value(const char *header, size_t len, const char *key, char *out, int cap);
+/* "UTC_START PICOSECONDS TSAMP" (the three values joined by spaces), the
+ * dada_fields argument of hella_format_coincidencer for DADA input
+ * (peaks.cu:215-220); -1 when one of them is missing (the reference's
+ * map::at throws). */
+int hella_dada_coincidencer_fields(const char *header, size_t len, char *out, int cap);
+
 /* Append one "%u" line per channel (the spectral-flag file of
  * casm:756-761). */
 hella_status hella_write_specflags(const char *path, const unsigned *counts, int nchan);

@@ -160,21 +160,29 @@ void orc_stats(const float *x, size_t h, size_t w, size_t stride,
                double *mu, double *var)
 {
   size_t wp = stats_width(w);
-  double *t1 = (double *)malloc((wp ? wp : 1) * sizeof(double));
-  double *t2 = (double *)malloc((wp ? wp : 1) * sizeof(double));
   double *S1 = (double *)calloc(h ? h : 1, sizeof(double));
   double *S2 = (double *)calloc(h ? h : 1, sizeof(double));
-  for (size_t r = 0; r < h; ++r) {
-    for (size_t j = 0; j < wp; ++j) {
-      double v = (double)x[r * stride + j];
-      t1[j] = v;
-      t2[j] = v * v;
+  /* rows are independent (each row's sums keep their canonical order);
+   * threads only change which core computes a row */
+#pragma omp parallel
+  {
+    double *t1 = (double *)malloc((wp ? wp : 1) * sizeof(double));
+    double *t2 = (double *)malloc((wp ? wp : 1) * sizeof(double));
+#pragma omp for schedule(static)
+    for (size_t r = 0; r < h; ++r) {
+      for (size_t j = 0; j < wp; ++j) {
+        double v = (double)x[r * stride + j];
+        t1[j] = v;
+        t2[j] = v * v;
+      }
+      S1[r] = orc_csum(t1, wp);
+      S2[r] = orc_csum(t2, wp);
     }
-    S1[r] = orc_csum(t1, wp);
-    S2[r] = orc_csum(t2, wp);
+    free(t1);
+    free(t2);
   }
   orc_stats_rows(S1, S2, h, (double)wp, mu, var);
-  free(t1); free(t2); free(S1); free(S2);
+  free(S1); free(S2);
 }
 
 void orc_stats_f(const float *x, size_t h, size_t w, size_t stride,
@@ -193,17 +201,22 @@ void orc_stats_f(const float *x, size_t h, size_t w, size_t stride,
 void orc_bandpass(const float *x, int nch, int W, float *bp)
 {
   size_t n = (size_t)(W / 32) * 32;
-  double *tmp = (double *)malloc((n ? n : 1) * sizeof(double));
-  for (int c = 0; c < nch; ++c) {
-    for (size_t j = 0; j < n; ++j) tmp[j] = (double)x[(size_t)c * W + j];
-    bp[c] = (float)(orc_csum(tmp, n) / (double)W);
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc((n ? n : 1) * sizeof(double));
+#pragma omp for schedule(static)
+    for (int c = 0; c < nch; ++c) {
+      for (size_t j = 0; j < n; ++j) tmp[j] = (double)x[(size_t)c * W + j];
+      bp[c] = (float)(orc_csum(tmp, n) / (double)W);
+    }
+    free(tmp);
   }
-  free(tmp);
 }
 
 /* divide_by_bp (src/flagger.cu:180-194) */
 void orc_divide_bp(float *x, int nch, int W, const float *bp)
 {
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < nch; ++c) {
     float b = bp[c];
     float *row = x + (size_t)c * W;
@@ -219,6 +232,7 @@ void orc_normalize(float *x, int nch, int W)
   float sf = (float)sqrt(v);
   double inv = 1.0 / (double)sf;
   size_t n = (size_t)nch * W;
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) x[i] = (float)(((double)x[i] - m) * inv);
 }
 
@@ -227,6 +241,7 @@ void orc_spec_flag(float *x, int nch, int W, float smin, float smax)
 {
   float *bp = (float *)malloc(sizeof(float) * (size_t)nch);
   orc_bandpass(x, nch, W, bp);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < nch; ++c) {
     float *row = x + (size_t)c * W;
     int bad = bp[c] < smin || bp[c] > smax;
@@ -243,6 +258,7 @@ void orc_box_mean(const float *x, int nch, int W, int ts, int fs, float *out)
 {
   float k = (float)(1. / ts / fs);
   int ax = ts / 2, ay = fs / 2;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < nch; ++c)
     for (int t = 0; t < W; ++t) {
       float acc = 0.f;
@@ -268,8 +284,10 @@ void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
   orc_box_mean(x, nch, W, ts, fs, S);
   float thr = (float)((double)thresh / sqrt(1. * ts * fs));
   float nthr = (float)(-1. * (double)thr);
+#pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) M[i] = (S[i] > thr) || (S[i] < nthr);
   int ax = ts / 2, ay = fs / 2;
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < nch; ++c) {
     int fixed = c >= flag1 && c < flag2;
     uint32_t cnt = 0;
@@ -310,6 +328,8 @@ double orc_chan_sum(const float *x, int nch, int W, int t)
 /* ts_correct (src/timeseries.cu:19-74) */
 void orc_ts_correct(float *x, int nch, int W, float *ts)
 {
+  /* samples are independent: each column's sum keeps its order */
+#pragma omp parallel for schedule(static, 256)
   for (int t = 0; t < W; ++t) {
     const double s = orc_chan_sum(x, nch, W, t);
     float tv = (float)(s / (double)nch);
@@ -334,6 +354,7 @@ static inline uint8_t quant(float x)
 
 void orc_quantize(const float *x, int nch, int W, uint8_t *out, size_t out_stride)
 {
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < nch; ++c)
     for (int t = 0; t < W; ++t) out[(size_t)c * out_stride + t] = quant(x[(size_t)c * W + t]);
 }
@@ -341,14 +362,16 @@ void orc_quantize(const float *x, int nch, int W, uint8_t *out, size_t out_strid
 /* transpose_input (src/transpose.cu:23-42) */
 void orc_ingest_u8_tf(const uint8_t *in, int nch, int W, float *x)
 {
-  for (int t = 0; t < W; ++t)
-    for (int c = 0; c < nch; ++c) x[(size_t)c * W + t] = (float)in[(size_t)t * nch + c];
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nch; ++c)
+    for (int t = 0; t < W; ++t) x[(size_t)c * W + t] = (float)in[(size_t)t * nch + c];
 }
 
 void orc_ingest_f32_tf(const float *in, int nch, int W, float *x)
 {
-  for (int t = 0; t < W; ++t)
-    for (int c = 0; c < nch; ++c) x[(size_t)c * W + t] = in[(size_t)t * nch + c];
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nch; ++c)
+    for (int t = 0; t < W; ++t) x[(size_t)c * W + t] = in[(size_t)t * nch + c];
 }
 
 static void bandpass_correct(float *x, int nch, int W)
@@ -504,6 +527,8 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
     {
       double *S1 = (double *)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
       double *S2 = (double *)malloc(sizeof(double) * (size_t)(rows > 0 ? rows : 1));
+      /* rows are independent: threads only change which core sums a row */
+#pragma omp parallel for schedule(static)
       for (int r = 0; r < rows; ++r) {
         const double *row = rawplane + (size_t)r * nout;
         if (cfg->mode == ORC_MODE_BOXCAR) {
@@ -519,9 +544,11 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
           S1[r] = (double)s1;
           S2[r] = (double)(uint64_t)(s2 >> 64) * 18446744073709551616.0 + (double)(uint64_t)s2;
         } else {
-          for (size_t j = 0; j < wp; ++j) tmp[j] = row[j] * row[j];
+          double *sq = (double *)malloc(sizeof(double) * (wp ? wp : 1));
+          for (size_t j = 0; j < wp; ++j) sq[j] = row[j] * row[j];
           S1[r] = orc_csum(row, wp);
-          S2[r] = orc_csum(tmp, wp);
+          S2[r] = orc_csum(sq, wp);
+          free(sq);
         }
       }
       orc_stats_rows(S1, S2, (size_t)rows, (double)wp, &m, &v);
@@ -534,6 +561,7 @@ long orc_search(const orc_search_cfg *cfg, const uint32_t *D, size_t d_stride,
     float *Yk = Y + (size_t)k * plane;
     if (sf > 0.f) {
       double inv = 1.0 / (double)sf;
+#pragma omp parallel for schedule(static)
       for (size_t i = 0; i < plane; ++i) Yk[i] = (float)((rawplane[i] - m) * inv);
     } else {
       for (size_t i = 0; i < plane; ++i) Yk[i] = (float)rawplane[i];

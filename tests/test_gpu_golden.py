@@ -40,3 +40,35 @@ def test_flagger_matches_fixture(hamd):
     h.process_gulp(g["data"])
     np.testing.assert_array_equal(h.window(0)[:, :2048], g["out"])
     np.testing.assert_array_equal(h.flag_counts(0), g["counts"])
+
+
+@pytest.mark.parametrize("mode", ["BOXCAR", "HELLA"])
+def test_search_matches_fixture(hamd, mode):
+    """The search stages (hella_smooth, hella_find_peaks) on the fixture's
+    dedispersed plane (24 trials x 3016 samples, widths 1..16, S/N 6, at most
+    1000 candidates): statistics, sigma0 and the candidate list of the
+    fixture.  16 channels over 500 MHz down in 6.25 MHz steps, DM 0-5 give the
+    24 trials and a 177-sample max delay, so a 3193-sample FILE gulp searches
+    3000 samples."""
+    g = _load("search_small")
+    name = mode.lower()
+    cfg = dict(INPUT="FILE", NCHAN=16, FCH1=500.0, FOFF=-6.25, TSAMP=256e-6, GULP=3193,
+               DM_MIN=0, DM_MAX=5, DM_TOL=1.25, PULSE_WIDTH=64, WIDTH_MIN=1, WIDTH_MAX=16,
+               SNR=6, MAX_GIANTS=1000, SEARCH=mode)
+    h = hamd.Hella(cfg)
+    i = h.info(0)
+    assert (i["ndm"], i["ntime_dd"], i["ntime_out"]) == (24, 3016, 3000)
+    h.set_dedisp(0, g["D"])
+    h.smooth(1)
+    h.find_peaks(0)
+    st = h.search_stats(0)
+    np.testing.assert_array_equal(st["mu"], g[f"{name}_mu"])
+    np.testing.assert_array_equal(st["sigma"], g[f"{name}_sigma"])
+    assert st["sigma0"] == g[f"{name}_sigma0"]
+    p = h.peaks()
+    ref = g[f"{name}_cands"]
+    assert len(p) == len(ref) > 0
+    for f in ("samp", "width", "dm_idx"):
+        np.testing.assert_array_equal(p[f], ref[f])
+    np.testing.assert_allclose(p["snr"], ref["snr"], rtol=1e-5, atol=1e-5)
+    h.close()
