@@ -7,6 +7,8 @@
 // dedisperse.cpp:40-46); nothing here exits the process.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "../../include/dedisp.h"
 #include "internal.hpp"
@@ -17,13 +19,18 @@ struct dedisp_plan_struct {
   PlanHost h;
   hipStream_t stream = nullptr;
   std::vector<dedisp_bool> killmask_pub;
-  // device-side copies, rebuilt when the DM list or the killmask change
+  // host tables (delays of the live channels, tile shape, channel-major
+  // delays, entry tables), rebuilt by the DM-list and killmask setters, and
+  // their device copies, uploaded by the first execute after a change
+  bool host_ready = false;
   bool dirty = true;
   DevBuf d_delta, d_live, d_delta_t, d_ent;
   int dt_stride = 0;
   int ent_stride = 0, ent_nd = 0;  // entry tables of the one-trial-per-wave kernel
   int nlive = 0;
-  std::vector<int32_t> delta_host, live_host;
+  std::vector<int32_t> delta_host, live_host, dt_host;
+  std::vector<uint32_t> ent_host;
+  DDBeamDev na_desc{};  // descriptor of the last scratch-only execute (host copy)
   DDTile tile{16, 32, 1};
   int max_spread = 0;
   DevBuf scratch_in, scratch_ft, scratch_out;
@@ -68,39 +75,105 @@ DDTile choose_dd_tile(const int32_t *delta, int ndm, int nlive, int *spread_out,
 
 namespace {
 
-dedisp_error refresh_device(dedisp_plan plan)
+// Host tables of the current DM list and killmask (host memory only; the
+// setters call it, so an execute finds them ready).
+void prepare_host(dedisp_plan plan)
 {
-  if (!plan->dirty) return DEDISP_NO_ERROR;
-  // the tile table depends on the tile shape chosen below (DM block, time
+  // the tile table depends on the tile shape chosen here (DM block, time
   // step), which a new DM list or killmask may change: rebuild it
   plan->tiles.clear();
   plan->h.delays_live(0, plan->h.dm_list.size(), plan->delta_host, plan->live_host);
   plan->nlive = (int)plan->live_host.size();
   const int ndm = (int)plan->h.dm_list.size();
+  plan->dt_host.clear();
+  plan->ent_host.clear();
+  plan->ent_nd = 0;
   if (plan->nlive > 0) {
     plan->tile = choose_dd_tile(plan->delta_host.data(), ndm, plan->nlive, &plan->max_spread,
                                 16, DD_LDS_BUDGET);
+    delays_transposed(plan->delta_host, ndm, plan->nlive, plan->dt_host, plan->dt_stride);
+    if (plan->tile.dma >= 3) {
+      plan->ent_nd = dd_tile_ND(plan->tile);
+      dd_entries(plan->delta_host, ndm, plan->nlive, plan->ent_nd, plan->ent_host,
+                 plan->ent_stride);
+    }
+  }
+  plan->host_ready = true;
+  plan->dirty = true;
+}
+
+dedisp_error refresh_device(dedisp_plan plan)
+{
+  if (!plan->host_ready) prepare_host(plan);
+  if (!plan->dirty) return DEDISP_NO_ERROR;
+  if (plan->nlive > 0) {
     plan->d_delta.ensure((plan->delta_host.size() + DD_DELTA_PAD) * 4);
     plan->d_live.ensure(plan->live_host.size() * 4);
     HAMD_CHECK(hipMemcpy(plan->d_delta.p, plan->delta_host.data(), plan->delta_host.size() * 4,
                          hipMemcpyHostToDevice));
     HAMD_CHECK(hipMemcpy(plan->d_live.p, plan->live_host.data(), plan->live_host.size() * 4,
                          hipMemcpyHostToDevice));
-    std::vector<int32_t> dt;
-    delays_transposed(plan->delta_host, ndm, plan->nlive, dt, plan->dt_stride);
-    plan->d_delta_t.ensure(dt.size() * 4);
-    HAMD_CHECK(hipMemcpy(plan->d_delta_t.p, dt.data(), dt.size() * 4, hipMemcpyHostToDevice));
-    plan->ent_nd = 0;
-    if (plan->tile.dma >= 3) {
-      std::vector<uint32_t> e;
-      plan->ent_nd = dd_tile_ND(plan->tile);
-      dd_entries(plan->delta_host, ndm, plan->nlive, plan->ent_nd, e, plan->ent_stride);
-      plan->d_ent.ensure(e.size() * 4);
-      HAMD_CHECK(hipMemcpy(plan->d_ent.p, e.data(), e.size() * 4, hipMemcpyHostToDevice));
+    plan->d_delta_t.ensure(plan->dt_host.size() * 4);
+    HAMD_CHECK(hipMemcpy(plan->d_delta_t.p, plan->dt_host.data(), plan->dt_host.size() * 4,
+                         hipMemcpyHostToDevice));
+    if (!plan->ent_host.empty()) {
+      plan->d_ent.ensure(plan->ent_host.size() * 4);
+      HAMD_CHECK(hipMemcpy(plan->d_ent.p, plan->ent_host.data(), plan->ent_host.size() * 4,
+                           hipMemcpyHostToDevice));
     }
   }
   plan->dirty = false;
   return DEDISP_NO_ERROR;
+}
+
+// ---- scratch of dedisp_execute_adv_noalloc ----------------------------------
+// A fixed device array (each GPU the library runs on holds its own copy until
+// the process ends): the call's descriptor, tile table, delay tables and the
+// channel-major copy of the input live here, so the call allocates nothing.
+// In host code the array's name is only a stand-in; the device address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64u << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+uint8_t *noalloc_scratch()
+{
+  constexpr int MAXDEV = 64;
+  static std::mutex mu;
+  static uint8_t *ptr[MAXDEV] = {};
+  int dev = 0;
+  HAMD_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAXDEV) throw HipError{hipErrorInvalidDevice, "device index", __LINE__};
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    HAMD_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = static_cast<uint8_t *>(p);
+  }
+  return ptr[dev];
+}
+
+// The tile table of dd_build_tiles for one beam in DM-block-major order,
+// written on the device: list entry k is (d0, t0) = ((k / nt) * ND,
+// (k % nt) * T), and slot i of the XCD-interleaved table holds entry
+// k = (i % 8) * q + i / 8 (q = slots / 8); slots past the list are empty.
+__global__ void k_fill_tiles(DDTileDesc *tiles, int slots, int n, int nt, int T, int ND)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= slots) return;
+  const int q = slots / 8;
+  const int k = (i % 8) * q + i / 8;
+  tiles[i] = k < n ? DDTileDesc{0, (k / nt) * ND, (k % nt) * T, 0} : DDTileDesc{-1, 0, 0, 0};
+}
+
+void fill_tiles(DDTileDesc *tiles, int slots, int n, int nt, int nd, int T, int ND,
+                hipStream_t s)
+{
+  (void)nd;
+  hipLaunchKernelGGL(k_fill_tiles, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, tiles,
+                     slots, n, nt, T, ND);
+  HAMD_CHECK(hipGetLastError());
 }
 
 dedisp_error gpu_error(const HipError &e)
@@ -142,7 +215,12 @@ dedisp_error dedisp_generate_dm_list(dedisp_plan plan, dedisp_float dm_start, de
   if (!plan) return DEDISP_INVALID_PLAN;
   if (!(dm_end >= dm_start) || !(tol > 1.f) || !(ti >= 0.f)) return DEDISP_UNKNOWN_ERROR;
   plan->h.generate_dm_list(dm_start, dm_end, (double)ti, (double)tol);
-  plan->dirty = true;
+  try {
+    prepare_host(plan);
+  } catch (...) {
+    plan->host_ready = false;
+    return DEDISP_MEM_ALLOC_FAILED;
+  }
   return DEDISP_NO_ERROR;
 }
 
@@ -153,7 +231,12 @@ dedisp_error dedisp_set_dm_list(dedisp_plan plan, const dedisp_float *dm_list, d
   for (dedisp_size i = 0; i < count; ++i)
     if (!(dm_list[i] >= 0.f) || (i && dm_list[i] < dm_list[i - 1])) return DEDISP_UNKNOWN_ERROR;
   plan->h.set_dm_list(dm_list, count);
-  plan->dirty = true;
+  try {
+    prepare_host(plan);
+  } catch (...) {
+    plan->host_ready = false;
+    return DEDISP_MEM_ALLOC_FAILED;
+  }
   return DEDISP_NO_ERROR;
 }
 
@@ -165,7 +248,14 @@ dedisp_error dedisp_set_killmask(dedisp_plan plan, const dedisp_bool *killmask)
     plan->h.killmask[c] = v;
     plan->killmask_pub[c] = v;
   }
+  plan->host_ready = false;
   plan->dirty = true;
+  if (plan->h.dm_list.empty()) return DEDISP_NO_ERROR;  // tables follow the DM list
+  try {
+    prepare_host(plan);
+  } catch (...) {
+    return DEDISP_MEM_ALLOC_FAILED;
+  }
   return DEDISP_NO_ERROR;
 }
 
@@ -319,6 +409,109 @@ dedisp_error dedisp_execute(const dedisp_plan plan, dedisp_size nsamps, const de
   const size_t in_stride = plan->h.nchans * in_nbits / 8;
   const size_t out_stride = (nsamps - plan->h.max_delay) * out_nbits / 8;
   return dedisp_execute_adv(plan, nsamps, in, in_nbits, in_stride, out, out_nbits, out_stride, flags);
+}
+
+dedisp_error dedisp_execute_adv_noalloc(const dedisp_plan plan, dedisp_size nsamps,
+                                        const dedisp_byte *in, dedisp_size in_nbits,
+                                        dedisp_size in_stride, dedisp_byte *out,
+                                        dedisp_size out_nbits, dedisp_size out_stride,
+                                        unsigned flags)
+{
+  if (!plan) return DEDISP_INVALID_PLAN;
+  if (plan->h.dm_list.empty()) return DEDISP_NO_DM_LIST_SET;
+  if (!in || !out) return DEDISP_INVALID_POINTER;
+  // device pointers only: host buffers would need device staging memory
+  if (!(flags & DEDISP_DEVICE_POINTERS) || (flags & DEDISP_HOST_POINTERS))
+    return DEDISP_INVALID_FLAG_COMBINATION;
+  if ((flags & DEDISP_WAIT) && (flags & DEDISP_ASYNC)) return DEDISP_INVALID_FLAG_COMBINATION;
+  if (in_nbits != 8) return DEDISP_UNSUPPORTED_IN_NBITS;
+  if (out_nbits != 32) return DEDISP_UNSUPPORTED_OUT_NBITS;
+  const size_t nch = plan->h.nchans;
+  const size_t md = plan->h.max_delay;
+  if (nsamps <= md) return DEDISP_TOO_FEW_NSAMPS;
+  const size_t nout = nsamps - md;
+  const size_t ndm = plan->h.dm_list.size();
+  if (in_stride < nch) return DEDISP_INVALID_STRIDE;
+  if (out_stride < nout * 4 || out_stride % 4) return DEDISP_INVALID_STRIDE;
+  if (nout > (size_t)INT32_MAX || ndm > (size_t)INT32_MAX) return DEDISP_TOO_FEW_NSAMPS;
+  // the setters leave the host tables ready; a plan whose last setter failed
+  // would need them rebuilt (host memory): refuse instead
+  if (!plan->host_ready) {
+    set_last_error("dedisp_execute_adv_noalloc: the plan's tables are not built "
+                   "(set the DM list again)");
+    return DEDISP_UNKNOWN_ERROR;
+  }
+  try {
+    hipStream_t s = plan->stream;
+    if (plan->nlive == 0) {
+      fill_zero_rows(out, (int64_t)(out_stride / 4), (int)nout, (int)ndm, s);
+    } else {
+      const DDTile t = plan->tile;
+      const size_t T = (size_t)dd_tile_T(t), ND = (size_t)dd_tile_ND(t);
+      const size_t nt = (nout + T - 1) / T, nd = (ndm + ND - 1) / ND;
+      const size_t ring = round_up(nt * T + md + (size_t)plan->max_spread + 64, 64);
+      // scratch layout (256-byte aligned parts)
+      const size_t n_tiles = (nt * nd + 7) / 8 * 8;
+      const size_t o_tiles = 256;
+      const size_t o_delta = round_up(o_tiles + n_tiles * sizeof(DDTileDesc), 256);
+      const size_t o_live = round_up(o_delta + (plan->delta_host.size() + DD_DELTA_PAD) * 4, 256);
+      const size_t o_dt = round_up(o_live + plan->live_host.size() * 4, 256);
+      const size_t o_ent = round_up(o_dt + plan->dt_host.size() * 4, 256);
+      const size_t o_win = round_up(o_ent + plan->ent_host.size() * 4, 256);
+      const size_t need = o_win + nch * ring;
+      if (need > noalloc_scratch_bytes()) {
+        set_last_error("dedisp_execute_adv_noalloc: needs " + std::to_string(need) +
+                       " bytes of scratch, the limit is " +
+                       std::to_string(noalloc_scratch_bytes()));
+        return DEDISP_MEM_ALLOC_FAILED;
+      }
+      kernels_setup();  // first call on a device: raised LDS limits (no allocation)
+      uint8_t *sc = noalloc_scratch();
+      auto up = [&](size_t off, const void *src, size_t bytes) {
+        if (bytes) HAMD_CHECK(hipMemcpyAsync(sc + off, src, bytes, hipMemcpyHostToDevice, s));
+      };
+      // every call rewrites what it reads: another call (another plan) on
+      // this device may have left its own tables in the shared scratch
+      up(o_delta, plan->delta_host.data(), plan->delta_host.size() * 4);
+      up(o_live, plan->live_host.data(), plan->live_host.size() * 4);
+      up(o_dt, plan->dt_host.data(), plan->dt_host.size() * 4);
+      up(o_ent, plan->ent_host.data(), plan->ent_host.size() * 4);
+      transpose_tf_to_ft(in, (int64_t)nsamps, (int64_t)in_stride, (int)nch, sc + o_win,
+                         (int64_t)ring, (int64_t)ring, s);
+      fill_tiles(reinterpret_cast<DDTileDesc *>(sc + o_tiles), (int)n_tiles, (int)(nt * nd),
+                 (int)nt, (int)nd, (int)T, (int)ND, s);
+      DDBeamDev &B = plan->na_desc;
+      B = DDBeamDev{};
+      B.base = sc + o_win;
+      B.row_stride = (int64_t)ring;
+      B.ring = (int64_t)ring;
+      B.start = 0;
+      B.delta = reinterpret_cast<const int32_t *>(sc + o_delta);
+      B.live = reinterpret_cast<const int32_t *>(sc + o_live);
+      B.delta_t = reinterpret_cast<const int32_t *>(sc + o_dt);
+      B.dt_stride = plan->dt_stride;
+      if (plan->ent_nd > 0) {
+        B.ent = reinterpret_cast<const uint32_t *>(sc + o_ent);
+        B.ent_stride = plan->ent_stride;
+        B.ent_nd = plan->ent_nd;
+      }
+      B.nlive = plan->nlive;
+      B.ndm = (int)ndm;
+      B.nout = (int)nout;
+      B.out_f32 = 1;
+      B.out = out;
+      B.out_stride = (int64_t)(out_stride / 4);
+      up(0, &B, sizeof(DDBeamDev));
+      dd_launch(reinterpret_cast<const DDBeamDev *>(sc), reinterpret_cast<DDTileDesc *>(sc + o_tiles),
+                (int)n_tiles, t, plan->max_spread, s, (int64_t)ring);
+    }
+    if (!(flags & DEDISP_ASYNC)) HAMD_CHECK(hipStreamSynchronize(s));
+  } catch (const HipError &e) {
+    return gpu_error(e);
+  } catch (...) {
+    return DEDISP_UNKNOWN_ERROR;
+  }
+  return DEDISP_NO_ERROR;
 }
 
 dedisp_error dedisp_set_device(int device_idx)

@@ -1054,16 +1054,17 @@ void kernels_setup()
 {
   // once per device: kernel attributes are per device, and setting them is
   // not a stream operation, so it must not happen inside a stream capture
+  // (a fixed table: after the first call on a device nothing is allocated)
+  constexpr int MAXDEV = 64;
   static std::mutex mu;
-  static std::vector<char> done;
+  static bool done[MAXDEV] = {};
   int dev = 0;
   HAMD_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(mu);
-  if ((size_t)dev < done.size() && done[(size_t)dev]) return;
+  if (dev >= 0 && dev < MAXDEV && done[dev]) return;
   dd_kernel_setup();
   cluster_kernel_setup();
-  if (done.size() <= (size_t)dev) done.resize((size_t)dev + 1, 0);
-  done[(size_t)dev] = 1;
+  if (dev >= 0 && dev < MAXDEV) done[dev] = true;
 }
 
 int dd_samples_per_thread() { return V; }

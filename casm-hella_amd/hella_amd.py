@@ -67,6 +67,7 @@ def _declare(L):
     d("dedisp_execute_adv", _I, _VP, _SZ, _VP, _SZ, _SZ, _VP, _SZ, _SZ, C.c_uint)
     d("dedisp_execute", _I, _VP, _SZ, _VP, _SZ, _VP, _SZ, C.c_uint)
     d("dedisp_execute_guru", _I, _VP, _SZ, _VP, _SZ, _SZ, _VP, _SZ, _SZ, _SZ, _SZ, C.c_uint)
+    d("dedisp_execute_adv_noalloc", _I, _VP, _SZ, _VP, _SZ, _SZ, _VP, _SZ, _SZ, C.c_uint)
     d("dedisp_set_device", _I, _I)
     d("dedisp_sync", _I)
     d("dedisp_get_error_string", C.c_char_p, _I)
@@ -142,6 +143,14 @@ class DedispPlan:
         _chk(lib().dedisp_execute_adv(self._p, nsamps, data_tf.ctypes.data, 8, in_stride,
                                       out.ctypes.data, 32, out.strides[0], flags))
         return out
+
+    def execute_device(self, d_in, nsamps, in_stride, d_out, out_stride, noalloc=False,
+                       flags=DEDISP_DEVICE_POINTERS):
+        """Device-pointer execute (the call hella makes): d_in and d_out are
+        device addresses (int); noalloc selects dedisp_execute_adv_noalloc.
+        Returns the dedisp_error code."""
+        f = lib().dedisp_execute_adv_noalloc if noalloc else lib().dedisp_execute_adv
+        return int(f(self._p, nsamps, d_in, 8, in_stride, d_out, 32, out_stride, flags))
 
     def execute_guru(self, data_tf, first_dm, dm_count, flags=0):
         data_tf = np.ascontiguousarray(data_tf, dtype=np.uint8)

@@ -109,6 +109,24 @@ dedisp_error dedisp_execute_guru(const dedisp_plan plan, dedisp_size nsamps,
                                  dedisp_size first_dm_idx, dedisp_size dm_count,
                                  unsigned flags);
 
+/* No counterpart in the reference or in dedisp: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  dedisp_execute_adv with device pointers only
+ * (DEDISP_DEVICE_POINTERS), allocating nothing: its scratch (descriptor, tile
+ * table, the plan's delay tables and the channel-major copy of the input) is
+ * one fixed 64 MiB device array per GPU, held until the process ends.  The
+ * plan's host tables are built by the DM-list and killmask setters.
+ * Requests needing more scratch than 64 MiB (about nchans * (nsamps + 2048)
+ * bytes plus the delay tables) return DEDISP_MEM_ALLOC_FAILED with the reason
+ * in hella_last_error().  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other threads, DEDISP_ASYNC calls
+ * still running) race and must be serialised by the caller. */
+dedisp_error dedisp_execute_adv_noalloc(const dedisp_plan plan, dedisp_size nsamps,
+                                        const dedisp_byte *in, dedisp_size in_nbits,
+                                        dedisp_size in_stride, dedisp_byte *out,
+                                        dedisp_size out_nbits, dedisp_size out_stride,
+                                        unsigned flags);
+
 dedisp_error dedisp_set_device(int device_idx);
 /* waits for every dedisp launch on the calling thread's plans */
 dedisp_error dedisp_sync(void);
