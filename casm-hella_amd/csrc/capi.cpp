@@ -301,9 +301,13 @@ hella_status hella_copy_window(hella_ctx c, int beam, void *host, size_t bytes)
     const long long g = c->gulp_index;  // gulps ingested
     const int64_t start = c->cfg.oneshot ? 0 : (int64_t)((g * (long long)c->W) % B.ntime);
     uint8_t *o = static_cast<uint8_t *>(host);
-    for (int ch = 0; ch < B.nch; ++ch)
-      for (int64_t x = 0; x < B.ntime; ++x)
-        o[(size_t)ch * B.ntime + x] = raw[(size_t)ch * B.ring_stride + (start + x) % B.ntime];
+    const size_t first = (size_t)(B.ntime - start);  // columns start .. ntime-1, then 0 ..
+    for (int ch = 0; ch < B.nch; ++ch) {
+      const uint8_t *r = raw.data() + (size_t)ch * B.ring_stride;
+      uint8_t *d = o + (size_t)ch * B.ntime;
+      std::memcpy(d, r + start, first);
+      std::memcpy(d + first, r, (size_t)start);
+    }
   });
 }
 
@@ -354,6 +358,22 @@ hella_status hella_get_search_stats(hella_ctx c, int beam, float *mu, float *sig
     }
     if (sigma0) *sigma0 = sc.sig0;
     if (gated) *gated = sc.gated;
+  });
+}
+
+hella_status hella_get_search_norm(hella_ctx c, int beam, double *mu, double *inv, float *thr)
+{
+  if (!valid_beam(c, beam)) return fail(HELLA_ERR_INVALID, "bad beam");
+  return guard([&] {
+    const BeamState &B = c->beams[(size_t)beam];
+    SScal sc;
+    HAMD_CHECK(hipStreamSynchronize(c->stream));
+    HAMD_CHECK(hipMemcpy(&sc, B.s_scal.p, sizeof(SScal), hipMemcpyDeviceToHost));
+    for (int k = 0; k < c->nw; ++k) {
+      if (mu) mu[k] = sc.mu[k];
+      if (inv) inv[k] = sc.inv[k];
+    }
+    if (thr) *thr = sc.thr;
   });
 }
 

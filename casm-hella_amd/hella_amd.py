@@ -270,6 +270,8 @@ def _declare_hella(L):
     d("hella_copy_window", _I, P, _I, P, C.c_size_t)
     d("hella_copy_dedisp", _I, P, _I, P, C.c_size_t)
     d("hella_set_dedisp", _I, P, _I, P, C.c_size_t)
+    d("hella_get_search_norm", _I, P, _I, C.POINTER(C.c_double), C.POINTER(C.c_double),
+      C.POINTER(_F))
     d("hella_get_search_stats", _I, P, _I, C.POINTER(_F), C.POINTER(_F), C.POINTER(_F),
       C.POINTER(_I))
     d("hella_stage_times", _I, P, C.POINTER(_F))
@@ -510,6 +512,17 @@ class Hella:
                                            sg.ctypes.data_as(C.POINTER(_F)), C.byref(s0),
                                            C.byref(gated)))
         return dict(mu=mu, sigma=sg, sigma0=s0.value, gated=bool(gated.value))
+
+    def search_norm(self, beam=0):
+        """per-width mean and 1/sigma (double) and the threshold of the last search"""
+        nw = self.info(beam)["nwidth"]
+        mu = np.zeros(nw, np.float64)
+        inv = np.zeros(nw, np.float64)
+        thr = _F()
+        _hchk(lib().hella_get_search_norm(self._c, beam, mu.ctypes.data_as(C.POINTER(C.c_double)),
+                                          inv.ctypes.data_as(C.POINTER(C.c_double)),
+                                          C.byref(thr)))
+        return mu, inv, thr.value
 
     def stage_times(self):
         ms = (_F * 4)()

@@ -147,6 +147,10 @@ hella_status hella_set_dedisp(hella_ctx ctx, int beam, const void *host, size_t 
 /* per-width mean/sigma (nwidth each), sigma0, gate flag of the beam's last search */
 hella_status hella_get_search_stats(hella_ctx ctx, int beam, float *mu, float *sigma,
                                     float *sigma0, int *gated);
+/* the exact normalisation of the beam's last search: per width the mean and
+ * 1/sigma in double (nwidth each; S/N = (float)((raw - mu) * inv)), and the
+ * threshold snr * sigma0 in float */
+hella_status hella_get_search_norm(hella_ctx ctx, int beam, double *mu, double *inv, float *thr);
 /* per-stage device time of the last gulp in ms: flag, dedisperse, search, peaks (4) */
 hella_status hella_stage_times(hella_ctx ctx, float *ms);
 /* enable per-stage event timing (adds syncs between stages) */

@@ -251,3 +251,113 @@ def test_c4_server_shape(hamd, orc, mode):
         ncand += len(r["cands"])
     assert checked == nbeam * (nseq - mg)
     assert ncand > 0
+
+
+# ------------------------------------------------------------------ stress shape, whole path
+def _box_rows(D, rows, nout, widths, maxW):
+    """box sums of the given rows of D for every width: [len(widths)][len(rows)][nout] u64"""
+    P = np.zeros((len(rows), D.shape[1] + 1), np.uint64)
+    np.cumsum(D[rows].astype(np.uint64), axis=1, out=P[:, 1:])
+    c0 = maxW // 2
+    out = []
+    for w in widths:
+        a = np.arange(nout) + c0 - w // 2
+        out.append(P[:, a + w] - P[:, a])
+    return np.stack(out)
+
+
+@pytest.mark.timeout(900)
+def test_c5_stress_whole_path(hamd, orc):
+    """One beam of the stress configuration (4096 ch FP32 N(0,1) input, 20% of
+    the channels at 30 sigma, broadband bursts every second, 65.536 us,
+    262144-sample chunks, DM 0-5000, widths 1..1024) through the whole plain
+    path, with pulses bright enough for more than 1e5 threshold crossings:
+    * the last chunk's flagged bytes equal the oracle's flagger;
+    * the DM-0 row equals the window's channel sums, sampled DM rows equal
+      the oracle's sums;
+    * every candidate's S/N, recomputed on the host from the plane with the
+      search's own normalisation, agrees within 1e-5;
+    * on sampled DM rows every width's crossings are exactly the samples whose
+      recomputed S/N exceeds the threshold;
+    * the clusters of the (uncapped) list equal the oracle's clusters."""
+    nch, W, ts = 4096, 262144, 65.536e-6
+    cfg = dict(INPUT="FILTERBANK", INPUT_DTYPE="F32", NCHAN=nch, FCH1=500.0, FOFF=-100.0 / nch,
+               TSAMP=ts, GULP=W, DM_MIN=0, DM_MAX=5000, DM_TOL=1.25, PULSE_WIDTH=64,
+               WIDTH_MIN=1, WIDTH_MAX=1024, SNR=7.0, SCRUNCH=[(1, 1, 6.0)],
+               MAX_GIANTS=1 << 21, CAND_CAP=1 << 21)
+    rng = np.random.default_rng(55)
+    rfi = rng.choice(nch, nch // 5, replace=False)
+    # bright pulses whose sweeps stay inside one chunk (the chunk repeats)
+    pulses = [(float(rng.uniform(0.3, 4.0)), float(rng.uniform(100, 900)),
+               float(rng.uniform(0.5, 4.0)), float(rng.uniform(40, 80))) for _ in range(24)]
+    h = hamd.Hella(cfg)
+    i = h.info(0)
+    assert (i["ndm"], i["max_delay"]) == (7511, 712492)
+    _say(f"c5: synthesising a {nch} x {W} FP32 chunk")
+    chunk = hamd.synth(nch, W, 500.0, -100.0 / nch, ts, seed=5005, dtype="f32", mean=0.0,
+                       sigma=1.0, rfi_chan=rfi, rfi_mult=30.0, pulses=pulses,
+                       burst_period=int(round(1.0 / ts)), burst_phase=500, burst_len=10,
+                       burst_amp=6.0)
+    buf = hamd.DeviceBuffer(chunk.nbytes)
+    buf.upload(chunk)
+    ng = i["min_gulp"] + 1
+    for g in range(ng):
+        _say(f"c5: gulp {g}")
+        h.process_gulp(buf)
+    buf.free()
+    crossings = h.crossings(0)
+    p = h.peaks()
+    _say(f"c5: {crossings} crossings, {len(p)} candidates; oracle flagger")
+    assert 100000 < crossings < (1 << 21) and len(p) == crossings
+    # flagger: every gulp is the same chunk, so the window is that chunk's
+    # flagged bytes repeated; the GPU's last chunk against the oracle
+    fl, _ = orc.flag(orc.ingest_f32_tf(chunk), -0.05, 0.15, -1, -1, ((1, 1, 6.0),))
+    del chunk
+    win = h.window(0)
+    a0 = ng * W - i["ntime"]
+    np.testing.assert_array_equal(win[:, -W:], fl)
+    np.testing.assert_array_equal(win[:, :W - a0], fl[:, a0:])
+    del fl
+    _say("c5: plane")
+    D = h.dedisp_plane(0)
+    ntdd = i["ntime_dd"]
+    np.testing.assert_array_equal(D[0], win[:, :ntdd].sum(axis=0, dtype=np.uint32))
+    op = orc.Plan(nch, ts, 500.0, -100.0 / nch, 0.0, 5000.0, 64.0, 1.25)
+    rows = np.unique(np.r_[i["ndm"] - 1, rng.choice(i["ndm"], 14, replace=False),
+                           np.unique(p["dm_idx"])[:: max(1, len(np.unique(p["dm_idx"])) // 6)]])
+    np.testing.assert_array_equal(D[rows], orc.dedisperse_ft(win, op.delta[rows], ntdd))
+    del win
+    # candidates: S/N from the plane with the search's normalisation
+    _say("c5: candidates")
+    mu, inv, thr = h.search_norm(0)
+    st = h.search_stats(0)
+    assert not st["gated"]
+    widths = [1 << k for k in range(i["nwidth"])]
+    nout = i["ntime_out"]
+    for k, w in enumerate(widths):
+        sel = p[p["width"] == k]
+        if not len(sel):
+            continue
+        lo = sel["samp"] + 512 - w // 2
+        box = np.zeros(len(sel), np.uint64)
+        for j in range(w):
+            box += D[sel["dm_idx"], lo + j]
+        snr = ((box.astype(np.float64) - mu[k]) * inv[k]).astype(np.float32)
+        np.testing.assert_allclose(sel["snr"], snr, rtol=1e-5, atol=1e-5)
+    # sampled rows: the crossings of every width are exactly the host's
+    srows = np.unique(np.r_[rows, np.unique(p["dm_idx"])[:8]])
+    B = _box_rows(D, srows, nout, widths, 1024)
+    for k in range(len(widths)):
+        y = ((B[k].astype(np.float64) - mu[k]) * inv[k]).astype(np.float32)
+        for j, r in enumerate(srows):
+            want = np.nonzero(y[j] > thr)[0]
+            got = p[(p["width"] == k) & (p["dm_idx"] == r)]["samp"]
+            np.testing.assert_array_equal(got, want)
+    del D, B
+    _say("c5: clusters")
+    ref = orc.cluster(p, 64, 16)
+    got = h.clusters()
+    assert [(int(c["rep"]), int(c["count"]), int(c["samp_min"]), int(c["samp_max"]),
+             int(c["dm_min"]), int(c["dm_max"])) for c in got] == \
+        [(c["rep"], c["count"], c["samp_min"], c["samp_max"], c["dm_min"], c["dm_max"]) for c in ref]
+    h.close()
