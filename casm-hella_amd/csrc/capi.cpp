@@ -280,6 +280,48 @@ int hella_get_flag_counts(hella_ctx c, int beam, unsigned *out, int cap)
   return st == HELLA_OK ? n : -1;
 }
 
+int hella_get_flag_spectrum(hella_ctx c, int beam, float *out, int cap)
+{
+  if (!valid_beam(c, beam) || cap < 0 || (cap > 0 && !out)) {
+    fail(HELLA_ERR_INVALID, "bad argument");
+    return -1;
+  }
+  int n = -1;
+  const hella_status st = guard([&] {
+    ctx_build(c);
+    const BeamState &B = c->beams[(size_t)beam];
+    n = B.nch;
+    if (cap > 0) {
+      HAMD_CHECK(hipStreamSynchronize(c->stream));
+      HAMD_CHECK(hipMemcpy(out, B.f_count.as<uint8_t>() + flag_state_spec_offset(B.nch),
+                           (size_t)(cap < n ? cap : n) * 4, hipMemcpyDeviceToHost));
+    }
+  });
+  return st == HELLA_OK ? n : -1;
+}
+
+hella_status hella_accumulate_specflags(hella_ctx c, int *specflags, int nchan)
+{
+  if (!c || nchan <= 0 || !specflags) return fail(HELLA_ERR_INVALID, "bad argument");
+  return guard([&] {
+    ctx_build(c);
+    std::vector<double> sum((size_t)nchan, 0.0);
+    std::vector<float> one((size_t)nchan);
+    long long ntime = -1;
+    HAMD_CHECK(hipStreamSynchronize(c->stream));
+    for (const BeamState &B : c->beams) {
+      if (B.nch != nchan) continue;
+      if (ntime < 0) ntime = B.ntime;
+      HAMD_CHECK(hipMemcpy(one.data(), B.f_count.as<uint8_t>() + flag_state_spec_offset(B.nch),
+                           (size_t)nchan * 4, hipMemcpyDeviceToHost));
+      for (int i = 0; i < nchan; ++i) sum[(size_t)i] += (double)one[(size_t)i];
+    }
+    if (ntime < 0) throw std::invalid_argument("no beam has that many channels");
+    for (int i = 0; i < nchan; ++i)
+      specflags[i] += (int)((double)ntime * sum[(size_t)i]);
+  });
+}
+
 long long hella_last_crossings(hella_ctx c, int beam)
 {
   if (!valid_beam(c, beam)) return -1;

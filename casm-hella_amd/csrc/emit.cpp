@@ -79,9 +79,9 @@ int hella_format_coincidencer(long long samp0, long long gulp, const char *dada_
   return (int)msg.size();
 }
 
-hella_status hella_write_specflags(const char *path, const unsigned *counts, int nchan)
+hella_status hella_write_specflags(const char *path, const int *specflags, int nchan)
 {
-  if (!path || nchan < 0 || (nchan > 0 && !counts)) {
+  if (!path || nchan < 0 || (nchan > 0 && !specflags)) {
     set_last_error("bad argument");
     return HELLA_ERR_INVALID;
   }
@@ -91,7 +91,7 @@ hella_status hella_write_specflags(const char *path, const unsigned *counts, int
     return HELLA_ERR_INVALID;
   }
   bool ok = true;
-  for (int i = 0; ok && i < nchan; ++i) ok = std::fprintf(f, "%u\n", counts[i]) > 0;
+  for (int i = 0; ok && i < nchan; ++i) ok = std::fprintf(f, "%d\n", specflags[i]) > 0;
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) {
     set_last_error(std::string("write to ") + path + " failed");

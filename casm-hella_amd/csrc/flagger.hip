@@ -88,7 +88,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
     bad = (b2 < a0) || (b2 > a1);
   }
   const bool fixed = c >= B.flag1 && c < B.flag2;
-  uint32_t cnt = 0;
+  uint32_t cnt = 0, cntw = 0;
   double acc[2] = {0.0, 0.0};
 #pragma unroll 4
   for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
@@ -111,6 +111,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
         x = (float)(((double)x - sc.mu) * sc.inv);
         const bool g = (x > a0) || (x < a1);
         cnt += g ? 1u : 0u;
+        cntw += (g && t < B.wp) ? 1u : 0u;
         if (g || fixed) x = 0.f;
         x = x + 1.f;
         row[t] = x;
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
     }
   }
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
+  if (cntw) atomicAdd(&B.gsum[c], (unsigned long long)cntw);
   if (part == FPART_NONE) return;
   tree256_multi(acc, sh);
   if (l == 0) {
@@ -139,8 +141,13 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
   }
 }
 
-// bandpass from partials: bp[c] = (float)(sum_b part[c][b] / W)
-__global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict__ beams, int W)
+// bandpass from partials: bp[c] = (float)(sum_b part[c][b] / W).  After a
+// scrunch (box = ts * fs > 0) also the flag spectrum: the mean of the grown
+// mask G over the statistics columns, G being the fraction of masked pixels
+// in each box (calc_bandpass + add_bandpass, flagger.cu:333-334):
+// fspec[c] += (float)(sum / box / wp), the scrunch's sum then cleared.
+__global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict__ beams, int W,
+                                                  int box)
 {
   const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.x * NT + threadIdx.x;
@@ -148,6 +155,11 @@ __global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict_
   double s = 0.0;
   for (int b = 0; b < B.nblk; ++b) s += B.part[(int64_t)c * B.nblk + b];
   B.bp[c] = (float)(s / (double)W);
+  if (box > 0) {
+    const double g = (double)B.gsum[c];
+    B.fspec[c] = B.fspec[c] + (float)(g / (double)box / (double)B.wp);
+    B.gsum[c] = 0ull;
+  }
 }
 
 // Statistics of the plane from the block sums (calculate_stddev,
@@ -263,15 +275,27 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
   __shared__ double sh[128];
   double acc[1] = {0.0};
   uint32_t cnt = 0;
+  unsigned long long nbox = 0;
   float *row = B.X + (int64_t)c * B.xs;
   for (int kk = 0; kk < FLAG_BLOCK / NT; ++kk) {
     const int t = base + l + NT * kk;
     if (t >= W) break;
     bool g = false;
-    for (int i = 0; i < fs && !g; ++i) {
-      const uint8_t *mrow = B.mask + (int64_t)clampi(c + i - ay, 0, B.nch - 1) * B.xs;
-      for (int j = 0; j < ts; ++j)
-        if (mrow[clampi(t + j - ax, 0, W - 1)]) { g = true; break; }
+    if (t < B.wp) {
+      // the statistics columns: every masked pixel of the box counts (G's sum)
+      uint32_t n = 0;
+      for (int i = 0; i < fs; ++i) {
+        const uint8_t *mrow = B.mask + (int64_t)clampi(c + i - ay, 0, B.nch - 1) * B.xs;
+        for (int j = 0; j < ts; ++j) n += mrow[clampi(t + j - ax, 0, W - 1)] ? 1u : 0u;
+      }
+      nbox += n;
+      g = n > 0;
+    } else {
+      for (int i = 0; i < fs && !g; ++i) {
+        const uint8_t *mrow = B.mask + (int64_t)clampi(c + i - ay, 0, B.nch - 1) * B.xs;
+        for (int j = 0; j < ts; ++j)
+          if (mrow[clampi(t + j - ax, 0, W - 1)]) { g = true; break; }
+      }
     }
     cnt += g ? 1u : 0u;
     float x = row[t];
@@ -281,6 +305,7 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
     if (t < npart) acc[0] += (double)x;
   }
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
+  if (nbox) atomicAdd(&B.gsum[c], nbox);
   tree256_multi(acc, sh);
   if (l == 0) B.part[(int64_t)c * B.nblk + blockIdx.x] = acc[0];
 }
@@ -456,7 +481,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
 #pragma unroll
   for (int j = 0; j < NB; ++j) *reinterpret_cast<u32x4f *>(&bytes[j][16 * l]) = d[j];
   __syncthreads();
-  uint32_t cnt = 0;
+  uint32_t cnt = 0, cntw = 0;
   double acc[2 * NB];
 #pragma unroll
   for (int j = 0; j < 2 * NB; ++j) acc[j] = 0.0;
@@ -478,6 +503,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       for (int k = 0; k < KB; ++k) {
         x[k] = tab[v[k]];
         cnt += flg[v[k]];
+        cntw += base + l + NT * k < B.wp ? flg[v[k]] : 0u;
       }
       if (TSD) {
 #pragma unroll
@@ -509,6 +535,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       const int v = bytes[j][l + NT * k];
       float x = tab[v];
       cnt += flg[v];
+      cntw += t < B.wp ? flg[v] : 0u;
       if (TSD) x = x / tsb[l + NT * k];
       if (to_ring) {
         x = (float)(((double)x - sc.mu) * sc.inv);
@@ -524,6 +551,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
     }
   }
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
+  if (cntw) atomicAdd(&B.gsum[c], (unsigned long long)cntw);
   if (part == FPART_NONE) return;
   tree256_multi(acc, sh);
   if (l == 0) {
@@ -644,7 +672,9 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
     hipLaunchKernelGGL(k_apply, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0, a1);
   };
-  auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
+  auto bp = [&](int box = 0) {
+    hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W, box);
+  };
   auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
 
   hipLaunchKernelGGL(k_ingest, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
@@ -677,7 +707,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
       hipLaunchKernelGGL(k_grow_replace, gplane, dim3(NT), 0, s, d_beams, W, S.tscrunch,
                          S.fscrunch, n32);
     }
-    bp();
+    bp(S.tscrunch * S.fscrunch);
   }
   // apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401): bandpass_correct,
   // ts_correct, normalize_data
@@ -725,7 +755,9 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
                          npart, a0, a1, cur, out);
     cur = out;
   };
-  auto bp = [&]() { hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W); };
+  auto bp = [&](int box = 0) {
+    hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W, box);
+  };
   auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
 
   hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
@@ -746,7 +778,7 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
     const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
     const float nthr = (float)(-1. * (double)thr);
     apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
-    bp();
+    bp(1);
   }
   // bandpass_correct of the final scrunch as a table, then ts_correct
   const int nxt = cur == 0 ? 1 : 0;

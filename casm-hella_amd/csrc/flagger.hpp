@@ -47,6 +47,14 @@ struct FlagBeamDev {
   float *bp;           // [nch] bandpass
   FScal *scal;
   uint32_t *flagcount; // [nch] grown-mask counts (accumulated)
+  // the reference's flagSpec (flagger.cu:116-124, 333-334): per channel, the
+  // mean over the statistics columns of each scrunch's grown mask G (the box
+  // mean of the threshold mask), accumulated in float over scrunches and
+  // gulps.  gsum collects a scrunch's integer sum of masked pixels over the
+  // boxes of the statistics columns; k_finish_bp folds it into fspec.
+  float *fspec;              // [nch]
+  unsigned long long *gsum;  // [nch]
+  int wp;                    // statistics columns: 256 * floor(W / 256) (W if W < 256)
   float *ts;           // [W] time series of the final pass
   uint8_t *mask;       // [nch][xs] threshold mask scratch
   // value-table path (uint8 input, 1 x 1 scrunches only): the input as a
@@ -107,6 +115,13 @@ inline bool flag_needs_mask(const FlagParams &P)
     if (P.scrunch[i].tscrunch != 1 || P.scrunch[i].fscrunch != 1) return true;
   return false;
 }
+
+// A beam's resident flagger state in one buffer: grown-mask counts (u32),
+// the flag spectrum (f32) and the scrunch's masked-pixel sums (u64), nch
+// each, all zero at the start.
+inline size_t flag_state_spec_offset(int nch) { return (size_t)nch * 4; }
+inline size_t flag_state_gsum_offset(int nch) { return ((size_t)nch * 8 + 7) / 8 * 8; }
+inline size_t flag_state_bytes(int nch) { return flag_state_gsum_offset(nch) + (size_t)nch * 8; }
 
 void zero_u32(uint32_t *p, int n, hipStream_t s);
 

@@ -176,7 +176,7 @@ void beam_setup(BeamState &B, const Config &cfg, int W, hipStream_t s)
   }
   B.f_bp.ensure((size_t)B.nch * 4);
   alloc_zero(B.f_scal, sizeof(FScal), s);
-  alloc_zero(B.f_count, (size_t)B.nch * 4, s);
+  alloc_zero(B.f_count, flag_state_bytes(B.nch), s);
   alloc_zero(B.s_scal, sizeof(SScal), s);
   B.gidx = 0;
   B.crossings = 0;
@@ -482,6 +482,10 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
   f.bp = B.f_bp.as<float>();
   f.scal = B.f_scal.as<FScal>();
   f.flagcount = B.f_count.as<uint32_t>();
+  f.fspec = reinterpret_cast<float *>(B.f_count.as<uint8_t>() + flag_state_spec_offset(B.nch));
+  f.gsum = reinterpret_cast<unsigned long long *>(B.f_count.as<uint8_t>() +
+                                                  flag_state_gsum_offset(B.nch));
+  f.wp = W >= 256 ? (W / 256) * 256 : W;
   f.ts = w.f_ts.as<float>();
   f.mask = w.mask.as<uint8_t>();
   if (flag_value_tables(cfg.input_fmt, cfg.flag)) {

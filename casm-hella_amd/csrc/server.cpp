@@ -158,8 +158,8 @@ template <class F> hella_status sguard(F &&f)
 size_t resident_bytes(const BeamState &g)
 {
   const auto r = [](size_t b) { return round_up(std::max<size_t>(b, 16), 256); };
-  return r((size_t)g.nch * g.ring_stride) + r((size_t)g.nch * 4) * 2 + r(sizeof(FScal)) +
-         r(sizeof(SScal));
+  return r((size_t)g.nch * g.ring_stride) + r((size_t)g.nch * 4) + r(flag_state_bytes(g.nch)) +
+         r(sizeof(FScal)) + r(sizeof(SScal));
 }
 
 // plan, sizing and device delay tables shared by every beam of the server
@@ -287,7 +287,7 @@ std::unique_ptr<BeamState> new_resident(hella_server s)
     B->ring.ensure((size_t)g.nch * g.ring_stride);
     B->f_bp.ensure((size_t)g.nch * 4);
     B->f_scal.ensure(std::max<size_t>(sizeof(FScal), 16));
-    B->f_count.ensure((size_t)g.nch * 4);
+    B->f_count.ensure(flag_state_bytes(g.nch));
     B->s_scal.ensure(std::max<size_t>(sizeof(SScal), 16));
   }
   // the null stream: the lanes are non-blocking streams, so this does not

@@ -310,7 +310,9 @@ def _declare_hella(L):
     d("hella_write_candidates", _I, C.c_char_p, C.POINTER(_Cand), _I, _I)
     d("hella_format_coincidencer", _I, C.c_longlong, C.c_longlong, C.c_char_p, C.POINTER(_Cand),
       _I, _I, C.c_char_p, _I)
-    d("hella_write_specflags", _I, C.c_char_p, C.POINTER(C.c_uint), _I)
+    d("hella_write_specflags", _I, C.c_char_p, C.POINTER(_I), _I)
+    d("hella_get_flag_spectrum", _I, P, _I, C.POINTER(_F), _I)
+    d("hella_accumulate_specflags", _I, P, C.POINTER(_I), _I)
     d("hella_dada_header_value", _I, C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, _I)
     d("hella_dada_coincidencer_fields", _I, C.c_char_p, C.c_size_t, C.c_char_p, _I)
     d("hella_sigproc_read_header", _I, C.c_char_p, C.POINTER(FilHeader))
@@ -480,6 +482,24 @@ class Hella:
         out = np.zeros(n, np.uint32)
         lib().hella_get_flag_counts(self._c, beam, out.ctypes.data_as(C.POINTER(C.c_uint)), n)
         return out
+
+    def flag_spectrum(self, beam=0):
+        """The beam's flag spectrum (the reference's flagSpec): per channel the
+        accumulated mean of every scrunch's grown mask."""
+        n = lib().hella_get_flag_spectrum(self._c, beam, None, 0)
+        if n < 0:
+            _hchk(1)
+        out = np.zeros(n, np.float32)
+        lib().hella_get_flag_spectrum(self._c, beam, out.ctypes.data_as(C.POINTER(_F)), n)
+        return out
+
+    def accumulate_specflags(self, specflags):
+        """specflags[i] += (int)(NTIME * flagSpec[i]) over the beams with
+        len(specflags) channels (casm:705-716); specflags: int32 array, updated"""
+        assert specflags.dtype == np.int32 and specflags.flags.c_contiguous
+        _hchk(lib().hella_accumulate_specflags(self._c, specflags.ctypes.data_as(C.POINTER(_I)),
+                                               int(specflags.size)))
+        return specflags
 
     def crossings(self, beam):
         return int(lib().hella_last_crossings(self._c, beam))
@@ -773,10 +793,10 @@ def dada_coincidencer_fields(header):
     return buf.value.decode()
 
 
-def write_specflags(path, counts):
-    """Append one line per channel with its flagged-sample count."""
-    a = np.ascontiguousarray(counts, np.uint32)
-    _hchk(lib().hella_write_specflags(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_uint)),
+def write_specflags(path, specflags):
+    """Append one "%d" line per channel (the spectral-flag file of casm:756-761)."""
+    a = np.ascontiguousarray(specflags, np.int32)
+    _hchk(lib().hella_write_specflags(str(path).encode(), a.ctypes.data_as(C.POINTER(_I)),
                                       int(a.size)))
 
 

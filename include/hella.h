@@ -208,11 +208,23 @@ long long hella_sigproc_read(const char *path, const hella_fil_header *hdr, long
 int hella_sigproc_config(const hella_fil_header *hdr, char *buf, int cap);
 
 /* ---- candidate emission (src/hella/src/peaks.cu:129-259) ----------------- */
-/* Per-channel count of flagged samples of the beam (the grown-mask counts of
- * every 1 x 1 / box scrunch, accumulated since the context was created; the
- * reference's flagSpec, flagger.cu:116-124, casm:705-716).  Returns nchan
- * (copies min(cap, nchan)), or -1 on error. */
+/* Per-channel count of flagged samples of the beam (pixels of every 1 x 1 /
+ * box scrunch's grown mask, accumulated since the context was created).
+ * Returns nchan (copies min(cap, nchan)), or -1 on error. */
 int hella_get_flag_counts(hella_ctx ctx, int beam, unsigned *out, int cap);
+/* The beam's flag spectrum, the reference's flagSpec (flagger.cu:116-124,
+ * 333-334): per channel, each scrunch adds the mean over the statistics
+ * columns (256 * floor(gulp / 256)) of its grown mask G, G being the fraction
+ * of masked pixels in each ts x fs box (1 x 1: the mask itself), as
+ * fspec += (float)(sum / (ts * fs) / columns), accumulated since the context
+ * was created.  Returns nchan (copies min(cap, nchan)), or -1 on error. */
+int hella_get_flag_spectrum(hella_ctx ctx, int beam, float *out, int cap);
+/* casm:705-716 after a gulp whose windows are full: for every channel i <
+ * nchan, specflags[i] += (int)(NTIME * flagSpec[i]), flagSpec being the sum
+ * (in beam order, double) of the flag spectra of the beams with nchan
+ * channels (the reference's beams share one flagSpec array) and NTIME the
+ * window length of the first of them. */
+hella_status hella_accumulate_specflags(hella_ctx ctx, int *specflags, int nchan);
 /* Append candidates to the text file path as output_peaks does
  * (peaks.cu:129-182): the header "SNR SAMP_START TIME_START WIDTH DM_IDX DM
  * BEAM_IDX" if the file is empty, then one "%g %d %g %d %d %g %d" line per
@@ -245,9 +257,9 @@ int hella_dada_header_value(const char *header, size_t len, const char *key, cha
  * map::at throws). */
 int hella_dada_coincidencer_fields(const char *header, size_t len, char *out, int cap);
 
-/* Append one "%u" line per channel (the spectral-flag file of
- * casm:756-761). */
-hella_status hella_write_specflags(const char *path, const unsigned *counts, int nchan);
+/* Append one "%d" line per channel: the spectral-flag file of casm:756-761
+ * (the specflags array hella_accumulate_specflags keeps). */
+hella_status hella_write_specflags(const char *path, const int *specflags, int nchan);
 
 /* ---- PSRDADA-style block ring (streaming ingest) -------------------------
  * Replaces the PSRDADA data block casm-hella reads from (src/hella/src/dada.cpp:16-88;

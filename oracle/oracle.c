@@ -276,8 +276,9 @@ void orc_box_mean(const float *x, int nch, int W, int ts, int fs, float *out)
 /* apply_scrunch flag branch (src/flagger.cu:302-340) on a plane that has
  * already been bandpass-corrected and normalised. */
 void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
-                      int flag1, int flag2, uint32_t *flagcount)
+                      int flag1, int flag2, uint32_t *flagcount, float *flagspec)
 {
+  const int wp = (int)stats_width((size_t)W);
   size_t n = (size_t)nch * W;
   float *S = (float *)malloc(n * sizeof(float));
   uint8_t *M = (uint8_t *)malloc(n);
@@ -291,19 +292,23 @@ void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
   for (int c = 0; c < nch; ++c) {
     int fixed = c >= flag1 && c < flag2;
     uint32_t cnt = 0;
+    uint64_t nbox = 0; /* masked pixels of the boxes of columns t < wp */
     for (int t = 0; t < W; ++t) {
-      int g = 0;
-      for (int i = 0; i < fs && !g; ++i) {
+      int n = 0;
+      for (int i = 0; i < fs; ++i) {
         const uint8_t *row = M + (size_t)clampi(c + i - ay, 0, nch - 1) * W;
-        for (int j = 0; j < ts; ++j)
-          if (row[clampi(t + j - ax, 0, W - 1)]) { g = 1; break; }
+        for (int j = 0; j < ts; ++j) n += row[clampi(t + j - ax, 0, W - 1)] ? 1 : 0;
       }
+      int g = n > 0;
+      if (t < wp) nbox += (uint64_t)n;
       cnt += (uint32_t)g;
       float *v = x + (size_t)c * W + t;
       if (g || fixed) *v = 0.f;
       *v = *v + 1.f;
     }
     if (flagcount) flagcount[c] += cnt;
+    if (flagspec)
+      flagspec[c] = flagspec[c] + (float)((double)nbox / (double)(ts * fs) / (double)wp);
   }
   free(S);
   free(M);
@@ -384,7 +389,7 @@ static void bandpass_correct(float *x, int nch, int W)
 
 /* fast_flagger for one beam (src/flagger.cu:345-411) */
 void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride,
-              uint32_t *flagcount)
+              uint32_t *flagcount, float *flagspec)
 {
   int nch = cfg->nch, W = cfg->W;
   /* bandpass_flag (flagger.cu:216-242) */
@@ -396,7 +401,7 @@ void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride
     bandpass_correct(x, nch, W);
     orc_normalize(x, nch, W);
     orc_scrunch_flag(x, nch, W, cfg->scrunch[s].tscrunch, cfg->scrunch[s].fscrunch,
-                     cfg->scrunch[s].thresh, cfg->flag1, cfg->flag2, flagcount);
+                     cfg->scrunch[s].thresh, cfg->flag1, cfg->flag2, flagcount, flagspec);
   }
   /* apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401) */
   bandpass_correct(x, nch, W);

@@ -89,8 +89,13 @@ void orc_divide_bp(float *x, int nch, int W, const float *bp);         /* divide
 void orc_normalize(float *x, int nch, int W);                          /* normalize_data */
 void orc_spec_flag(float *x, int nch, int W, float smin, float smax);  /* calc_bandpass_new + replace_data_and_add_bandpass */
 void orc_box_mean(const float *x, int nch, int W, int ts, int fs, float *out); /* npp_convolve_handler */
+/* apply_scrunch flag branch; flagspec (may be NULL) accumulates the mean of
+ * the grown mask G over the statistics columns (flagger.cu:333-334): G is the
+ * fraction of masked pixels in each box, so per channel
+ * flagspec[c] += (float)((double)n / (ts*fs) / wp), n the masked pixels of
+ * the boxes of columns t < wp = 256*floor(W/256) (W when W < 256) */
 void orc_scrunch_flag(float *x, int nch, int W, int ts, int fs, float thresh,
-                      int flag1, int flag2, uint32_t *flagcount);     /* apply_scrunch flag branch */
+                      int flag1, int flag2, uint32_t *flagcount, float *flagspec);
 /* channel sum of sample t: ORC_TS_PARTS partitions of ceil(nch/16)
  * consecutive channels, each summed in channel order, the partials combined
  * pairwise (p[q] += p[q+h], h = 8, 4, 2, 1); double */
@@ -103,9 +108,10 @@ void orc_ingest_u8_tf(const uint8_t *in, int nch, int W, float *x);
 void orc_ingest_f32_tf(const float *in, int nch, int W, float *x);
 /* whole fast_flagger for one beam; x is the FT plane (modified in place),
  * out receives u8 FT rows with stride out_stride.  flagcount (may be NULL)
- * accumulates the grown-mask count per channel. */
+ * accumulates the grown-mask count per channel, flagspec (may be NULL) the
+ * flag spectrum of every scrunch (orc_scrunch_flag). */
 void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride,
-              uint32_t *flagcount);
+              uint32_t *flagcount, float *flagspec);
 
 /* ---- A12/A13 search: boxcar or hella smoothing, normalise, threshold ---- */
 enum { ORC_MODE_BOXCAR = 0, ORC_MODE_HELLA = 1 };

@@ -65,13 +65,13 @@ _d("orc_divide_bp", None, _VP, _I, _I, _VP)
 _d("orc_normalize", None, _VP, _I, _I)
 _d("orc_spec_flag", None, _VP, _I, _I, _F, _F)
 _d("orc_box_mean", None, _VP, _I, _I, _I, _I, _VP)
-_d("orc_scrunch_flag", None, _VP, _I, _I, _I, _I, _F, _I, _I, _VP)
+_d("orc_scrunch_flag", None, _VP, _I, _I, _I, _I, _F, _I, _I, _VP, _VP)
 _d("orc_ts_correct", None, _VP, _I, _I, _VP)
 _d("orc_chan_sum", _D, _VP, _I, _I, _I)
 _d("orc_quantize", None, _VP, _I, _I, _VP, _SZ)
 _d("orc_ingest_u8_tf", None, _VP, _I, _I, _VP)
 _d("orc_ingest_f32_tf", None, _VP, _I, _I, _VP)
-_d("orc_flag", None, C.POINTER(FlagCfg), _VP, _VP, _SZ, _VP)
+_d("orc_flag", None, C.POINTER(FlagCfg), _VP, _VP, _SZ, _VP, _VP)
 _d("orc_nwidth", _I, C.POINTER(SearchCfg))
 _d("orc_width_of", _I, C.POINTER(SearchCfg), _I)
 _d("orc_hella_kernel", None, _I, _VP)
@@ -203,11 +203,13 @@ def box_mean(x, ts, fs):
     return out
 
 
-def scrunch_flag(x, ts, fs, thresh, flag1=-1, flag2=-1):
+def scrunch_flag(x, ts, fs, thresh, flag1=-1, flag2=-1, want_spec=False):
     x = np.array(x, np.float32, copy=True)
     cnt = np.zeros(x.shape[0], np.uint32)
-    L.orc_scrunch_flag(_p(x), x.shape[0], x.shape[1], ts, fs, thresh, flag1, flag2, _p(cnt))
-    return x, cnt
+    spec = np.zeros(x.shape[0], np.float32)
+    L.orc_scrunch_flag(_p(x), x.shape[0], x.shape[1], ts, fs, thresh, flag1, flag2, _p(cnt),
+                       _p(spec))
+    return (x, cnt, spec) if want_spec else (x, cnt)
 
 
 def chan_sum(x, t):
@@ -245,8 +247,9 @@ def ingest_f32_tf(data_tf):
     return x
 
 
-def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=()):
-    """Whole per-beam flagger on a float FT plane; returns (u8 FT, flagcount)."""
+def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), spec=None):
+    """Whole per-beam flagger on a float FT plane; returns (u8 FT, flagcount).
+    spec: a float32 [nch] array that accumulates the flag spectrum."""
     x = np.array(x_ft, np.float32, copy=True)
     nch, W = x.shape
     sc = (Scrunch * max(1, len(scrunches)))()
@@ -255,7 +258,9 @@ def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=()):
     cfg = FlagCfg(nch, W, spec_min, spec_max, flag1, flag2, len(scrunches), sc)
     out = np.zeros((nch, W), np.uint8)
     cnt = np.zeros(nch, np.uint32)
-    L.orc_flag(C.byref(cfg), _p(x), _p(out), W, _p(cnt))
+    if spec is not None:
+        assert spec.dtype == np.float32 and spec.size == nch and spec.flags.c_contiguous
+    L.orc_flag(C.byref(cfg), _p(x), _p(out), W, _p(cnt), _p(spec) if spec is not None else None)
     return out, cnt
 
 

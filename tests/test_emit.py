@@ -61,9 +61,10 @@ def test_coincidencer_truncates_like_snprintf(hamd):
 
 def test_specflags_file(hamd, tmp_path):
     p = tmp_path / "spec.txt"
-    hamd.write_specflags(p, [0, 5, 4294967295])
+    # the reference writes its int specflags with "%d" (casm:756-761)
+    hamd.write_specflags(p, [0, 5, 2147483647])
     hamd.write_specflags(p, [1])
-    assert p.read_text().splitlines() == ["0", "5", "4294967295", "1"]
+    assert p.read_text().splitlines() == ["0", "5", "2147483647", "1"]
 
 
 def test_bad_paths(hamd, tmp_path):

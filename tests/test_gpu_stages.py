@@ -141,6 +141,46 @@ def test_dada_fp16_channel_major_input(hamd, orc):
     h.close()
 
 
+@pytest.mark.parametrize("dtype,scr", [("U8", ((1, 1, 5.0), (1, 1, 3.0))),
+                                       ("F32", ((1, 1, 5.0), (4, 8, 3.0)))])
+def test_flag_spectrum_and_specflags(hamd, orc, dtype, scr):
+    """The flag spectrum (the reference's flagSpec: per scrunch, the mean of
+    the grown mask over the statistics columns, accumulated over gulps) of
+    two beams on the value-table path (u8, 1 x 1 scrunches) and on the FP32
+    path (a 4 x 8 box scrunch) equals the oracle's, bit for bit, and the
+    spectral-flag counts follow casm:705-716 (specflags[i] += (int)(NTIME x
+    flagSpec[i]), the beams' spectra summed)."""
+    W = 4000                     # statistics columns 3840: the window matters
+    cfg = dict(GEOM, INPUT="FILTERBANK", INPUT_DTYPE=dtype, GULP=W, NBEAM=2, DM_MIN=0, DM_MAX=40,
+               WIDTH_MIN=1, WIDTH_MAX=8, SNR=6, SCRUNCH=list(scr))
+    h = hamd.Hella(cfg)
+    i = h.info(0)
+    specs = [np.zeros(64, np.float32), np.zeros(64, np.float32)]
+    specflags = np.zeros(64, np.int32)
+    want_flags = np.zeros(64, np.int32)
+    for g in range(i["min_gulp"] + 3):
+        parts = []
+        for b in range(2):
+            kw = dict(dtype="f32", mean=0.0, sigma=1.0) if dtype == "F32" else {}
+            x = hamd.synth(64, W, 500.0, -1.5625, 256e-6, seed=300 + b, t_offset=g * W,
+                           rfi_chan=(3 + b, 40), rfi_mult=6.0, burst_period=1700,
+                           burst_phase=100 * b, burst_len=6, burst_amp=5.0,
+                           pulses=[(0.4, 30.0, 1.0, 40.0)], **kw)
+            parts.append(x)
+            ft = orc.ingest_f32_tf(x) if dtype == "F32" else orc.ingest_u8_tf(x)
+            orc.flag(ft, -0.05, 0.15, -1, -1, scr, spec=specs[b])
+        h.process_gulp(np.concatenate([p.ravel() for p in parts]))
+        for b in range(2):
+            np.testing.assert_array_equal(h.flag_spectrum(b), specs[b])
+        if g >= i["min_gulp"]:
+            h.accumulate_specflags(specflags)
+            tot = specs[0].astype(np.float64) + specs[1].astype(np.float64)
+            want_flags += (i["ntime"] * tot).astype(np.int32)
+            np.testing.assert_array_equal(specflags, want_flags)
+    assert specs[0].max() > 0 and specflags.max() > 0
+    h.close()
+
+
 def _spread4(delta, live):
     d = delta[:, live]
     return int(max((d[j + 3] - d[j]).max() for j in range(d.shape[0] - 3)))

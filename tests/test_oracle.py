@@ -258,6 +258,42 @@ def test_scrunch_flag_against_padded_form(orc, ts, fs, f1, f2):
     np.testing.assert_array_equal(cnt, G.sum(axis=1))
 
 
+@pytest.mark.parametrize("ts,fs,W", [(1, 1, 300), (3, 2, 300), (4, 1, 48), (2, 5, 520)])
+def test_flag_spectrum_is_the_grown_mask_mean(orc, ts, fs, W):
+    """flagSpec (flagger.cu:333-334): each scrunch adds, per channel, the mean
+    of G = box mean of the threshold mask over the statistics columns (the
+    first 256*floor(W/256), all W below 256); written here as the masked
+    pixels of each edge-padded box counted directly."""
+    rng = np.random.default_rng(ts * 100 + fs * 10 + W)
+    nch = 9
+    x = (rng.integers(-8, 9, (nch, W)) / 4.0).astype(np.float32)
+    x[4, 10:14] = 12.0
+    x[7, W - 20:] = -9.0          # flags past the statistics columns do not count
+    thresh = 3.0
+    S = _box_mean_padded(x, ts, fs)
+    thr = np.float32(thresh / math.sqrt(ts * fs))
+    M = ((S > thr) | (S < -thr)).astype(np.int64)
+    ax, ay = ts // 2, fs // 2
+    Mp = np.pad(M, ((ay, fs - 1 - ay), (ax, ts - 1 - ax)), mode="edge")
+    N = np.zeros((nch, W), np.int64)
+    for i in range(fs):
+        for j in range(ts):
+            N += Mp[i:i + nch, j:j + W]
+    wp = (W // 256) * 256 or W
+    want = (N[:, :wp].sum(axis=1) / (ts * fs) / wp).astype(np.float32)
+    assert want.max() > 0
+    _, _, spec = orc.scrunch_flag(x, ts, fs, thresh, want_spec=True)
+    np.testing.assert_array_equal(spec, want)
+    # a second scrunch accumulates in float
+    spec2 = spec.copy()
+    L = orc.L
+    y = np.array(x, np.float32, copy=True)
+    cnt = np.zeros(nch, np.uint32)
+    L.orc_scrunch_flag(y.ctypes.data, nch, W, ts, fs, thresh, -1, -1, cnt.ctypes.data,
+                       spec2.ctypes.data)
+    np.testing.assert_array_equal(spec2, (want + want).astype(np.float32))
+
+
 def test_ts_correct(orc):
     x = np.vstack([np.full(16, 2.0), np.full(16, 4.0)]).astype(np.float32)
     y, ts = orc.ts_correct(x)
