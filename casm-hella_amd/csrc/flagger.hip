@@ -31,6 +31,32 @@ __device__ __forceinline__ uint8_t quant(float x)
   return (uint8_t)v;
 }
 
+// The normalisation of a pass: the plane's mean and 1/sigma (normalize_data,
+// flagger.cu:198-214), or with robust statistics the channel's median m and
+// scale s (0 when s is 0: a channel whose samples are mostly one value)
+struct NormP {
+  double mu, inv;
+  float m, s;
+  int robust;
+};
+
+__device__ __forceinline__ NormP norm_params(const FlagBeamDev &B, int c)
+{
+  const FScal sc = *B.scal;
+  NormP n{sc.mu, sc.inv, 0.f, 0.f, B.robust};
+  if (B.robust) {
+    n.m = B.rstat[2 * c];
+    n.s = B.rstat[2 * c + 1];
+  }
+  return n;
+}
+
+__device__ __forceinline__ float normv(float x, const NormP &n)
+{
+  if (n.robust) return n.s > 0.f ? (float)(((double)x - (double)n.m) * (1.0 / (double)n.s)) : 0.f;
+  return (float)(((double)x - n.mu) * n.inv);
+}
+
 // ---- ingest: input gulp -> X (transpose_input, src/hella/src/transpose.cu:23-42)
 __global__ __launch_bounds__(256) void k_ingest(const FlagBeamDev *__restrict__ beams, int W)
 {
@@ -79,7 +105,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
   float *row = B.X + (int64_t)c * B.xs;
   const int base = blk * FLAG_BLOCK;
   __shared__ double sh[2 * 128];
-  const FScal sc = *B.scal;
+  const NormP np = norm_params(B, c);
   float bpv = 0.f;
   bool bad = false;
   if (op == FOP_DIVBP || op == FOP_DIVBP_TS) bpv = B.bp[c];
@@ -97,10 +123,10 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
     float x = row[t];
     switch (op) {
       case FOP_DIVBP: x = bpv > 0.f ? x / bpv : 0.f; row[t] = x; break;
-      case FOP_NORM: x = (float)(((double)x - sc.mu) * sc.inv); row[t] = x; break;
+      case FOP_NORM: x = normv(x, np); row[t] = x; break;
       case FOP_SPEC: x = bad ? 1.f : x + 1.f; row[t] = x; break;
       case FOP_NORM_QUANT: {
-        x = (float)(((double)x - sc.mu) * sc.inv);
+        x = normv(x, np);
         int64_t col = B.col0 + t;
         if (col >= B.ring_len) col -= B.ring_len;
         B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
@@ -108,7 +134,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
       }
       case FOP_NORM_FLAG1: {
         // box of 1 x 1: the box mean is the sample itself (kernel 1.0f)
-        x = (float)(((double)x - sc.mu) * sc.inv);
+        x = normv(x, np);
         const bool g = (x > a0) || (x < a1);
         cnt += g ? 1u : 0u;
         cntw += (g && t < B.wp) ? 1u : 0u;
@@ -355,16 +381,16 @@ __global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__r
 // The same operations as k_apply, applied to one value of channel c; the
 // statistics (bp, mu, inv) are those k_apply would read in the same pass.
 __device__ __forceinline__ float value_op(int op, float x, float bpv, bool bad, bool fixed,
-                                          const FScal &sc, float a0, float a1, bool &g)
+                                          const NormP &np, float a0, float a1, bool &g)
 {
   g = false;
   switch (op) {
     case FOP_DIVBP:
     case FOP_DIVBP_TS: return bpv > 0.f ? x / bpv : 0.f;
-    case FOP_NORM: return (float)(((double)x - sc.mu) * sc.inv);
+    case FOP_NORM: return normv(x, np);
     case FOP_SPEC: return bad ? 1.f : x + 1.f;
     case FOP_NORM_FLAG1: {
-      x = (float)(((double)x - sc.mu) * sc.inv);
+      x = normv(x, np);
       g = (x > a0) || (x < a1);
       if (g || fixed) x = 0.f;
       return x + 1.f;
@@ -467,13 +493,13 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       *reinterpret_cast<float4 *>(&tsb[4 * (l + NT * j)]) = v;
     }
   }
-  const FScal sc = *B.scal;
+  const NormP np = norm_params(B, c);
   const float bpv = B.bp[c];
   const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
   const bool fixed = c >= B.flag1 && c < B.flag2;
   {
     bool g;
-    const float y = value_op(op, x0, bpv, bad, fixed, sc, a0, a1, g);
+    const float y = value_op(op, x0, bpv, bad, fixed, np, a0, a1, g);
     tab[l] = y;
     flg[l] = g ? 1 : 0;
     if (blk0 == 0 && lout >= 0 && lout != lin) B.lut[((int64_t)lout * B.nch + c) * 256 + l] = y;
@@ -512,7 +538,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       if (to_ring) {
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
-          x[k] = (float)(((double)x[k] - sc.mu) * sc.inv);
+          x[k] = normv(x[k], np);
           int64_t col = B.col0 + base + l + NT * k;
           if (col >= B.ring_len) col -= B.ring_len;
           B.ring[(int64_t)c * B.ring_stride + col] = quant(x[k]);
@@ -538,7 +564,7 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       cntw += t < B.wp ? flg[v] : 0u;
       if (TSD) x = x / tsb[l + NT * k];
       if (to_ring) {
-        x = (float)(((double)x - sc.mu) * sc.inv);
+        x = normv(x, np);
         int64_t col = B.col0 + t;
         if (col >= B.ring_len) col -= B.ring_len;
         B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
@@ -574,12 +600,12 @@ __global__ __launch_bounds__(NT) void k_table(const FlagBeamDev *__restrict__ be
   if (c >= B.nch) return;
   const int l = threadIdx.x;
   const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
-  const FScal sc = *B.scal;
+  const NormP np = norm_params(B, c);
   const float bpv = B.bp[c];
   const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
   const bool fixed = c >= B.flag1 && c < B.flag2;
   bool g;
-  B.lut[((int64_t)lout * B.nch + c) * 256 + l] = value_op(op, x0, bpv, bad, fixed, sc, a0, a1, g);
+  B.lut[((int64_t)lout * B.nch + c) * 256 + l] = value_op(op, x0, bpv, bad, fixed, np, a0, a1, g);
 }
 
 // k_ts_sum on the value-table path, in two steps.  k_ts_part: workgroup
@@ -646,6 +672,95 @@ __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict
   B.ts[t] = (float)(p[0] / (double)B.nch);
 }
 
+// ---- robust statistics (FLAG_STATS ROBUST) ------------------------------
+// Key of a float whose unsigned order is the float order (-0 before +0, NaNs
+// past the infinities), and back.
+__device__ __forceinline__ uint32_t fkey(float f)
+{
+  const uint32_t b = __float_as_uint(f);
+  return b ^ ((b >> 31) ? 0xffffffffu : 0x80000000u);
+}
+__device__ __forceinline__ float kval(uint32_t k)
+{
+  return __uint_as_float((k >> 31) ? (k ^ 0x80000000u) : ~k);
+}
+
+// The element of rank k (0-based, ascending key order) among the keys of
+// value(t), t < n: a radix select, 8 bits a pass from the top, each pass
+// counting in an LDS histogram the keys that share the digits chosen so far
+// and keeping the bucket where rank k falls (an exclusive scan of the 256
+// counts by the 256 threads).
+template <class F>
+__device__ uint32_t rselect(int n, uint32_t k, F value, uint32_t *hist, uint32_t *pick)
+{
+  const int l = threadIdx.x;
+  uint32_t prefix = 0, mask = 0;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[l] = 0u;
+    __syncthreads();
+    for (int t = l; t < n; t += NT) {
+      const uint32_t key = fkey(value(t));
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    // inclusive scan of the counts (Hillis-Steele over 256 entries)
+    uint32_t h = hist[l], v = h;
+    for (int off = 1; off < NT; off <<= 1) {
+      __syncthreads();
+      const uint32_t u = l >= off ? hist[l - off] : 0u;
+      __syncthreads();
+      v += u;
+      hist[l] = v;
+    }
+    __syncthreads();
+    const uint32_t before = v - h;  // keys of lower buckets
+    if (h > 0 && k >= before && k < v) {
+      pick[0] = (uint32_t)l;
+      pick[1] = k - before;
+    }
+    __syncthreads();
+    prefix |= pick[0] << shift;
+    mask |= 0xffu << shift;
+    k = pick[1];
+    __syncthreads();
+  }
+  return prefix;
+}
+
+// Per channel (one workgroup) the median m of the statistics columns and
+// the scale s = 1.4826 * median(|x - m|) (x - m in float), both lower medians
+// (rank (wp - 1) / 2).  SRC 0: the FP32 plane; 1: the uint8 plane through
+// table slot lin; 2: the same divided by ts[t] (the final normalisation of
+// the value-table path).
+template <int SRC>
+__global__ __launch_bounds__(NT) void k_robust(const FlagBeamDev *__restrict__ beams, int lin)
+{
+  const FlagBeamDev B = beams[blockIdx.z];
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int l = threadIdx.x;
+  __shared__ uint32_t hist[NT];
+  __shared__ uint32_t pick[2];
+  __shared__ float tab[256];
+  if (SRC >= 1) tab[l] = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  __syncthreads();
+  const float *row = B.X + (int64_t)c * B.xs;
+  const uint8_t *row8 = B.X8 + (int64_t)c * B.xs;
+  auto value = [&](int t) -> float {
+    if (SRC == 0) return row[t];
+    const float x = tab[row8[t]];
+    return SRC == 2 ? x / B.ts[t] : x;
+  };
+  const int n = B.wp;
+  const uint32_t k = (uint32_t)(n - 1) / 2u;
+  const float m = kval(rselect(n, k, value, hist, pick));
+  const float d = kval(rselect(n, k, [&](int t) { return fabsf(value(t) - m); }, hist, pick));
+  if (l == 0) {
+    B.rstat[2 * c] = m;
+    B.rstat[2 * c + 1] = (float)(1.4826 * (double)d);
+  }
+}
+
 __global__ void k_zero_u32(uint32_t *p, int n)
 {
   const int i = blockIdx.x * NT + threadIdx.x;
@@ -676,6 +791,19 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
     hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W, box);
   };
   auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
+  // a division pass (by the bandpass, or also by the time series) and the
+  // statistics of its result: the plane's mean and sigma from the pass's
+  // partial sums, or with robust statistics each channel's median and MAD
+  auto divide_stats = [&](int op) {
+    if (P.robust) {
+      apply(op, FPART_NONE, 0);
+      hipLaunchKernelGGL(k_robust<0>, dim3(1u, (unsigned)max_nch, (unsigned)nbeam), dim3(NT), 0, s,
+                         d_beams, -1);
+    } else {
+      apply(op, FPART_SUM2, wp);
+      stats();
+    }
+  };
 
   hipLaunchKernelGGL(k_ingest, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                     (unsigned)nbeam),
@@ -684,8 +812,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   // channel means of the normalised data, spectral flag
   apply(FOP_NONE, FPART_SUM, n32);
   bp();
-  apply(FOP_DIVBP, FPART_SUM2, wp);
-  stats();
+  divide_stats(FOP_DIVBP);
   apply(FOP_NORM, FPART_SUM, n32);
   bp();
   apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
@@ -693,8 +820,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   // configured scrunches (flagger.cu:365-384)
   for (int i = 0; i < P.nscrunch; ++i) {
     const FlagScrunch &S = P.scrunch[i];
-    apply(FOP_DIVBP, FPART_SUM2, wp);
-    stats();
+    divide_stats(FOP_DIVBP);
     const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
     const float nthr = (float)(-1. * (double)thr);
     if (S.tscrunch == 1 && S.fscrunch == 1) {
@@ -713,8 +839,7 @@ void FlagRunner::run(const FlagBeamDev *d_beams, int nbeam, int max_nch, int W,
   // ts_correct, normalize_data
   hipLaunchKernelGGL(k_ts_sum, dim3((unsigned)((W + 63) / 64), 1, (unsigned)nbeam), dim3(64 * TS_PARTS), 0,
                      s, d_beams, W);
-  apply(FOP_DIVBP_TS, FPART_SUM2, wp);
-  stats();
+  divide_stats(FOP_DIVBP_TS);
   // transpose_output_handler (transpose.cu:120-145): quantise into the ring
   apply(FOP_NORM_QUANT, FPART_NONE, 0);
   HAMD_CHECK(hipGetLastError());
@@ -759,22 +884,36 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
     hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W, box);
   };
   auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
+  const dim3 gchan(1u, (unsigned)max_nch, (unsigned)nbeam);
+  // bandpass division and the statistics of its result: a pass with the
+  // partial sums of the plane's mean and sigma, or with robust statistics
+  // the division as a table and each channel's median and MAD through it
+  auto divide_stats = [&]() {
+    if (P.robust) {
+      const int nxt = cur == 0 ? 1 : 0;
+      hipLaunchKernelGGL(k_table, gchan, dim3(NT), 0, s, d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur,
+                         nxt);
+      cur = nxt;
+      hipLaunchKernelGGL(k_robust<1>, gchan, dim3(NT), 0, s, d_beams, cur);
+    } else {
+      apply(FOP_DIVBP, FPART_SUM2, wp);
+      stats();
+    }
+  };
 
   hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                        (unsigned)nbeam),
                      dim3(256), 0, s, d_beams, W);
   apply(FOP_NONE, FPART_SUM, n32);
   bp();
-  apply(FOP_DIVBP, FPART_SUM2, wp);
-  stats();
+  divide_stats();
   apply(FOP_NORM, FPART_SUM, n32);
   bp();
   apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
   bp();
   for (int i = 0; i < P.nscrunch; ++i) {
     const FlagScrunch &S = P.scrunch[i];
-    apply(FOP_DIVBP, FPART_SUM2, wp);
-    stats();
+    divide_stats();
     const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
     const float nthr = (float)(-1. * (double)thr);
     apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
@@ -789,8 +928,12 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
                      dim3(NT), 0, s, d_beams, W, cur);
   hipLaunchKernelGGL(k_ts_combine, dim3((unsigned)((W + NT - 1) / NT), 1, (unsigned)nbeam),
                      dim3(NT), 0, s, d_beams, W);
-  apply(FOP_TSDIV, FPART_SUM2, wp);
-  stats();
+  if (P.robust) {
+    hipLaunchKernelGGL(k_robust<2>, gchan, dim3(NT), 0, s, d_beams, cur);
+  } else {
+    apply(FOP_TSDIV, FPART_SUM2, wp);
+    stats();
+  }
   apply(FOP_NORM_QUANT, FPART_NONE, 0);
   HAMD_CHECK(hipGetLastError());
 }

@@ -55,6 +55,12 @@ struct FlagBeamDev {
   float *fspec;              // [nch]
   unsigned long long *gsum;  // [nch]
   int wp;                    // statistics columns: 256 * floor(W / 256) (W if W < 256)
+  // robust statistics (FLAG_STATS ROBUST): every normalisation uses the
+  // channel's own median m and scale s = 1.4826 * MAD over the statistics
+  // columns, rstat[2c] = m, rstat[2c + 1] = s, instead of the plane's mean
+  // and standard deviation
+  int robust;
+  float *rstat;        // [nch][2]
   float *ts;           // [W] time series of the final pass
   uint8_t *mask;       // [nch][xs] threshold mask scratch
   // value-table path (uint8 input, 1 x 1 scrunches only): the input as a
@@ -74,6 +80,7 @@ struct FlagParams {
   float spec_min = -0.05f, spec_max = 0.15f;
   int nscrunch = 0;
   FlagScrunch scrunch[8];
+  bool robust = false;  // per-channel median / MAD normalisation (opt-in)
 };
 
 struct FlagRunner {

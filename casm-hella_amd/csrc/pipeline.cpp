@@ -70,6 +70,12 @@ bool Config::parse(const std::string &text, std::string &err, bool beam_only)
       else if (v == "FILE") { input_fmt = FLAG_IN_U8_TF; oneshot = true; }
       else if (v == "FILTERBANK") { input_fmt = FLAG_IN_U8_TF; oneshot = false; }
       else { err = "unsupported INPUT " + v; return false; }
+    } else if (k == "FLAG_STATS") {
+      // the flagger's normalisation: MEAN (the reference's plane mean and
+      // sigma) or ROBUST (each channel's median and 1.4826 x MAD)
+      if (v == "MEAN") flag.robust = false;
+      else if (v == "ROBUST") flag.robust = true;
+      else { err = "unsupported FLAG_STATS " + v; return false; }
     } else if (k == "INPUT_DTYPE") {
       if (v == "U8") input_fmt = FLAG_IN_U8_TF;
       else if (v == "F32") input_fmt = FLAG_IN_F32_TF;
@@ -240,7 +246,8 @@ static WorkSizes work_sizes(const WorkDims &d)
   z.D = (size_t)d.ndm * d.dstride * 4;
   z.planes = d.planes ? (size_t)d.nw * d.rows * round_up((size_t)d.nout, 64) * 4 : 0;
   z.fpart = (size_t)d.nch * nblk * 8;
-  z.fts = (size_t)d.W * 4;
+  // the final pass's time series, then the per-channel robust statistics
+  z.fts = round_up((size_t)d.W, 64) * 4 + (size_t)d.nch * 8;
   z.spart = ncnt * 8;
   z.scnt = ncnt * 4;
   z.stiles = ((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4;
@@ -487,6 +494,8 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
                                                   flag_state_gsum_offset(B.nch));
   f.wp = W >= 256 ? (W / 256) * 256 : W;
   f.ts = w.f_ts.as<float>();
+  f.robust = cfg.flag.robust ? 1 : 0;
+  f.rstat = w.f_ts.as<float>() + round_up((size_t)W, 64);
   f.mask = w.mask.as<uint8_t>();
   if (flag_value_tables(cfg.input_fmt, cfg.flag)) {
     f.X8 = w.X.as<uint8_t>();

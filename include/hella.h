@@ -82,6 +82,9 @@ typedef struct {
  *   search:    SEARCH BOXCAR|HELLA [BOXCAR]  MAX_GIANTS [10000]  CAND_CAP [1048576]
  *     CLUSTER_DT [64] CLUSTER_DDM [16]  TIME_LIMIT [100.1] (s, casm:723)
  *   input:     INPUT_DTYPE U8|F32|F16 (default from INPUT)
+ *   flagger:   FLAG_STATS MEAN|ROBUST [MEAN]  (ROBUST: every normalisation uses
+ *     each channel's median and 1.4826 x MAD over the statistics columns
+ *     instead of the plane's mean and sigma; no reference counterpart)
  * Unknown keys are ignored, as in the reference. */
 hella_status hella_create(const char *config, hella_ctx *out);
 void hella_destroy(hella_ctx ctx);

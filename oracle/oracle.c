@@ -236,6 +236,56 @@ void orc_normalize(float *x, int nch, int W)
   for (size_t i = 0; i < n; ++i) x[i] = (float)(((double)x[i] - m) * inv);
 }
 
+static inline uint32_t fkey(float f)
+{
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  return b ^ ((b >> 31) ? 0xffffffffu : 0x80000000u);
+}
+
+static inline float kval(uint32_t k)
+{
+  uint32_t b = (k >> 31) ? (k ^ 0x80000000u) : ~k;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+static int cmp_u32(const void *a, const void *b)
+{
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : (x > y);
+}
+
+void orc_normalize_robust(float *x, int nch, int W, float *ms)
+{
+  const size_t wp = stats_width((size_t)W);
+  const size_t k = (wp - 1) / 2;
+#pragma omp parallel
+  {
+    uint32_t *key = (uint32_t *)malloc((wp ? wp : 1) * sizeof(uint32_t));
+#pragma omp for schedule(static)
+    for (int c = 0; c < nch; ++c) {
+      float *row = x + (size_t)c * W;
+      for (size_t t = 0; t < wp; ++t) key[t] = fkey(row[t]);
+      qsort(key, wp, sizeof(uint32_t), cmp_u32);
+      const float m = kval(key[k]);
+      for (size_t t = 0; t < wp; ++t) key[t] = fkey(fabsf(row[t] - m));
+      qsort(key, wp, sizeof(uint32_t), cmp_u32);
+      const float d = kval(key[k]);
+      const float sc = (float)(1.4826 * (double)d);
+      if (ms) {
+        ms[2 * c] = m;
+        ms[2 * c + 1] = sc;
+      }
+      const double inv = 1.0 / (double)sc;
+      for (int t = 0; t < W; ++t)
+        row[t] = sc > 0.f ? (float)(((double)row[t] - (double)m) * inv) : 0.f;
+    }
+    free(key);
+  }
+}
+
 /* bandpass_flag tail (src/flagger.cu:226-238) */
 void orc_spec_flag(float *x, int nch, int W, float smin, float smax)
 {
@@ -394,19 +444,22 @@ void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride
   int nch = cfg->nch, W = cfg->W;
   /* bandpass_flag (flagger.cu:216-242) */
   bandpass_correct(x, nch, W);
-  orc_normalize(x, nch, W);
+  if (cfg->robust) orc_normalize_robust(x, nch, W, NULL);
+  else orc_normalize(x, nch, W);
   orc_spec_flag(x, nch, W, cfg->spec_min, cfg->spec_max);
   /* configured scrunches (flagger.cu:365-384) */
   for (int s = 0; s < cfg->nscrunch; ++s) {
     bandpass_correct(x, nch, W);
-    orc_normalize(x, nch, W);
+    if (cfg->robust) orc_normalize_robust(x, nch, W, NULL);
+    else orc_normalize(x, nch, W);
     orc_scrunch_flag(x, nch, W, cfg->scrunch[s].tscrunch, cfg->scrunch[s].fscrunch,
                      cfg->scrunch[s].thresh, cfg->flag1, cfg->flag2, flagcount, flagspec);
   }
   /* apply_scrunch(8,8,100,flag=0,ts=1) (flagger.cu:385-401) */
   bandpass_correct(x, nch, W);
   orc_ts_correct(x, nch, W, NULL);
-  orc_normalize(x, nch, W);
+  if (cfg->robust) orc_normalize_robust(x, nch, W, NULL);
+  else orc_normalize(x, nch, W);
   /* transpose_output_handler (transpose.cu:120-145) */
   orc_quantize(x, nch, W, out, out_stride);
 }

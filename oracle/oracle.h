@@ -81,12 +81,21 @@ typedef struct {
   int flag1, flag2; /* channel range [flag1, flag2) zeroed by each scrunch */
   int nscrunch;
   const orc_scrunch *scrunch;
+  int robust;       /* 1: every normalisation is orc_normalize_robust */
 } orc_flag_cfg;
 
 /* stage functions on a float FT plane x[nch][W] (row stride W) */
 void orc_bandpass(const float *x, int nch, int W, float *bp);          /* calc_bandpass_new */
 void orc_divide_bp(float *x, int nch, int W, const float *bp);         /* divide_by_bp */
 void orc_normalize(float *x, int nch, int W);                          /* normalize_data */
+/* Robust per-channel normalisation (no reference counterpart; the opt-in
+ * FLAG_STATS ROBUST mode): over the statistics columns t < wp (see
+ * orc_stats) of channel c, m = the element of rank (wp-1)/2 in ascending
+ * order, d = the same of |x - m| (float subtraction), s = (float)(1.4826 * d);
+ * every sample becomes (float)(((double)x - m) * (1.0 / (double)s)), or 0
+ * when s is 0.  Ascending order is that of the floats with -0 before +0 and
+ * NaNs past the infinities.  m and s per channel go to ms (may be NULL). */
+void orc_normalize_robust(float *x, int nch, int W, float *ms);
 void orc_spec_flag(float *x, int nch, int W, float smin, float smax);  /* calc_bandpass_new + replace_data_and_add_bandpass */
 void orc_box_mean(const float *x, int nch, int W, int ts, int fs, float *out); /* npp_convolve_handler */
 /* apply_scrunch flag branch; flagspec (may be NULL) accumulates the mean of

@@ -29,7 +29,8 @@ class Scrunch(C.Structure):
 
 class FlagCfg(C.Structure):
     _fields_ = [("nch", _I), ("W", _I), ("spec_min", _F), ("spec_max", _F),
-                ("flag1", _I), ("flag2", _I), ("nscrunch", _I), ("scrunch", C.POINTER(Scrunch))]
+                ("flag1", _I), ("flag2", _I), ("nscrunch", _I), ("scrunch", C.POINTER(Scrunch)),
+                ("robust", _I)]
 
 
 class SearchCfg(C.Structure):
@@ -63,6 +64,7 @@ _d("orc_stats_f", None, _VP, _SZ, _SZ, _SZ, C.POINTER(_F), C.POINTER(_F))
 _d("orc_bandpass", None, _VP, _I, _I, _VP)
 _d("orc_divide_bp", None, _VP, _I, _I, _VP)
 _d("orc_normalize", None, _VP, _I, _I)
+_d("orc_normalize_robust", None, _VP, _I, _I, _VP)
 _d("orc_spec_flag", None, _VP, _I, _I, _F, _F)
 _d("orc_box_mean", None, _VP, _I, _I, _I, _I, _VP)
 _d("orc_scrunch_flag", None, _VP, _I, _I, _I, _I, _F, _I, _I, _VP, _VP)
@@ -190,6 +192,14 @@ def normalize(x):
     return x
 
 
+def normalize_robust(x):
+    """per-channel median / MAD normalisation; returns (plane, [nch][2] m, s)"""
+    x = np.array(x, np.float32, copy=True)
+    ms = np.zeros((x.shape[0], 2), np.float32)
+    L.orc_normalize_robust(_p(x), x.shape[0], x.shape[1], _p(ms))
+    return x, ms
+
+
 def spec_flag(x, smin, smax):
     x = np.array(x, np.float32, copy=True)
     L.orc_spec_flag(_p(x), x.shape[0], x.shape[1], smin, smax)
@@ -247,7 +257,8 @@ def ingest_f32_tf(data_tf):
     return x
 
 
-def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), spec=None):
+def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), spec=None,
+         robust=False):
     """Whole per-beam flagger on a float FT plane; returns (u8 FT, flagcount).
     spec: a float32 [nch] array that accumulates the flag spectrum."""
     x = np.array(x_ft, np.float32, copy=True)
@@ -255,7 +266,7 @@ def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), 
     sc = (Scrunch * max(1, len(scrunches)))()
     for i, (ts, fs, th) in enumerate(scrunches):
         sc[i] = Scrunch(ts, fs, th)
-    cfg = FlagCfg(nch, W, spec_min, spec_max, flag1, flag2, len(scrunches), sc)
+    cfg = FlagCfg(nch, W, spec_min, spec_max, flag1, flag2, len(scrunches), sc, 1 if robust else 0)
     out = np.zeros((nch, W), np.uint8)
     cnt = np.zeros(nch, np.uint32)
     if spec is not None:

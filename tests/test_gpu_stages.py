@@ -181,6 +181,45 @@ def test_flag_spectrum_and_specflags(hamd, orc, dtype, scr):
     h.close()
 
 
+@pytest.mark.parametrize("dtype,scr,W", [("U8", ((1, 1, 5.0), (1, 1, 3.0)), 4000),
+                                         ("U8", ((1, 1, 5.0),), 4096),
+                                         ("F32", ((1, 1, 5.0), (4, 8, 3.0)), 4000)])
+def test_robust_statistics_mode(hamd, orc, dtype, scr, W):
+    """FLAG_STATS ROBUST (per-channel median and 1.4826 x MAD in every
+    normalisation; radix selection on the device): on the value-table path
+    (u8, 1 x 1 scrunches) and on the FP32 path (a 4 x 8 box scrunch), the
+    streaming window equals the oracle's robust flagger bit for bit, and the
+    dedispersed sums and candidates follow."""
+    cfg = dict(GEOM, INPUT="FILTERBANK", INPUT_DTYPE=dtype, GULP=W, DM_MIN=0, DM_MAX=60,
+               WIDTH_MIN=1, WIDTH_MAX=8, SNR=6, SCRUNCH=list(scr), FLAG_STATS="ROBUST")
+    h = hamd.Hella(cfg)
+    i = h.info(0)
+    op = orc.Plan(64, 256e-6, 500.0, -1.5625, 0.0, 60.0, 64.0, 1.25)
+    flagged = []
+    for g in range(i["min_gulp"] + 2):
+        kw = dict(dtype="f32", mean=0.0, sigma=1.0) if dtype == "F32" else {}
+        x = hamd.synth(64, W, 500.0, -1.5625, 256e-6, seed=71, t_offset=g * W, rfi_chan=(9, 33),
+                       rfi_mult=12.0, burst_period=1500, burst_phase=300, burst_len=6,
+                       burst_amp=5.0, pulses=[(t, 40.0, 1.0, 30.0) for t in (0.6, 1.4, 2.2, 3.0)],
+                       bp_slope=0.4, **kw)
+        h.process_gulp(x)
+        ft = orc.ingest_f32_tf(x) if dtype == "F32" else orc.ingest_u8_tf(x)
+        out, _ = orc.flag(ft, -0.05, 0.15, -1, -1, scr, robust=True)
+        flagged.append(out)
+    F = np.concatenate(flagged, axis=1)
+    a0 = (i["min_gulp"] + 2) * W - i["ntime"]
+    win = F[:, a0:a0 + i["ntime"]]
+    np.testing.assert_array_equal(h.window(0), win)
+    D = h.dedisp_plane(0)
+    np.testing.assert_array_equal(D, orc.dedisperse_ft(win, op.delta, i["ntime_dd"]))
+    ref = orc.search(D, i["ntime_out"], 8, 1, orc.MODE_BOXCAR, 6.0)
+    p = h.peaks()
+    assert h.crossings(0) == ref["count"] > 0
+    for f in ("samp", "width", "dm_idx"):
+        np.testing.assert_array_equal(p[f], ref["cands"][f])
+    h.close()
+
+
 def _spread4(delta, live):
     d = delta[:, live]
     return int(max((d[j + 3] - d[j]).max() for j in range(d.shape[0] - 3)))

@@ -294,6 +294,53 @@ def test_flag_spectrum_is_the_grown_mask_mean(orc, ts, fs, W):
     np.testing.assert_array_equal(spec2, (want + want).astype(np.float32))
 
 
+@pytest.mark.parametrize("W", [100, 300, 777])
+def test_robust_normalise_brute_force(orc, W):
+    """orc_normalize_robust against numpy sorts: per channel the lower median
+    of the statistics columns, the lower median of |x - m| (float), the scale
+    1.4826 x MAD, and the normalised plane; a constant channel gives zeros"""
+    rng = np.random.default_rng(W)
+    nch = 6
+    x = (rng.standard_normal((nch, W)) * np.arange(1, nch + 1)[:, None] + 3).astype(np.float32)
+    x[2, :] = 5.0                                  # MAD 0: the channel becomes 0
+    x[3, ::7] = 1e6                                # outliers move mean and sigma, not the median
+    y, ms = orc.normalize_robust(x)
+    wp = (W // 256) * 256 or W
+    k = (wp - 1) // 2
+    for c in range(nch):
+        m = np.sort(x[c, :wp])[k]
+        d = np.sort(np.abs(x[c, :wp] - m))[k]
+        s = np.float32(1.4826 * np.float64(d))
+        assert ms[c, 0] == m and ms[c, 1] == s
+        want = np.zeros(W, np.float32) if s == 0 else \
+            ((x[c].astype(np.float64) - np.float64(m)) * (1.0 / np.float64(s))).astype(np.float32)
+        np.testing.assert_array_equal(y[c], want)
+    assert not y[2].any()
+    # closed form: the first 256 columns holding 0..255 in any order have
+    # lower median 127 and |x - 127| lower median 64 (ranks 0..127 of the
+    # deviations 0, 1, 1, 2, 2, ..., 127, 127, 128)
+    z = np.r_[rng.permutation(256), [999.0]].astype(np.float32)[None, :]
+    _, ms = orc.normalize_robust(np.repeat(z, 2, axis=0))
+    assert ms[0, 0] == 127.0 and ms[0, 1] == np.float32(1.4826 * 64.0)
+
+
+def test_robust_flag_ignores_a_loud_minority(orc):
+    """with robust statistics a channel whose few samples are huge keeps its
+    noise level after normalisation (median and MAD ignore them), where the
+    plane's sigma would be inflated; the default mode is unchanged"""
+    rng = np.random.default_rng(21)
+    nch, W = 16, 4096
+    x = (100 + 10 * rng.standard_normal((nch, W))).astype(np.float32)
+    x[5, ::50] = 1e5
+    y, ms = orc.normalize_robust(x)
+    clean = np.delete(np.arange(W), np.arange(0, W, 50))
+    assert 0.8 < np.std(y[5, clean]) < 1.25
+    assert abs(ms[5, 0] - 100) < 2 and abs(ms[5, 1] - 10) < 1.5
+    a, _ = orc.flag(x, robust=False)
+    b, _ = orc.flag(x, robust=True)
+    assert not np.array_equal(a, b)
+
+
 def test_ts_correct(orc):
     x = np.vstack([np.full(16, 2.0), np.full(16, 4.0)]).astype(np.float32)
     y, ts = orc.ts_correct(x)
