@@ -384,6 +384,44 @@ def test_ingest_transposes(orc):
     np.testing.assert_array_equal(orc.ingest_f32_tf(b), b.T)
 
 
+@pytest.mark.parametrize("scr,robust", [((), False), (((1, 1, 5.0),), False),
+                                        (((1, 1, 4.0), (4, 2, 3.0)), False),
+                                        (((2, 3, 3.5),), True)])
+def test_flag_is_the_stages_in_reference_order(orc, scr, robust):
+    """orc_flag equals its individually pinned stages composed in the order
+    of fast_flagger (flagger.cu:345-411): bandpass_flag = bandpass, divide,
+    normalise, spectral flag (flagger.cu:216-242); per scrunch bandpass,
+    divide, normalise, flag (flagger.cu:365-384); then bandpass, divide,
+    ts_correct, normalise (apply_scrunch(8,8,100,0,1), flagger.cu:385-401);
+    quantise (transpose.cu:65-75).  Dropping or reordering a stage changes
+    the bytes."""
+    rng = np.random.default_rng(33)
+    nch, W = 24, 1000
+    x = (50 + 5 * rng.standard_normal((nch, W)) * np.linspace(0.5, 2, nch)[:, None])
+    x[3] += 40 * rng.standard_normal(W)
+    x[:, 300:305] += 30
+    x = x.astype(np.float32)
+    norm = (lambda y: orc.normalize_robust(y)[0]) if robust else orc.normalize
+
+    def stages(y, drop=None):
+        y = orc.divide_bp(y, orc.bandpass(y))
+        y = norm(y)
+        y = orc.spec_flag(y, -0.05, 0.15)
+        for ts, fs, th in scr:
+            y = orc.divide_bp(y, orc.bandpass(y))
+            y = norm(y)
+            y, _ = orc.scrunch_flag(y, ts, fs, th)
+        y = orc.divide_bp(y, orc.bandpass(y))
+        if drop != "ts":
+            y, _ = orc.ts_correct(y)
+        y = norm(y)
+        return orc.quantize(y)
+
+    got, _ = orc.flag(x, -0.05, 0.15, -1, -1, scr, robust=robust)
+    np.testing.assert_array_equal(got, stages(x))
+    assert not np.array_equal(got, stages(x, drop="ts"))
+
+
 def test_flag_removes_bandpass_and_flags_rfi(orc):
     rng = np.random.default_rng(12)
     nch, W = 32, 4096
