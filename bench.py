@@ -193,7 +193,8 @@ def serve_saturation(hamd, pool, mg, nbeam, nchunk, **opts):
             "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99)}
 
 
-def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opts):
+def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, budget=SERVE_BUDGET,
+                   dwell_s=20.0, **opts):
     """nbeam warm beams stream chunks on the real-time cadence (random phase);
     beams leave after exponential dwell (mean 20 s) and a new beam joins in
     their place, filling its window before it is searched.  Latency = chunk
@@ -201,7 +202,7 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opt
     segment_s seconds through the same server (a progress line after each)."""
     rng = np.random.default_rng(seed)
     chunk_s = SERVE_CHUNK * TSAMP
-    srv = hamd.Server(serve_config(), hbm_budget=SERVE_BUDGET, **opts)
+    srv = hamd.Server(serve_config(), hbm_budget=budget, **opts)
     P = len(pool)
     ring = hamd.Ring(256, pool[0].nbytes, f"NCHAN {NCH}\n")
     warm = [(0.0, b, (3 * b + k) % P, k, 0) for k in range(mg) for b in range(nbeam)]
@@ -215,7 +216,7 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opt
     for slot in range(nbeam):
         # a warm beam continues its stream; when it leaves, a new beam joins
         bid, seq, t = slot, mg, float(rng.uniform(0, chunk_s))
-        leave = float(rng.exponential(20.0))
+        leave = float(rng.exponential(dwell_s)) if dwell_s > 0 else float("inf")
         while t < seconds:
             last = t + chunk_s > leave
             ev.append((t, bid, (5 * bid + seq) % P, seq, hamd.RING_LAST if last else 0))
@@ -224,7 +225,7 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opt
             if last:
                 bid, seq = next_id, 0
                 next_id += 1
-                leave = t + float(rng.exponential(20.0))
+                leave = t + float(rng.exponential(dwell_s))
     ev.sort(key=lambda e: (e[0], e[1]))
     t0 = time.perf_counter()
     late, seg0 = 0.0, 0.0
@@ -250,7 +251,8 @@ def serve_realtime(hamd, pool, mg, nbeam, seconds, seed=4, segment_s=60.0, **opt
             "p99_below_chunk": bool(srch) and pct(srch, 99) < chunk_s * 1e3,
             "writer_max_late_s": late, "max_resident": st["max_resident"],
             "beams_waited": st["beams_waited"], "gpu_busy_frac": st["busy_s"] / max(st["wall_s"], 1e-9),
-            "work_set_GB": st["work_bytes"] / 2**30}
+            "work_set_GB": st["work_bytes"] / 2**30, "hbm_budget_GB": budget / 2**30,
+            "resident_GB": st["peak_resident_bytes"] / 2**30}
 
 
 def cpu_model():
@@ -335,6 +337,13 @@ def main():
                     help="length of the real-time run (0 skips it)")
     ap.add_argument("--serve-rt-segment", type=float, default=60.0,
                     help="the real-time stream is replayed in segments of this length")
+    ap.add_argument("--no-saturation", action="store_true",
+                    help="skip the saturation runs of the streaming server")
+    ap.add_argument("--sustain-beams", type=int, default=1216,
+                    help="also stream this many beams at the real cadence (beams sustained)")
+    ap.add_argument("--sustain-seconds", type=float, default=10.0)
+    ap.add_argument("--sustain-budget-gb", type=int, default=240,
+                    help="HBM admission budget of the sustained run")
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--serve-lanes", type=int, default=2)
     ap.add_argument("--uneven", type=int, default=0,
@@ -493,8 +502,13 @@ def main():
         "crossings_last_step": crossings,
         "pcie_inclusive": pcie,
         "realtime_factor": chunk_s / t_plain,
-        "realtime_beams": {"plain": int(chunk_s / t_plain),
-                           "batched": int(nb * chunk_s / t_batch), "batched_beams_run": nb},
+        # throughput bounds of the production chunk (chunk duration over the
+        # step time), not sustained counts; the HBM bound: windows of this
+        # configuration that fit the 288 GB next to one work set
+        "realtime_beams": {"plain_step_bound": int(chunk_s / t_plain),
+                           "batched_step_bound": int(nb * chunk_s / t_batch),
+                           "batched_beams_run": nb,
+                           "hbm_bound": int((288e9 - 1.0e9) // (NCH * info["ntime"]))},
         "latency_ms": {"plain": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
                        "batched": {"p50": pct(blat, 50), "p99": pct(blat, 99), "beams": nb}},
         "stages_ms": stages,
@@ -520,9 +534,19 @@ def main():
     }
     if not args.no_serve:
         out["serving"] = serving(hamd, args, rfi)
-        out["realtime_beams"]["serving_plain"] = out["serving"]["saturation"]["plain"]["realtime_beams"]
-        out["realtime_beams"]["serving_batched"] = \
-            out["serving"]["saturation"]["batched"]["realtime_beams"]
+        sv = out["serving"]
+        if sv["saturation"]:
+            out["realtime_beams"]["serving_plain_capacity"] = sv["saturation"]["plain"]["realtime_beams"]
+            out["realtime_beams"]["serving_batched_capacity"] = \
+                sv["saturation"]["batched"]["realtime_beams"]
+        # beams sustained (the largest count streamed at the real cadence
+        # with p99 chunk-to-candidate latency below the chunk duration), on
+        # the streaming configuration's 8192-sample chunks
+        sus = sv.get("sustained")
+        if sus:
+            out["realtime_beams"]["sustained"] = sus["beams"] if sus["p99_below_chunk"] else None
+            out["realtime_beams"]["sustained_p99_ms"] = sus["latency_ms_p99"]
+            out["realtime_beams"]["sustained_chunk_ms"] = sv["chunk_s"] * 1e3
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hamd, host_chunks[0], args.cpu_rows)
     print(json.dumps(out), flush=True)
@@ -675,6 +699,8 @@ def serving(hamd, args, rfi):
     sat = {}
     for mode, opts in (("plain", dict(plain=True)),
                        ("batched", dict(max_batch=args.serve_batch, lanes=args.serve_lanes))):
+        if args.no_saturation:
+            break
         sat[mode] = serve_saturation(hamd, pool, mg, args.serve_beams, args.serve_chunks, **opts)
         print(f"# serving {mode}: {sat[mode]['chunks_per_s']:.0f} chunks/s -> "
               f"{sat[mode]['realtime_beams']} real-time beams", file=sys.stderr, flush=True)
@@ -685,14 +711,27 @@ def serving(hamd, args, rfi):
                             lanes=args.serve_lanes)
         print(f"# serving real time: {rt['beams']} beams, p50 {rt['latency_ms_p50']} ms, "
               f"p99 {rt['latency_ms_p99']} ms", file=sys.stderr, flush=True)
+    # beams sustained: the real-time cadence with as many beams as the HBM
+    # holds and the GPU keeps up with; sustained when p99 < the chunk
+    sus = None
+    if args.sustain_beams > 0:
+        # every beam stays (no departures): every chunk of every beam is searched
+        sus = serve_realtime(hamd, pool, mg, args.sustain_beams, args.sustain_seconds,
+                             segment_s=args.serve_rt_segment, budget=args.sustain_budget_gb << 30,
+                             dwell_s=0.0, max_batch=args.serve_batch, lanes=args.serve_lanes)
+        print(f"# serving sustained: {sus['beams']} beams, p50 {sus['latency_ms_p50']} ms, "
+              f"p99 {sus['latency_ms_p99']} ms, GPU busy {sus['gpu_busy_frac']:.2f}",
+              file=sys.stderr, flush=True)
     return {"workload": f"streaming: 1024 ch, {SERVE_CHUNK}-sample chunks "
                         f"({SERVE_CHUNK * TSAMP:.3f} s), DM 0-2000, widths 1..256, "
                         f"HBM budget {SERVE_BUDGET >> 30} GB, pulses 0.05/s per beam, "
                         "bandpass slope drifting between -0.6 and +0.6",
             "chunk_s": SERVE_CHUNK * TSAMP, "window_fill_chunks": mg,
             "saturation": sat,
-            "batched_over_plain": sat["batched"]["chunks_per_s"] / sat["plain"]["chunks_per_s"],
-            "realtime": rt, "max_batch": args.serve_batch, "lanes": args.serve_lanes}
+            "batched_over_plain": (sat["batched"]["chunks_per_s"] / sat["plain"]["chunks_per_s"]
+                                   if sat else None),
+            "realtime": rt, "sustained": sus, "max_batch": args.serve_batch,
+            "lanes": args.serve_lanes}
 
 
 if __name__ == "__main__":
