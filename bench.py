@@ -278,10 +278,8 @@ def cpu_baseline(hamd, chunk_tf, sample_rows=0):
     plan = orc.Plan(NCH, TSAMP, 500.0, FOFF, 0.0, 2000.0, 64.0, 1.25)
     ndm = plan.dms.size
     t0 = time.perf_counter()
-    x = orc.ingest_u8_tf(chunk_tf)
-    flagged, _ = orc.flag(x, -0.05, 0.15, -1, -1, ((1, 1, 6.0),))
+    flagged, _ = orc.flag_u8(chunk_tf, -0.05, 0.15, -1, -1, ((1, 1, 6.0),))
     t_flag = time.perf_counter() - t0
-    del x
     # window: the flagged chunk repeated stands in for chunk + tail (the cost
     # does not depend on the bytes)
     ntime_dd = CHUNK + MAXW

@@ -672,6 +672,128 @@ __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict
   B.ts[t] = (float)(p[0] / (double)B.nch);
 }
 
+// ---- byte-histogram path (value-table path with mean statistics) ----------
+// Before the time-series correction every value of channel c is a function
+// T_c(b) of the input byte b, so every channel sum the stages need is a sum
+// over the 256 byte values of (count of b) x T_c(b): the bandpass over
+// [0, n32), the statistics over [0, wp), the flag counts over [0, W).  One
+// pass builds the histograms; the stages then work on 256-entry tables only,
+// one workgroup per channel with thread v holding byte v, each sum being the
+// canonical 256-lane tree of the products in double.
+
+// per channel the histograms of [0, wp), [wp, n32), [n32, W): one LDS copy per
+// wave (fewer colliding increments), 16 bytes per thread and load
+__global__ __launch_bounds__(NT) void k_hist_u8(const FlagBeamDev *__restrict__ beams, int W)
+{
+  const FlagBeamDev B = beams[blockIdx.z];
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int l = threadIdx.x, w = l >> 6;
+  __shared__ uint32_t h[NT / 64][768];
+  for (int i = l; i < (NT / 64) * 768; i += NT) (&h[0][0])[i] = 0u;
+  __syncthreads();
+  const int n32 = (W / 32) * 32, wp = B.wp;
+  const uint8_t *row = B.X8 + (int64_t)c * B.xs;
+  // the row holds xs >= W rounded up to 64 bytes: a 16-byte load that starts
+  // below W stays in the row
+  for (int t0 = 16 * l; t0 < W; t0 += 16 * NT) {
+    const u32x4f d = *reinterpret_cast<const u32x4f *>(row + t0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int t = t0 + i;
+      if (t >= W) break;
+      const uint32_t b = (d[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      atomicAdd(&h[w][(t < wp ? 0 : (t < n32 ? 256 : 512)) + b], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t *out = B.hist + (int64_t)c * 768;
+  for (int i = l; i < 768; i += NT) out[i] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+}
+
+// One step of the table chain for one channel (workgroup), thread v = byte v.
+// STEP 0: bandpass of the input bytes, division by it, statistics;
+// STEP 1: normalisation, bandpass, spectral flag (a0, a1: SPEC_MIN/MAX),
+//         bandpass, division;
+// STEP 2: normalisation and the 1 x 1 scrunch flag (a0, a1: +/- threshold),
+//         flag counts and spectrum, bandpass, division.
+// Statistics of the result (S1, S2 per channel into the partials' first
+// block, the other blocks 0) when `stats`; bp[c] ends as the last bandpass.
+// The values are those of the plane passes (k_apply_lut), the sums the
+// histogram order.
+template <int STEP>
+__global__ __launch_bounds__(NT) void k_hstep(const FlagBeamDev *__restrict__ beams, int W, int lin,
+                                              int lout, float a0, float a1, int stats)
+{
+  const FlagBeamDev B = beams[blockIdx.z];
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int v = threadIdx.x;
+  __shared__ double sh[2 * 128];
+  __shared__ double bc[2];
+  __shared__ unsigned long long cntsh[2];
+  const uint32_t *H = B.hist + (int64_t)c * 768;
+  const double hA = (double)H[v];
+  const double h32 = hA + (double)H[256 + v];
+  const double hW = h32 + (double)H[512 + v];
+  float x = lin < 0 ? (float)v : B.lut[((int64_t)lin * B.nch + c) * 256 + v];
+  // bandpass of the current values (k_finish_bp's (float)(sum / W))
+  auto bandpass = [&]() -> float {
+    double s[1] = {h32 * (double)x};
+    tree256_multi(s, sh);
+    if (v == 0) bc[0] = s[0];
+    __syncthreads();
+    const double r = bc[0];
+    __syncthreads();
+    return (float)(r / (double)W);
+  };
+  auto divide = [&](float bp) { x = bp > 0.f ? x / bp : 0.f; };
+  if (STEP == 0) {
+    const float bp = bandpass();
+    if (v == 0) B.bp[c] = bp;
+    divide(bp);
+  } else {
+    const NormP np = norm_params(B, c);
+    x = normv(x, np);
+    if (STEP == 1) {
+      const float b2 = bandpass();
+      const bool bad = (b2 < a0) || (b2 > a1);
+      x = bad ? 1.f : x + 1.f;
+    } else {
+      const bool g = (x > a0) || (x < a1);
+      const bool fixed = c >= B.flag1 && c < B.flag2;
+      if (g || fixed) x = 0.f;
+      x = x + 1.f;
+      if (v < 2) cntsh[v] = 0ull;
+      __syncthreads();
+      if (g) {
+        atomicAdd(&cntsh[0], (unsigned long long)hW);
+        atomicAdd(&cntsh[1], (unsigned long long)hA);
+      }
+      __syncthreads();
+      if (v == 0) {
+        B.flagcount[c] += (uint32_t)cntsh[0];
+        // k_finish_bp's flag spectrum of a 1 x 1 scrunch
+        B.fspec[c] = B.fspec[c] + (float)((double)cntsh[1] / (double)1 / (double)B.wp);
+      }
+    }
+    const float bp = bandpass();
+    if (v == 0) B.bp[c] = bp;
+    divide(bp);
+  }
+  B.lut[((int64_t)lout * B.nch + c) * 256 + v] = x;
+  if (stats) {
+    const double d = (double)x;
+    double s[2] = {hA * d, hA * (d * d)};
+    tree256_multi(s, sh);
+    double *p1 = B.part + (int64_t)c * B.nblk, *p2 = B.part2 + (int64_t)c * B.nblk;
+    for (int b = v; b < B.nblk; b += NT) {
+      p1[b] = b == 0 ? s[0] : 0.0;
+      p2[b] = b == 0 ? s[1] : 0.0;
+    }
+  }
+}
+
 // ---- robust statistics (FLAG_STATS ROBUST) ------------------------------
 // Key of a float whose unsigned order is the float order (-0 before +0, NaNs
 // past the infinities), and back.
@@ -904,26 +1026,47 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
   hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
                                        (unsigned)nbeam),
                      dim3(256), 0, s, d_beams, W);
-  apply(FOP_NONE, FPART_SUM, n32);
-  bp();
-  divide_stats();
-  apply(FOP_NORM, FPART_SUM, n32);
-  bp();
-  apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
-  bp();
-  for (int i = 0; i < P.nscrunch; ++i) {
-    const FlagScrunch &S = P.scrunch[i];
+  if (!P.robust) {
+    // mean statistics: the stages up to the time-series correction from the
+    // byte histograms, on 256-entry tables (k_hstep), no plane pass
+    hipLaunchKernelGGL(k_hist_u8, gchan, dim3(NT), 0, s, d_beams, W);
+    hipLaunchKernelGGL(k_hstep<0>, gchan, dim3(NT), 0, s, d_beams, W, -1, 0, 0.f, 0.f, 1);
+    stats();
+    hipLaunchKernelGGL(k_hstep<1>, gchan, dim3(NT), 0, s, d_beams, W, 0, 1, P.spec_min,
+                       P.spec_max, P.nscrunch > 0 ? 1 : 0);
+    cur = 1;
+    for (int i = 0; i < P.nscrunch; ++i) {
+      const FlagScrunch &S = P.scrunch[i];
+      stats();  // of the table the previous step divided (its S1, S2)
+      const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
+      const float nthr = (float)(-1. * (double)thr);
+      const int nxt = cur == 0 ? 1 : 0;
+      hipLaunchKernelGGL(k_hstep<2>, gchan, dim3(NT), 0, s, d_beams, W, cur, nxt, thr, nthr,
+                         i + 1 < P.nscrunch ? 1 : 0);
+      cur = nxt;
+    }
+  } else {
+    apply(FOP_NONE, FPART_SUM, n32);
+    bp();
     divide_stats();
-    const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
-    const float nthr = (float)(-1. * (double)thr);
-    apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
-    bp(1);
+    apply(FOP_NORM, FPART_SUM, n32);
+    bp();
+    apply(FOP_SPEC, FPART_SUM, n32, P.spec_min, P.spec_max);
+    bp();
+    for (int i = 0; i < P.nscrunch; ++i) {
+      const FlagScrunch &S = P.scrunch[i];
+      divide_stats();
+      const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
+      const float nthr = (float)(-1. * (double)thr);
+      apply(FOP_NORM_FLAG1, FPART_SUM, n32, thr, nthr);
+      bp(1);
+    }
+    // bandpass_correct of the final scrunch as a table, then ts_correct
+    const int nxt = cur == 0 ? 1 : 0;
+    hipLaunchKernelGGL(k_table, gchan, dim3(NT), 0, s, d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur,
+                       nxt);
+    cur = nxt;
   }
-  // bandpass_correct of the final scrunch as a table, then ts_correct
-  const int nxt = cur == 0 ? 1 : 0;
-  hipLaunchKernelGGL(k_table, dim3(1u, (unsigned)max_nch, (unsigned)nbeam), dim3(NT), 0, s,
-                     d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur, nxt);
-  cur = nxt;
   hipLaunchKernelGGL(k_ts_part, dim3((unsigned)((W + TS_BLOCK - 1) / TS_BLOCK), TS_PARTS, (unsigned)nbeam),
                      dim3(NT), 0, s, d_beams, W, cur);
   hipLaunchKernelGGL(k_ts_combine, dim3((unsigned)((W + NT - 1) / NT), 1, (unsigned)nbeam),

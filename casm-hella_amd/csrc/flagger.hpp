@@ -69,6 +69,10 @@ struct FlagBeamDev {
   uint8_t *X8;         // [nch][xs]
   float *lut;          // [2][nch][256]
   double *tsp;         // [16][xs] partition sums of the time series
+  // per channel the input-byte histograms of columns [0, wp), [wp, n32) and
+  // [n32, W) (n32 = 32 * floor(W / 32)): mean statistics on the value-table
+  // path take every channel sum before the time-series correction from them
+  uint32_t *hist;      // [nch][3][256]
 };
 
 struct FlagScrunch {
@@ -109,11 +113,15 @@ inline bool flag_value_tables(int in_fmt, const FlagParams &P)
 }
 // byte offset of the tables behind the uint8 plane in a work buffer
 inline size_t flag_table_offset(int nch, int64_t xs) { return ((size_t)nch * xs + 255) / 256 * 256; }
-// bytes of the value-table path's work buffer: plane, two tables and the
-// 16 partition sums of the time series
-inline size_t flag_table_bytes(int nch, int64_t xs)
+// bytes of the value-table path's work buffer: plane, two tables, the 16
+// partition sums of the time series and the byte histograms
+inline size_t flag_hist_offset(int nch, int64_t xs)
 {
   return flag_table_offset(nch, xs) + (size_t)2 * nch * 256 * 4 + (size_t)16 * xs * 8;
+}
+inline size_t flag_table_bytes(int nch, int64_t xs)
+{
+  return flag_hist_offset(nch, xs) + (size_t)nch * 768 * 4;
 }
 // a box scrunch (ts x fs > 1) needs the mask plane
 inline bool flag_needs_mask(const FlagParams &P)

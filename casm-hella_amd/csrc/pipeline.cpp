@@ -501,6 +501,7 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
     f.X8 = w.X.as<uint8_t>();
     f.lut = reinterpret_cast<float *>(w.X.as<uint8_t>() + flag_table_offset(B.nch, f.xs));
     f.tsp = reinterpret_cast<double *>(f.lut + (size_t)2 * B.nch * 256);
+    f.hist = reinterpret_cast<uint32_t *>(w.X.as<uint8_t>() + flag_hist_offset(B.nch, f.xs));
   }
   return f;
 }

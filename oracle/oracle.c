@@ -437,11 +437,149 @@ static void bandpass_correct(float *x, int nch, int W)
   free(bp);
 }
 
+/* Byte-histogram order of the flagger up to the time-series correction (8-bit
+ * input, 1 x 1 scrunches, mean statistics).  Every value before ts_correct is
+ * a function T_c(b) of the channel and the input byte, so each channel sum of
+ * the reference's stages is a sum over the 256 byte values of count x value:
+ *   bandpass   bp[c] = (float)(csum_v(h32[v] * T(v)) / W)   (columns < 32*floor(W/32))
+ *   statistics S1 = csum_v(hA[v] * T(v)), S2 = csum_v(hA[v] * (T(v) * T(v)))
+ *              (columns < wp), combined over channels by orc_stats_rows
+ * with csum_v the canonical 256-lane tree (orc_csum of the 256 products, in
+ * double).  The stage values T are the same float operations as on the plane.
+ * On return x holds T(x) after the last bandpass division (the input of
+ * ts_correct). */
+static void flag_hist(const orc_flag_cfg *cfg, float *x, uint32_t *flagcount, float *flagspec)
+{
+  const int nch = cfg->nch, W = cfg->W;
+  const int n32 = (W / 32) * 32, wp = (int)stats_width((size_t)W);
+  uint32_t *H = (uint32_t *)calloc((size_t)nch * 768, sizeof(uint32_t));
+  float *T = (float *)malloc(sizeof(float) * (size_t)nch * 256);
+  double *S1 = (double *)malloc(sizeof(double) * (size_t)nch);
+  double *S2 = (double *)malloc(sizeof(double) * (size_t)nch);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nch; ++c) {
+    uint32_t *h = H + (size_t)c * 768;
+    for (int t = 0; t < W; ++t) {
+      const int v = (int)x[(size_t)c * W + t];
+      h[(t < wp ? 0 : (t < n32 ? 256 : 512)) + v] += 1u;
+    }
+    for (int v = 0; v < 256; ++v) T[(size_t)c * 256 + v] = (float)v;
+  }
+  /* per channel: counts over [0, wp), [0, n32) and [0, W) */
+#define HA(c, v) ((double)H[(size_t)(c) * 768 + (v)])
+#define H32(c, v) (HA(c, v) + (double)H[(size_t)(c) * 768 + 256 + (v)])
+#define HW(c, v) (H32(c, v) + (double)H[(size_t)(c) * 768 + 512 + (v)])
+  double mu = 0.0, inv = 0.0;
+  /* bandpass of the channel's current values */
+#define BANDSUM(c, out)                                                   \
+  do {                                                                    \
+    double p_[256];                                                       \
+    for (int v = 0; v < 256; ++v) p_[v] = H32(c, v) * (double)T[(size_t)(c) * 256 + v]; \
+    out = (float)(orc_csum(p_, 256) / (double)W);                         \
+  } while (0)
+  /* statistics of the channels' current values, then mu, inv */
+#define STATS()                                                           \
+  do {                                                                    \
+    _Pragma("omp parallel for schedule(static)")                          \
+    for (int c = 0; c < nch; ++c) {                                       \
+      double p1[256], p2[256];                                            \
+      for (int v = 0; v < 256; ++v) {                                     \
+        const double d = (double)T[(size_t)c * 256 + v];                  \
+        p1[v] = HA(c, v) * d;                                             \
+        p2[v] = HA(c, v) * (d * d);                                       \
+      }                                                                   \
+      S1[c] = orc_csum(p1, 256);                                          \
+      S2[c] = orc_csum(p2, 256);                                          \
+    }                                                                     \
+    double var_;                                                          \
+    orc_stats_rows(S1, S2, (size_t)nch, (double)wp, &mu, &var_);          \
+    const float sf_ = (float)sqrt(var_);                                  \
+    inv = 1.0 / (double)sf_;                                              \
+  } while (0)
+  /* bandpass_flag (flagger.cu:216-242) */
+  for (int c = 0; c < nch; ++c) {
+    float bp;
+    BANDSUM(c, bp);
+    for (int v = 0; v < 256; ++v) {
+      float *y = &T[(size_t)c * 256 + v];
+      *y = bp > 0.f ? *y / bp : 0.f;
+    }
+  }
+  STATS();
+  for (int c = 0; c < nch; ++c) {
+    float *y = T + (size_t)c * 256, bp2;
+    for (int v = 0; v < 256; ++v) y[v] = (float)(((double)y[v] - mu) * inv);
+    BANDSUM(c, bp2);
+    const int bad = bp2 < cfg->spec_min || bp2 > cfg->spec_max;
+    for (int v = 0; v < 256; ++v) y[v] = bad ? 1.f : y[v] + 1.f;
+  }
+  /* configured 1 x 1 scrunches (flagger.cu:365-384) */
+  for (int s = 0; s < cfg->nscrunch; ++s) {
+    for (int c = 0; c < nch; ++c) {
+      float *y = T + (size_t)c * 256, bp;
+      BANDSUM(c, bp);
+      for (int v = 0; v < 256; ++v) y[v] = bp > 0.f ? y[v] / bp : 0.f;
+    }
+    STATS();
+    const float thr = (float)((double)cfg->scrunch[s].thresh / sqrt(1.));
+    const float nthr = (float)(-1. * (double)thr);
+    for (int c = 0; c < nch; ++c) {
+      float *y = T + (size_t)c * 256;
+      const int fixed = c >= cfg->flag1 && c < cfg->flag2;
+      uint64_t cnt = 0, nbox = 0;
+      for (int v = 0; v < 256; ++v) {
+        float z = (float)(((double)y[v] - mu) * inv);
+        const int g = (z > thr) || (z < nthr);
+        if (g) {
+          cnt += (uint64_t)HW(c, v);
+          nbox += (uint64_t)HA(c, v);
+        }
+        if (g || fixed) z = 0.f;
+        y[v] = z + 1.f;
+      }
+      if (flagcount) flagcount[c] += (uint32_t)cnt;
+      if (flagspec)
+        flagspec[c] = flagspec[c] + (float)((double)nbox / (double)1 / (double)wp);
+    }
+  }
+  /* the bandpass division of apply_scrunch(8,8,100,flag=0,ts=1) */
+  for (int c = 0; c < nch; ++c) {
+    float *y = T + (size_t)c * 256, bp;
+    BANDSUM(c, bp);
+    for (int v = 0; v < 256; ++v) y[v] = bp > 0.f ? y[v] / bp : 0.f;
+  }
+#undef STATS
+#undef BANDSUM
+#undef HW
+#undef H32
+#undef HA
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < nch; ++c)
+    for (int t = 0; t < W; ++t) {
+      float *p = &x[(size_t)c * W + t];
+      *p = T[(size_t)c * 256 + (int)*p];
+    }
+  free(H);
+  free(T);
+  free(S1);
+  free(S2);
+}
+
 /* fast_flagger for one beam (src/flagger.cu:345-411) */
 void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride,
               uint32_t *flagcount, float *flagspec)
 {
   int nch = cfg->nch, W = cfg->W;
+  int hist = cfg->u8 && !cfg->robust;
+  for (int s = 0; hist && s < cfg->nscrunch; ++s)
+    hist = cfg->scrunch[s].tscrunch == 1 && cfg->scrunch[s].fscrunch == 1;
+  if (hist) {
+    flag_hist(cfg, x, flagcount, flagspec);
+    orc_ts_correct(x, nch, W, NULL);
+    orc_normalize(x, nch, W);
+    orc_quantize(x, nch, W, out, out_stride);
+    return;
+  }
   /* bandpass_flag (flagger.cu:216-242) */
   bandpass_correct(x, nch, W);
   if (cfg->robust) orc_normalize_robust(x, nch, W, NULL);

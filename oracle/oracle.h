@@ -82,6 +82,9 @@ typedef struct {
   int nscrunch;
   const orc_scrunch *scrunch;
   int robust;       /* 1: every normalisation is orc_normalize_robust */
+  int u8;           /* 1: the plane holds 8-bit input bytes (0..255): with only
+                       1 x 1 scrunches and mean statistics the stages up to
+                       ts_correct take the byte-histogram order (orc_flag) */
 } orc_flag_cfg;
 
 /* stage functions on a float FT plane x[nch][W] (row stride W) */
@@ -115,8 +118,12 @@ void orc_quantize(const float *x, int nch, int W, uint8_t *out, size_t out_strid
 /* input conversion (transpose_input): u8 TF [W][nch] -> float FT */
 void orc_ingest_u8_tf(const uint8_t *in, int nch, int W, float *x);
 void orc_ingest_f32_tf(const float *in, int nch, int W, float *x);
-/* whole fast_flagger for one beam; x is the FT plane (modified in place),
- * out receives u8 FT rows with stride out_stride.  flagcount (may be NULL)
+/* whole fast_flagger for one beam; x is the FT plane (modified in place).
+ * With cfg->u8, 1 x 1 scrunches and mean statistics, every channel sum
+ * before ts_correct is a sum over the channel's byte histogram (count x
+ * stage value of the byte, the 256 products in the canonical tree of
+ * orc_csum); otherwise sums run over the plane in the canonical blocked
+ * order.  out receives u8 FT rows with stride out_stride.  flagcount (may be NULL)
  * accumulates the grown-mask count per channel, flagspec (may be NULL) the
  * flag spectrum of every scrunch (orc_scrunch_flag). */
 void orc_flag(const orc_flag_cfg *cfg, float *x, uint8_t *out, size_t out_stride,

@@ -47,7 +47,7 @@ def flag_small():
     x[:, 7] = np.clip(np.rint(rng.normal(128.0, 160.0, size=W)), 0, 255)   # a noisy channel
     x[1000:1010, :] = np.clip(x[1000:1010, :] + 96, 0, 255)                # a broadband burst
     data = x.astype(np.uint8)
-    out, cnt = orc.flag(orc.ingest_u8_tf(data), g["SPEC_MIN"], g["SPEC_MAX"], g["FLAG1"],
+    out, cnt = orc.flag_u8(data, g["SPEC_MIN"], g["SPEC_MAX"], g["FLAG1"],
                         g["FLAG2"], g["SCRUNCH"])
     return dict(data=data, out=out, counts=cnt)
 

@@ -30,7 +30,7 @@ class Scrunch(C.Structure):
 class FlagCfg(C.Structure):
     _fields_ = [("nch", _I), ("W", _I), ("spec_min", _F), ("spec_max", _F),
                 ("flag1", _I), ("flag2", _I), ("nscrunch", _I), ("scrunch", C.POINTER(Scrunch)),
-                ("robust", _I)]
+                ("robust", _I), ("u8", _I)]
 
 
 class SearchCfg(C.Structure):
@@ -258,7 +258,7 @@ def ingest_f32_tf(data_tf):
 
 
 def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), spec=None,
-         robust=False):
+         robust=False, u8=False):
     """Whole per-beam flagger on a float FT plane; returns (u8 FT, flagcount).
     spec: a float32 [nch] array that accumulates the flag spectrum."""
     x = np.array(x_ft, np.float32, copy=True)
@@ -266,13 +266,23 @@ def flag(x_ft, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(), 
     sc = (Scrunch * max(1, len(scrunches)))()
     for i, (ts, fs, th) in enumerate(scrunches):
         sc[i] = Scrunch(ts, fs, th)
-    cfg = FlagCfg(nch, W, spec_min, spec_max, flag1, flag2, len(scrunches), sc, 1 if robust else 0)
+    cfg = FlagCfg(nch, W, spec_min, spec_max, flag1, flag2, len(scrunches), sc, 1 if robust else 0,
+                  1 if u8 else 0)
     out = np.zeros((nch, W), np.uint8)
     cnt = np.zeros(nch, np.uint32)
     if spec is not None:
         assert spec.dtype == np.float32 and spec.size == nch and spec.flags.c_contiguous
     L.orc_flag(C.byref(cfg), _p(x), _p(out), W, _p(cnt), _p(spec) if spec is not None else None)
     return out, cnt
+
+
+def flag_u8(gulp_tf, spec_min=-0.05, spec_max=0.15, flag1=-1, flag2=-1, scrunches=(),
+            spec=None, robust=False):
+    """The flagger on an 8-bit time-major gulp ([W][nch] uint8), as the
+    pipeline takes it: byte-histogram order where it applies.  Returns
+    (u8 FT, flagcount)."""
+    return flag(ingest_u8_tf(gulp_tf), spec_min, spec_max, flag1, flag2, scrunches, spec=spec,
+                robust=robust, u8=True)
 
 
 # ---------------------------------------------------------------- search

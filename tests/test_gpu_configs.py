@@ -34,8 +34,7 @@ def _plan(orc, cfg):
 
 
 def _flag(orc, cfg, chunk_tf):
-    out, _ = orc.flag(orc.ingest_u8_tf(chunk_tf), -0.05, 0.15, -1, -1,
-                      tuple(cfg.get("SCRUNCH", ())))
+    out, _ = orc.flag_u8(chunk_tf, -0.05, 0.15, -1, -1, tuple(cfg.get("SCRUNCH", ())))
     return out
 
 

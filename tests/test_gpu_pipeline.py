@@ -29,8 +29,7 @@ def _synth(hamd, cfg, nsamp, seed, t_offset=0, pulses=(), rfi=(), rfi_mult=8.0, 
 
 def _oracle_flag(orc, cfg, gulp_tf):
     sc = cfg.get("SCRUNCH", ())
-    x = orc.ingest_u8_tf(gulp_tf)
-    out, _ = orc.flag(x, cfg["SPEC_MIN"], cfg["SPEC_MAX"], cfg.get("FLAG1", -1),
+    out, _ = orc.flag_u8(gulp_tf, cfg["SPEC_MIN"], cfg["SPEC_MAX"], cfg.get("FLAG1", -1),
                       cfg.get("FLAG2", -1), sc)
     return out
 

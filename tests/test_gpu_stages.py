@@ -59,7 +59,7 @@ def test_stage_api_equals_gulp_and_oracle(hamd, orc):
         parts = _gulp(hamd, g)
         gulp = np.concatenate([p.ravel() for p in parts])
         for b in range(3):
-            flagged[b].append(orc.flag(orc.ingest_u8_tf(parts[b]), -0.05, 0.15, -1, -1,
+            flagged[b].append(orc.flag_u8(parts[b], -0.05, 0.15, -1, -1,
                                        ((1, 1, 5.0),))[0])
         hg.process_gulp(gulp)
         hs.clear_peaks()
@@ -167,8 +167,10 @@ def test_flag_spectrum_and_specflags(hamd, orc, dtype, scr):
                            burst_phase=100 * b, burst_len=6, burst_amp=5.0,
                            pulses=[(0.4, 30.0, 1.0, 40.0)], **kw)
             parts.append(x)
-            ft = orc.ingest_f32_tf(x) if dtype == "F32" else orc.ingest_u8_tf(x)
-            orc.flag(ft, -0.05, 0.15, -1, -1, scr, spec=specs[b])
+            if dtype == "F32":
+                orc.flag(orc.ingest_f32_tf(x), -0.05, 0.15, -1, -1, scr, spec=specs[b])
+            else:
+                orc.flag_u8(x, -0.05, 0.15, -1, -1, scr, spec=specs[b])
         h.process_gulp(np.concatenate([p.ravel() for p in parts]))
         for b in range(2):
             np.testing.assert_array_equal(h.flag_spectrum(b), specs[b])
@@ -203,8 +205,10 @@ def test_robust_statistics_mode(hamd, orc, dtype, scr, W):
                        burst_amp=5.0, pulses=[(t, 40.0, 1.0, 30.0) for t in (0.6, 1.4, 2.2, 3.0)],
                        bp_slope=0.4, **kw)
         h.process_gulp(x)
-        ft = orc.ingest_f32_tf(x) if dtype == "F32" else orc.ingest_u8_tf(x)
-        out, _ = orc.flag(ft, -0.05, 0.15, -1, -1, scr, robust=True)
+        if dtype == "F32":
+            out, _ = orc.flag(orc.ingest_f32_tf(x), -0.05, 0.15, -1, -1, scr, robust=True)
+        else:
+            out, _ = orc.flag_u8(x, -0.05, 0.15, -1, -1, scr, robust=True)
         flagged.append(out)
     F = np.concatenate(flagged, axis=1)
     a0 = (i["min_gulp"] + 2) * W - i["ntime"]
