@@ -546,9 +546,15 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
     sl = sload(last_row, kk);
     sch = k < nlive ? sload(B.live, kk) : -1;
   };
-  (void)sl;
   u32x4 P[NU];
   const int lane16 = lane * 16;
+  // bytes of the staged row that the block's trials read: a trial starts at
+  // the dword holding its first sample, (first & 15) + (its delay - sb) <=
+  // (first & 15) + (sl - sb) bytes in, and reads 64 * V + 4 bytes, so the
+  // rest of the row (sized for the widest block of the beam) is neither
+  // loaded nor stored.  A channel past the last live one is staged whole:
+  // its readers take their offsets from padding entries.
+  int need = lrow;
   // One load site for every case, so the loads stay in flight until the
   // store after the sums (loads issued on several paths make the compiler
   // merge their results with register moves, which wait for the data).  A
@@ -559,6 +565,7 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
     int first = S + sb;
     first -= first >= ring ? ring : 0;
     const int al = first & ~15;
+    need = live ? (first & 15) + (sl - sb) + 64 * V + 4 : lrow;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(B.base + (live ? (int64_t)sch * B.row_stride : 0)), (short)0, live ? ring : 0,
         0x00020000);
@@ -579,15 +586,17 @@ __global__ __launch_bounds__(64 * NDW) void k_dedisp_w(const DDBeamDev *__restri
         col[u] = c;
       }
     }
+    // unit 0 is always needed (need >= 64 * V + 4 > 1024)
 #pragma unroll
     for (int u = 0; u < NU; ++u)
-      P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col[u], nowrap ? u * 1024 : 0, 0);
+      if (u == 0 || lane16 + u * 1024 < need)
+        P[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, col[u], nowrap ? u * 1024 : 0, 0);
   };
   auto store = [&](int buf) {
     uint8_t *dst = lds + (size_t)(buf * CC + wave) * lrow + lane16;
 #pragma unroll
     for (int u = 0; u < NU; ++u)
-      if (u + 1 < NU || lane16 + u * 1024 < lrow) *reinterpret_cast<u32x4 *>(dst + u * 1024) = P[u];
+      if (u == 0 || lane16 + u * 1024 < need) *reinterpret_cast<u32x4 *>(dst + u * 1024) = P[u];
   };
 
   // LDS entry of this wave's trial for the CC channels of a chunk (scalar
