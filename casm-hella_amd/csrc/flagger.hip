@@ -590,6 +590,58 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
   }
 }
 
+// The quantising pass of the value-table path (FOP_NORM_QUANT: the value of
+// table slot lin divided by ts[t], normalised and quantised into the ring):
+// it has no sums and no flags, so a thread takes 16 consecutive samples of
+// one channel, with one 16-byte load of the bytes and, when the ring columns
+// are 16-aligned and do not wrap, one 16-byte store.  Each sample gets the
+// operations of k_apply_lut's quantising pass.
+__global__ __launch_bounds__(NT) void k_quant_lut(const FlagBeamDev *__restrict__ beams, int W,
+                                                  int lin)
+{
+  const FlagBeamDev B = beams[blockIdx.z];
+  const int c = blockIdx.y;
+  if (c >= B.nch) return;
+  const int l = threadIdx.x;
+  __shared__ float tab[256];
+  tab[l] = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  const NormP np = norm_params(B, c);
+  __syncthreads();
+  const int t0 = (blockIdx.x * NT + l) * 16;
+  if (t0 >= W) return;
+  // the row holds xs >= W rounded up to 64 bytes and ts holds W rounded up
+  // to 64 floats: 16-byte loads that start below W stay inside both
+  const u32x4f d = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)c * B.xs + t0);
+  float ts[16];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float4 v = *reinterpret_cast<const float4 *>(B.ts + t0 + 4 * j);
+    ts[4 * j] = v.x;
+    ts[4 * j + 1] = v.y;
+    ts[4 * j + 2] = v.z;
+    ts[4 * j + 3] = v.w;
+  }
+  uint32_t q[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const uint32_t b = (d[i >> 2] >> (8 * (i & 3))) & 0xffu;
+    const float x = normv(tab[b] / ts[i], np);
+    q[i >> 2] |= (uint32_t)quant(x) << (8 * (i & 3));
+  }
+  int64_t col = B.col0 + t0;
+  if (col >= B.ring_len) col -= B.ring_len;
+  uint8_t *dst = B.ring + (int64_t)c * B.ring_stride;
+  if (t0 + 16 <= W && (col & 15) == 0 && col + 16 <= B.ring_len) {
+    *reinterpret_cast<u32x4f *>(dst + col) = u32x4f{q[0], q[1], q[2], q[3]};
+  } else {
+    for (int i = 0; i < 16 && t0 + i < W; ++i) {
+      int64_t ci = col + i;
+      if (ci >= B.ring_len) ci -= B.ring_len;
+      dst[ci] = (uint8_t)(q[i >> 2] >> (8 * (i & 3)));
+    }
+  }
+}
+
 // a channel's table of the next pass on its own (the pass that reads it
 // needs the table before its own statistics exist)
 __global__ __launch_bounds__(NT) void k_table(const FlagBeamDev *__restrict__ beams, int op,
@@ -681,19 +733,28 @@ __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict
 // one workgroup per channel with thread v holding byte v, each sum being the
 // canonical 256-lane tree of the products in double.
 
-// per channel the histograms of [0, wp), [wp, n32), [n32, W): one LDS copy per
-// wave (fewer colliding increments), 16 bytes per thread and load
+// per channel the histograms of [0, wp), [wp, n32), [n32, W), 16 bytes per
+// thread and load.  PACK16: 16 LDS copies (copy = lane & 15) of u16 counters,
+// two bins per word; a copy counts at most W / 16 samples, so the caller
+// takes this form while W / 16 < 65536.  Copy k starts k words past a
+// multiple of 32, so one bin in different copies falls in different banks;
+// with 16 copies, 4 lanes of a wave share one (few colliding increments).
+// Otherwise one u32 copy per wave.
+template <bool PACK16>
 __global__ __launch_bounds__(NT) void k_hist_u8(const FlagBeamDev *__restrict__ beams, int W)
 {
   const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
   if (c >= B.nch) return;
   const int l = threadIdx.x, w = l >> 6;
-  __shared__ uint32_t h[NT / 64][768];
-  for (int i = l; i < (NT / 64) * 768; i += NT) (&h[0][0])[i] = 0u;
+  constexpr int NCOPY = PACK16 ? 16 : NT / 64;
+  constexpr int CW = PACK16 ? 385 : 768;  // words per copy (PACK16: 384 + 1)
+  __shared__ uint32_t h[NCOPY * CW];
+  for (int i = l; i < NCOPY * CW; i += NT) h[i] = 0u;
   __syncthreads();
   const int n32 = (W / 32) * 32, wp = B.wp;
   const uint8_t *row = B.X8 + (int64_t)c * B.xs;
+  uint32_t *mine = h + (PACK16 ? (l & 15) : w) * CW;
   // the row holds xs >= W rounded up to 64 bytes: a 16-byte load that starts
   // below W stays in the row
   for (int t0 = 16 * l; t0 < W; t0 += 16 * NT) {
@@ -703,12 +764,22 @@ __global__ __launch_bounds__(NT) void k_hist_u8(const FlagBeamDev *__restrict__ 
       const int t = t0 + i;
       if (t >= W) break;
       const uint32_t b = (d[i >> 2] >> (8 * (i & 3))) & 0xffu;
-      atomicAdd(&h[w][(t < wp ? 0 : (t < n32 ? 256 : 512)) + b], 1u);
+      const uint32_t bin = (t < wp ? 0u : (t < n32 ? 256u : 512u)) + b;
+      if (PACK16)
+        atomicAdd(&mine[bin >> 1], 1u << (16 * (bin & 1u)));
+      else
+        atomicAdd(&mine[bin], 1u);
     }
   }
   __syncthreads();
   uint32_t *out = B.hist + (int64_t)c * 768;
-  for (int i = l; i < 768; i += NT) out[i] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+  for (int i = l; i < 768; i += NT) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < NCOPY; ++k)
+      s += PACK16 ? (h[k * CW + (i >> 1)] >> (16 * (i & 1))) & 0xffffu : h[k * CW + i];
+    out[i] = s;
+  }
 }
 
 // One step of the table chain for one channel (workgroup), thread v = byte v.
@@ -988,7 +1059,11 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
     const int out = same ? cur : (cur == 0 ? 1 : 0);
     // NB blocks per workgroup: 4, or all of a short gulp's blocks (each
     // block's partial sums are its own, whatever the grouping)
-    if (op == FOP_TSDIV || op == FOP_NORM_QUANT)
+    if (op == FOP_NORM_QUANT && part == FPART_NONE)
+      hipLaunchKernelGGL(k_quant_lut, dim3((unsigned)((W + 16 * NT - 1) / (16 * NT)),
+                                           (unsigned)max_nch, (unsigned)nbeam),
+                         dim3(NT), 0, s, d_beams, W, cur);
+    else if (op == FOP_TSDIV || op == FOP_NORM_QUANT)
       hipLaunchKernelGGL((k_apply_lut<1, true>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
                          npart, a0, a1, cur, out);
     else if (nblk >= 4)
@@ -1029,7 +1104,10 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
   if (!P.robust) {
     // mean statistics: the stages up to the time-series correction from the
     // byte histograms, on 256-entry tables (k_hstep), no plane pass
-    hipLaunchKernelGGL(k_hist_u8, gchan, dim3(NT), 0, s, d_beams, W);
+    if (W / 16 < 65536)
+      hipLaunchKernelGGL(k_hist_u8<true>, gchan, dim3(NT), 0, s, d_beams, W);
+    else
+      hipLaunchKernelGGL(k_hist_u8<false>, gchan, dim3(NT), 0, s, d_beams, W);
     hipLaunchKernelGGL(k_hstep<0>, gchan, dim3(NT), 0, s, d_beams, W, -1, 0, 0.f, 0.f, 1);
     stats();
     hipLaunchKernelGGL(k_hstep<1>, gchan, dim3(NT), 0, s, d_beams, W, 0, 1, P.spec_min,
