@@ -40,6 +40,21 @@ PEAK_ADD_T = 78.6  # packed u16 integer adds/s (256 CU x 64 lanes x 2 x 2.4 GHz)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "plain_1024ch_dm2000_counters.md")
+TUNING_DIR = os.path.join(ROOT, "casm-hella_amd", "tuning")
+
+
+def tuning_record(h, name, tune):
+    """The committed autotuner record of a workload (TUNING_DIR/<name>.json),
+    applied to a fresh context before its first gulp.  With --tune nothing is
+    applied: the tuner runs once the windows have filled and writes
+    <name>.json to --out-dir.  Returns {"best": ..., "source": ...}, or the
+    built-in choice when there is no record."""
+    path = os.path.join(TUNING_DIR, name + ".json")
+    if tune:
+        return {"source": "autotuned in this run", "record": name + ".json"}
+    if not os.path.exists(path):
+        return {"source": "built-in choice (no record)"}
+    return {"best": h.use_tuning(path), "source": os.path.relpath(path, ROOT)}
 
 
 def profiled_traffic(name, key="traffic_bytes_per_launch"):
@@ -312,7 +327,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=4)
+    # untimed steps before the timed ones; at 1024 ch / DM 2000 the first
+    # three gulps only fill the window, and the clocks settle over the next few
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch-beams", type=int, default=4)
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="time this many DM trials of the CPU baseline and scale (0: all)")
@@ -366,7 +383,7 @@ def main():
         raise SystemExit("no GPU visible")
     pulses, rfi = stream_params()
     if args.uneven:
-        print(json.dumps({"uneven": uneven(hamd, args.uneven)}), flush=True)
+        print(json.dumps({"uneven": uneven(hamd, args.uneven, tune=args.tune)}), flush=True)
         return
     if args.stress:
         print(json.dumps({"stress": stress(hamd, args.stress, args.tune)}), flush=True)
@@ -378,6 +395,9 @@ def main():
     if args.tile:
         lanes, chunk, order, loader = (int(v) for v in args.tile.split(","))
         h.set_tile(lanes, chunk, order | (loader << 1))
+        tile_plain = {"source": "--tile " + args.tile}
+    else:
+        tile_plain = tuning_record(h, "plain_1024ch_dm2000", args.tune)
     info = h.info(0)
     need = max(args.warmup, info["min_gulp"] + 1)
     nchunks = need + args.steps
@@ -397,7 +417,7 @@ def main():
     for g in range(need):
         h.process_gulp(dev[g % ndistinct])
     if args.tune:
-        h.autotune(os.path.join(OUT_DIR, "autotune_plain.json"))
+        h.autotune(os.path.join(OUT_DIR, "plain_1024ch_dm2000.json"))
         h.process_gulp(dev[0])
     lat, ncands, nclust = [], [], []
     hamd.device_sync()
@@ -455,6 +475,7 @@ def main():
     # ---------------- batched path (graph-captured), several beams
     nb = args.batch_beams
     hb = hamd.Hella(config(nb))
+    tile_batched = tuning_record(hb, f"batched_{nb}x1024ch_dm2000", args.tune)
     gulp_dev = []
     for i in range(ndistinct):
         buf = hamd.DeviceBuffer(host_chunks[i].nbytes * nb)
@@ -463,7 +484,7 @@ def main():
     for g in range(need):
         hb.process_gulp(gulp_dev[g % ndistinct], batched=True)
     if args.tune:
-        hb.autotune(os.path.join(OUT_DIR, "autotune_c2.json"))
+        hb.autotune(os.path.join(OUT_DIR, f"batched_{nb}x1024ch_dm2000.json"))
         hb.process_gulp(gulp_dev[0], batched=True)
     blat = []
     hamd.device_sync()
@@ -511,6 +532,7 @@ def main():
                        "batched": {"p50": pct(blat, 50), "p99": pct(blat, 99), "beams": nb}},
         "stages_ms": stages,
         "kernels_ms": {"dedisperse": ms_dd, "flagger": ms_flag, "search": ms_search},
+        "dedisp_tile": {"plain": tile_plain, "batched": tile_batched},
         "roofline": {"bound": "valu", "kernel": "k_dedisp", "achieved": achieved_add,
                      "peak": PEAK_ADD_T, "unit": "Tadd/s", "frac": achieved_add / PEAK_ADD_T,
                      "traffic": profiled_traffic("k_dedisp"),
@@ -560,6 +582,7 @@ def stress(hamd, gulps, tune=False):
                TSAMP=tsamp, GULP=W, DM_MIN=0, DM_MAX=5000, DM_TOL=1.25, PULSE_WIDTH=64,
                WIDTH_MIN=1, WIDTH_MAX=1024, SNR=7.0, NBEAM=1, SCRUNCH=[(1, 1, 6.0)])
     h = hamd.Hella(cfg)
+    tile = tuning_record(h, "stress_4096ch_dm5000", tune)
     info = h.info(0)
     rng = np.random.default_rng(5)
     rfi = rng.choice(nch, nch // 5, replace=False)
@@ -581,7 +604,7 @@ def stress(hamd, gulps, tune=False):
     for g in range(need):
         h.process_gulp(bufs[g % 2])
     if tune:
-        h.autotune(os.path.join(OUT_DIR, "autotune_stress.json"))
+        h.autotune(os.path.join(OUT_DIR, "stress_4096ch_dm5000.json"))
         h.process_gulp(bufs[need % 2])
         need += 1
     hamd.device_sync()
@@ -603,10 +626,11 @@ def stress(hamd, gulps, tune=False):
             "ms_per_chunk": ms, "realtime_factor": chunk_s / (ms * 1e-3),
             "beams_per_gpu": int(chunk_s / (ms * 1e-3)), "stages_ms": stages,
             "dedisp_Tadd_s": adds / (stages["dedisperse"] * 1e-3) / 1e12,
-            "raw_crossings_per_chunk": crossings, "candidates_per_chunk": ncand}
+            "raw_crossings_per_chunk": crossings, "candidates_per_chunk": ncand,
+            "dedisp_tile": tile}
 
 
-def uneven(hamd, nbeam, reps=3):
+def uneven(hamd, nbeam, reps=3, tune=False):
     """The uneven multi-beam configuration: nbeam beams split as evenly as
     possible (earlier kinds first) over 512 ch / DM 0-1000, 1024 ch / DM
     0-2000 and 2048 ch / DM 0-3000 (500->400 MHz, 131.072 us), 65536-sample
@@ -644,19 +668,23 @@ def uneven(hamd, nbeam, reps=3):
     print(f"# uneven: {nb} beams ({counts[0]}/{counts[1]}/{counts[2]}), input ready in "
           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
+    tile = {}
+
     def ctx():
+        # the batched form's tuned tile serves both forms (same shapes)
         h = hamd.Hella(cfg)
         for b, (nch, dmx) in enumerate(beams):
             h.set_beam(b, NCHAN=nch, FOFF=-100.0 / nch, DM_MAX=dmx)
             h.set_killmask(b, kms[b])
+        tile.update(tuning_record(h, f"uneven_{nb}beams", tune))
         return h
 
     def run(h, batched):
         need = max(h.info(b)["min_gulp"] for b in range(nb)) + 1
         for g in range(need):
             h.process_gulp(buf, batched=batched)
-        if batched and nb <= 32:
-            h.autotune(os.path.join(OUT_DIR, "autotune_uneven.json"))
+        if batched and tune:
+            h.autotune(os.path.join(OUT_DIR, f"uneven_{nb}beams.json"))
         h.process_gulp(buf, batched=batched)
         hamd.device_sync()
         print(f"# uneven: {'batched' if batched else 'plain'} warm after {need + 1} chunks",
@@ -685,7 +713,8 @@ def uneven(hamd, nbeam, reps=3):
             "batched_Tadd_s_end_to_end": adds / t_b / 1e12,
             "realtime_factor_batched": CHUNK * TSAMP / t_b,
             "candidates": {"batched": nc_b, "plain": nc_p},
-            "last_chunk_identical": bool(np.array_equal(pk_b, pk_p))}
+            "last_chunk_identical": bool(np.array_equal(pk_b, pk_p)),
+            "dedisp_tile": tile}
 
 
 def serving(hamd, args, rfi):

@@ -8,6 +8,7 @@ kernels for gfx950.  Loading fails loudly if the library has not been built
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -406,6 +407,15 @@ class Hella:
     def set_tile(self, lanes, chunk, order=0):
         _hchk(lib().hella_set_tile(self._c, lanes, chunk, order))
 
+    def use_tuning(self, path):
+        """Apply an autotuner record (the JSON that autotune() writes) before
+        the first gulp: its best dedispersion tile is forced for every beam.
+        Returns the record's "best" entry."""
+        with open(path) as f:
+            best = json.load(f)["best"]
+        self.set_tile(*tile_args(best))
+        return best
+
     def set_timing(self, on=True):
         _hchk(lib().hella_set_timing(self._c, 1 if on else 0))
 
@@ -561,6 +571,19 @@ class Hella:
     @property
     def stream(self):
         return lib().hella_get_stream(self._c)
+
+
+def tile_args(best):
+    """(lanes_per_dm, chunk, order) for Hella.set_tile from the "best" entry of
+    an autotuner record, whose "dma" field is the loader number the tuner
+    reports (0 registers, 1 LDS DMA, 2 register prefetch, 3 one trial per wave,
+    4 as 3 with every LDS read of a chunk issued first, 5 as 3 with the
+    windows copied by LDS DMA two chunks ahead)."""
+    loader = {0: 1 << 1, 1: 0, 2: 2 << 1, 3: 3 << 1, 4: (3 << 1) | 8, 5: (3 << 1) | 16}
+    dma = int(best["dma"])
+    if dma not in loader:
+        raise ValueError(f"autotuner record: unknown loader {dma}")
+    return int(best["lanes_per_dm"]), int(best["chunk"]), (int(best["order"]) & 1) | loader[dma]
 
 
 def device_count():
