@@ -108,8 +108,9 @@ bool Config::parse(const std::string &text, std::string &err, bool beam_only)
 }
 
 // ------------------------------------------------------------------ build
-// candidates and clusters staged in pinned memory by the plain path per beam
-constexpr size_t STAGE_N = 2048;
+// clusters staged in pinned memory by the plain path per beam (more are
+// fetched after the counts)
+constexpr size_t KSTAGE_N = 1024;
 
 static void alloc_zero(DevBuf &b, size_t bytes, hipStream_t s)
 {
@@ -429,8 +430,10 @@ void ctx_build(hella_ctx c)
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
-  HAMD_CHECK(hipHostMalloc((void **)&c->h_cstage, STAGE_N * sizeof(Cand), hipHostMallocDefault));
-  HAMD_CHECK(hipHostMalloc((void **)&c->h_kstage, STAGE_N * sizeof(ClusterRec), hipHostMallocDefault));
+  c->cstage_n = std::max<size_t>(1, std::min<size_t>((size_t)std::max(cfg.max_giants, 0),
+                                                      (size_t)std::max(cfg.cand_cap, 0)));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_cstage, c->cstage_n * sizeof(Cand), hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_kstage, KSTAGE_N * sizeof(ClusterRec), hipHostMallocDefault));
   c->d_fdesc.ensure(nb * sizeof(FlagBeamDev));
   c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
@@ -704,11 +707,12 @@ int ctx_find_peaks(hella_ctx c, int b)
     SearchBeamDev *sd = c->d_sdesc.as<SearchBeamDev>() + b;
     SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
     SearchRunner::cluster(sd, 1, cluster_args(c->cfg, rep0, false), c->stream);
-    // the counts and the first STAGE_N candidates and clusters in one round
-    // trip (pinned staging); longer lists fetch the rest after the counts
+    // the counts, the candidates the host can keep and the first KSTAGE_N
+    // clusters in one round trip (pinned staging); longer cluster lists
+    // fetch the rest after the counts
     uint32_t *hn = c->h_ncand + 4 * (size_t)b;
-    const size_t cap = std::min<size_t>(STAGE_N, (size_t)w.s_cands.bytes / sizeof(Cand));
-    const size_t kcap = std::min<size_t>(STAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
+    const size_t cap = std::min<size_t>(c->cstage_n, (size_t)w.s_cands.bytes / sizeof(Cand));
+    const size_t kcap = std::min<size_t>(KSTAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
     HAMD_CHECK(hipMemcpyAsync(hn, w.s_ncand.p, 12, hipMemcpyDeviceToHost, c->stream));
     HAMD_CHECK(hipMemcpyAsync(c->h_cstage, w.s_cands.p, cap * sizeof(Cand), hipMemcpyDeviceToHost,
                               c->stream));

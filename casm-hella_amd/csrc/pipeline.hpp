@@ -123,10 +123,12 @@ struct hella_ctx_s {
   hamd::DDBeamDev *h_ddesc = nullptr;
   hamd::SearchBeamDev *h_sdesc = nullptr;
   uint32_t *h_ncand = nullptr;
-  // pinned staging of the plain path's first candidates and clusters (one
-  // round trip when a beam's lists fit)
+  // pinned staging of the plain path's candidates and first clusters: every
+  // candidate the host keeps of a beam (at most min(MAX_GIANTS, CAND_CAP))
+  // and the first clusters arrive in one round trip
   hamd::Cand *h_cstage = nullptr;
   hamd::ClusterRec *h_kstage = nullptr;
+  size_t cstage_n = 0;
   hamd::DevBuf d_fdesc, d_ddesc, d_sdesc;
   hamd::DevBuf d_in;           // staged gulp
   uint8_t *h_in = nullptr;     // pinned staging for host gulps
