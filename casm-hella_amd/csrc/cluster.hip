@@ -103,6 +103,218 @@ __device__ uint32_t scan_excl(uint32_t v, uint32_t *sh, uint32_t *total)
   return before + incl - v;
 }
 
+// The four phases and the grid clean-up.  RES: the list has at most
+// CT x RES_PT candidates, so each thread keeps its candidates (S/N, sample,
+// DM row), their cells and, after phase 1, their cells' heads in registers,
+// loaded once: the later phases read no candidate or grid word from memory,
+// and every round's loads are issued together.  Otherwise each round reads
+// them from memory.
+constexpr int RES_PT = 10;
+
+template <bool RES>
+__device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const ClusterArgs &A, int n,
+                                               uint32_t *par, unsigned long long *key,
+                                               uint32_t *cnt, uint32_t *smin, uint32_t *smax,
+                                               uint32_t *dmin, uint32_t *dmax, uint32_t *sh)
+{
+  const int l = threadIdx.x, lane = l & 63;
+  const int nround = (n + CT - 1) / CT;
+  constexpr int UNR = RES ? RES_PT : 1;  // rounds unrolled (all of them when resident)
+  auto cell_of = [&](int samp, int dm) { return (dm / A.cd) * B.cl_cols + samp / A.ct; };
+  float rsnr[RES ? RES_PT : 1];
+  int rsamp[RES ? RES_PT : 1], rdm[RES ? RES_PT : 1], rcell[RES ? RES_PT : 1];
+  uint32_t rhead[RES ? RES_PT : 1];
+  if constexpr (RES) {
+#pragma unroll
+    for (int k = 0; k < RES_PT; ++k) {
+      const int i = l + CT * k;
+      if (i < n) {
+        const Cand c = B.cands[i];
+        rsnr[k] = c.snr;
+        rsamp[k] = c.samp;
+        rdm[k] = c.dm_idx;
+        rcell[k] = cell_of(c.samp, c.dm_idx);
+      }
+    }
+  }
+  // candidate i = l + CT * k of this thread, and its cell
+  auto cand = [&](int k, float &snr, int &samp, int &dm, int &cl) {
+    if constexpr (RES) {
+      snr = rsnr[k];
+      samp = rsamp[k];
+      dm = rdm[k];
+      cl = rcell[k];
+    } else {
+      const Cand c = B.cands[l + CT * k];
+      snr = c.snr;
+      samp = c.samp;
+      dm = c.dm_idx;
+      cl = cell_of(samp, dm);
+    }
+  };
+
+  // 1. cells, and every candidate's words (only the cell heads' are used)
+#pragma unroll UNR
+  for (int k = 0; k < (RES ? RES_PT : nround); ++k) {
+    const int i = l + CT * k;
+    if (i >= n) break;
+    float snr;
+    int samp, dm, cl;
+    cand(k, snr, samp, dm, cl);
+    atomicMin(B.cl_grid + cl, (uint32_t)i);
+    par[i] = (uint32_t)i;
+    key[i] = 0ull;
+    cnt[i] = 0u;
+    smin[i] = NONE;
+    smax[i] = 0u;
+    dmin[i] = NONE;
+    dmax[i] = 0u;
+  }
+  __threadfence();
+  __syncthreads();
+  // the head of every candidate's cell (fixed from here on)
+  auto head = [&](int k, int cl) -> uint32_t {
+    if constexpr (RES)
+      return rhead[k];
+    else
+      return ld(B.cl_grid + cl);
+  };
+  if constexpr (RES) {
+#pragma unroll
+    for (int k = 0; k < RES_PT; ++k)
+      if (l + CT * k < n) rhead[k] = ld(B.cl_grid + rcell[k]);
+  }
+  // 2. cell heads join the occupied cells around them
+#pragma unroll UNR
+  for (int k = 0; k < (RES ? RES_PT : nround); ++k) {
+    const int i = l + CT * k;
+    if (i >= n) break;
+    float snr;
+    int samp, dm, cl;
+    cand(k, snr, samp, dm, cl);
+    if (head(k, cl) != (uint32_t)i) continue;
+    const int r = dm / A.cd, t = samp / A.ct;
+    uint32_t g[8];
+    int q = 0;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+      for (int dt = -1; dt <= 1; ++dt) {
+        if (!dr && !dt) continue;
+        const int rr = r + dr, tt = t + dt;
+        g[q++] = (rr < 0 || rr >= B.cl_rows || tt < 0 || tt >= B.cl_cols)
+                     ? NONE
+                     : ld(B.cl_grid + rr * B.cl_cols + tt);
+      }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (g[j] != NONE) unite(par, (uint32_t)i, g[j]);
+  }
+  __threadfence();
+  __syncthreads();
+  // 3. every candidate into its root (all lanes take part in every round)
+#pragma unroll UNR
+  for (int k = 0; k < (RES ? RES_PT : nround); ++k) {
+    if (CT * k >= n) break;
+    const int i = l + CT * k;
+    bool act = i < n;
+    uint32_t root = 0, sm = 0, dmv = 0;
+    unsigned long long kv = 0ull;
+    if (act) {
+      float snr;
+      int samp, dm, cl;
+      cand(k, snr, samp, dm, cl);
+      root = find(par, head(k, cl));
+      kv = ((unsigned long long)ordered(snr) << 32) | (unsigned long long)(NONE - (uint32_t)i);
+      sm = (uint32_t)samp;
+      dmv = (uint32_t)dm;
+    }
+    // up to four groups of lanes with a common root, combined by shuffles
+    // (identities from the lanes outside the group); the rest add directly
+    for (int g = 0; g < 4; ++g) {
+      const unsigned long long pend = __ballot(act);
+      if (!pend) break;
+      const int lead = __ffsll((long long)pend) - 1;
+      const uint32_t r = (uint32_t)__shfl((int)root, lead, 64);
+      const bool mine = act && root == r;
+      unsigned long long kk = mine ? kv : 0ull;
+      uint32_t cc = mine ? 1u : 0u, s0 = mine ? sm : NONE, s1 = mine ? sm : 0u;
+      uint32_t d0 = mine ? dmv : NONE, d1 = mine ? dmv : 0u;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long ko = __shfl_xor(kk, o, 64);
+        kk = ko > kk ? ko : kk;
+        cc += (uint32_t)__shfl_xor((int)cc, o, 64);
+        s0 = min(s0, (uint32_t)__shfl_xor((int)s0, o, 64));
+        s1 = max(s1, (uint32_t)__shfl_xor((int)s1, o, 64));
+        d0 = min(d0, (uint32_t)__shfl_xor((int)d0, o, 64));
+        d1 = max(d1, (uint32_t)__shfl_xor((int)d1, o, 64));
+      }
+      if (lane == lead) {
+        atomicMax(key + r, kk);
+        atomicAdd(cnt + r, cc);
+        atomicMin(smin + r, s0);
+        atomicMax(smax + r, s1);
+        atomicMin(dmin + r, d0);
+        atomicMax(dmax + r, d1);
+      }
+      act = act && !mine;
+    }
+    if (act) {
+      atomicMax(key + root, kv);
+      atomicAdd(cnt + root, 1u);
+      atomicMin(smin + root, sm);
+      atomicMax(smax + root, sm);
+      atomicMin(dmin + root, dmv);
+      atomicMax(dmax + root, dmv);
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  // 4. representatives in index order
+  uint32_t run = 0;
+#pragma unroll UNR
+  for (int k = 0; k < (RES ? RES_PT : nround); ++k) {
+    if (CT * k >= n) break;
+    const int i = l + CT * k;
+    uint32_t root = 0;
+    bool rep = false;
+    if (i < n) {
+      float snr;
+      int samp, dm, cl;
+      cand(k, snr, samp, dm, cl);
+      root = find(par, head(k, cl));
+      rep = NONE - (uint32_t)(ld64(key + root) & 0xffffffffull) == (uint32_t)i;
+    }
+    uint32_t tot;
+    const uint32_t slot = run + scan_excl(rep ? 1u : 0u, sh, &tot);
+    if (rep) {
+      ClusterRec o;
+      o.rep = i;
+      o.count = (int)ld(cnt + root);
+      o.samp_min = (int)ld(smin + root);
+      o.samp_max = (int)ld(smax + root);
+      o.dm_min = (int)ld(dmin + root);
+      o.dm_max = (int)ld(dmax + root);
+      o.beam = 0;
+      B.clusters[slot] = o;
+    }
+    run += tot;
+  }
+  if (l == 0) B.cl_n[0] = run;
+  // the grid is left empty for the next gulp
+  __syncthreads();
+#pragma unroll UNR
+  for (int k = 0; k < (RES ? RES_PT : nround); ++k) {
+    const int i = l + CT * k;
+    if (i >= n) break;
+    float snr;
+    int samp, dm, cl;
+    cand(k, snr, samp, dm, cl);
+    B.cl_grid[cl] = NONE;
+  }
+}
+
 __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict__ beams,
                                                 ClusterArgs A)
 {
@@ -139,131 +351,10 @@ __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict_
   uint32_t *smax = smin + cap;
   uint32_t *dmin = smax + cap;
   uint32_t *dmax = dmin + cap;
-  auto cell = [&](const Cand &c) {
-    return (c.dm_idx / A.cd) * B.cl_cols + c.samp / A.ct;
-  };
-
-  // 1. cells, and every candidate's words (only the cell heads' are used)
-  for (int i = l; i < n; i += CT) {
-    atomicMin(B.cl_grid + cell(B.cands[i]), (uint32_t)i);
-    par[i] = (uint32_t)i;
-    key[i] = 0ull;
-    cnt[i] = 0u;
-    smin[i] = NONE;
-    smax[i] = 0u;
-    dmin[i] = NONE;
-    dmax[i] = 0u;
-  }
-  __threadfence();
-  __syncthreads();
-  // 2. cell heads join the occupied cells around them
-  for (int i = l; i < n; i += CT) {
-    const Cand c = B.cands[i];
-    const int r = c.dm_idx / A.cd, t = c.samp / A.ct;
-    if (ld(B.cl_grid + r * B.cl_cols + t) != (uint32_t)i) continue;
-    uint32_t g[8];
-    int k = 0;
-#pragma unroll
-    for (int dr = -1; dr <= 1; ++dr)
-#pragma unroll
-      for (int dt = -1; dt <= 1; ++dt) {
-        if (!dr && !dt) continue;
-        const int rr = r + dr, tt = t + dt;
-        g[k++] = (rr < 0 || rr >= B.cl_rows || tt < 0 || tt >= B.cl_cols)
-                     ? NONE
-                     : ld(B.cl_grid + rr * B.cl_cols + tt);
-      }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (g[q] != NONE) unite(par, (uint32_t)i, g[q]);
-  }
-  __threadfence();
-  __syncthreads();
-  // 3. every candidate into its root
-  const int lane = l & 63;
-  for (int base = 0; base < n; base += CT) {
-    const int i = base + l;
-    bool act = i < n;
-    uint32_t root = 0, sm = 0, dm = 0;
-    unsigned long long k = 0ull;
-    if (act) {
-      const Cand c = B.cands[i];
-      root = find(par, ld(B.cl_grid + cell(c)));
-      k = ((unsigned long long)ordered(c.snr) << 32) | (unsigned long long)(NONE - (uint32_t)i);
-      sm = (uint32_t)c.samp;
-      dm = (uint32_t)c.dm_idx;
-    }
-    // up to four groups of lanes with a common root, combined by shuffles
-    // (identities from the lanes outside the group); the rest add directly
-    for (int g = 0; g < 4; ++g) {
-      const unsigned long long pend = __ballot(act);
-      if (!pend) break;
-      const int lead = __ffsll((long long)pend) - 1;
-      const uint32_t r = (uint32_t)__shfl((int)root, lead, 64);
-      const bool mine = act && root == r;
-      unsigned long long kk = mine ? k : 0ull;
-      uint32_t cc = mine ? 1u : 0u, s0 = mine ? sm : NONE, s1 = mine ? sm : 0u;
-      uint32_t d0 = mine ? dm : NONE, d1 = mine ? dm : 0u;
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        const unsigned long long ko = __shfl_xor(kk, o, 64);
-        kk = ko > kk ? ko : kk;
-        cc += (uint32_t)__shfl_xor((int)cc, o, 64);
-        s0 = min(s0, (uint32_t)__shfl_xor((int)s0, o, 64));
-        s1 = max(s1, (uint32_t)__shfl_xor((int)s1, o, 64));
-        d0 = min(d0, (uint32_t)__shfl_xor((int)d0, o, 64));
-        d1 = max(d1, (uint32_t)__shfl_xor((int)d1, o, 64));
-      }
-      if (lane == lead) {
-        atomicMax(key + r, kk);
-        atomicAdd(cnt + r, cc);
-        atomicMin(smin + r, s0);
-        atomicMax(smax + r, s1);
-        atomicMin(dmin + r, d0);
-        atomicMax(dmax + r, d1);
-      }
-      act = act && !mine;
-    }
-    if (act) {
-      atomicMax(key + root, k);
-      atomicAdd(cnt + root, 1u);
-      atomicMin(smin + root, sm);
-      atomicMax(smax + root, sm);
-      atomicMin(dmin + root, dm);
-      atomicMax(dmax + root, dm);
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  // 4. representatives in index order
-  uint32_t run = 0;
-  for (int base = 0; base < n; base += CT) {
-    const int i = base + l;
-    uint32_t root = 0;
-    bool rep = false;
-    if (i < n) {
-      root = find(par, ld(B.cl_grid + cell(B.cands[i])));
-      rep = NONE - (uint32_t)(ld64(key + root) & 0xffffffffull) == (uint32_t)i;
-    }
-    uint32_t tot;
-    const uint32_t slot = run + scan_excl(rep ? 1u : 0u, sh, &tot);
-    if (rep) {
-      ClusterRec o;
-      o.rep = i;
-      o.count = (int)ld(cnt + root);
-      o.samp_min = (int)ld(smin + root);
-      o.samp_max = (int)ld(smax + root);
-      o.dm_min = (int)ld(dmin + root);
-      o.dm_max = (int)ld(dmax + root);
-      o.beam = 0;
-      B.clusters[slot] = o;
-    }
-    run += tot;
-  }
-  if (l == 0) B.cl_n[0] = run;
-  // the grid is left empty for the next gulp
-  __syncthreads();
-  for (int i = l; i < n; i += CT) B.cl_grid[cell(B.cands[i])] = NONE;
+  if (n <= CT * RES_PT)
+    cluster_phases<true>(B, A, n, par, key, cnt, smin, smax, dmin, dmax, sh);
+  else
+    cluster_phases<false>(B, A, n, par, key, cnt, smin, smax, dmin, dmax, sh);
 }
 
 }  // namespace
