@@ -241,19 +241,117 @@ __device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool va
   return total / (double)B.nch;
 }
 
+// Lanes l < 256 add the row values rowv[l + 256k] of rows below nch in
+// ascending k, the canonical tree folds the lanes (flag_rows_csum's last
+// steps for one 4096-row group).
+__device__ double flag_rows_group(int nch, const double *rowv, double *sh)
+{
+  const int l = threadIdx.x;
+  __syncthreads();
+  double acc[1] = {0.0};
+  if (l < NT) {
+    for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+      if (l + NT * k >= nch) break;
+      acc[0] += rowv[l + NT * k];
+    }
+  }
+  tree256_multi(acc, sh);
+  if (l == 0) sh[128] = acc[0];
+  __syncthreads();
+  double total = 0.0;
+  total += sh[128];
+  __syncthreads();
+  return total / (double)nch;
+}
+
 __global__ __launch_bounds__(FS_NT) void k_finish_stats(const FlagBeamDev *__restrict__ beams, int n)
 {
   const FlagBeamDev B = beams[blockIdx.x];
   __shared__ double rowv[FLAG_BLOCK];
   __shared__ double sh[136];
-  const double mu = flag_rows_csum(B, n, 0.0, false, rowv, sh);
-  const double var = flag_rows_csum(B, n, mu, true, rowv, sh);
+  double mu, var;
+  if (B.nch <= FLAG_BLOCK) {
+    // one group of rows: each thread sums its rows' S1 and S2 once, with
+    // every load issued before the additions, and keeps them for both passes
+    constexpr int RPT = FLAG_BLOCK / FS_NT;
+    const int l = threadIdx.x;
+    const double dn = (double)n;
+    double s1[RPT], s2[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = l + FS_NT * i;
+      s1[i] = 0.0;
+      s2[i] = 0.0;
+      if (r < B.nch) {
+        const double *p1 = B.part + (int64_t)r * B.nblk;
+        const double *p2 = B.part2 + (int64_t)r * B.nblk;
+#pragma unroll 8
+        for (int b = 0; b < B.nblk; ++b) {
+          s1[i] += p1[b];
+          s2[i] += p2[b];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) rowv[l + FS_NT * i] = l + FS_NT * i < B.nch ? s1[i] / dn : 0.0;
+    mu = flag_rows_group(B.nch, rowv, sh);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+      rowv[l + FS_NT * i] =
+          l + FS_NT * i < B.nch ? ((s2[i] - (2.0 * mu) * s1[i]) + (dn * mu) * mu) / dn : 0.0;
+    var = flag_rows_group(B.nch, rowv, sh);
+  } else {
+    mu = flag_rows_csum(B, n, 0.0, false, rowv, sh);
+    var = flag_rows_csum(B, n, mu, true, rowv, sh);
+  }
   if (threadIdx.x == 0) {
     const float sf = (float)sqrt(var);
     B.scal->mu = mu;
     B.scal->sigf = sf;
     B.scal->inv = 1.0 / (double)sf;
   }
+}
+
+// The plane's statistics of a table step's values (k_hstep), formed by
+// each workgroup of the next step from the per-channel sums hs[c] = (S1, S2):
+// the values and order of k_finish_stats over partials whose blocks past the
+// first are zero (adding a zero block leaves a row sum unchanged), lane l
+// forming rows l + 256k itself in ascending k.
+__device__ NormP hstep_norm(const FlagBeamDev &B, const double *hs, double *sh)
+{
+  const int l = threadIdx.x;
+  const double dn = (double)B.wp;
+  double mu = 0.0, var = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    double total = 0.0;
+    for (int r0 = 0; r0 < B.nch; r0 += FLAG_BLOCK) {
+      double acc[1] = {0.0};
+#pragma unroll 4
+      for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+        const int r = r0 + l + NT * k;
+        if (r < B.nch) {
+          const double s1 = 0.0 + hs[2 * r];
+          if (pass == 0) {
+            acc[0] += s1 / dn;
+          } else {
+            const double s2 = 0.0 + hs[2 * r + 1];
+            acc[0] += ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
+          }
+        }
+      }
+      tree256_multi(acc, sh);
+      if (l == 0) sh[128] = acc[0];
+      __syncthreads();
+      total += sh[128];
+      __syncthreads();
+    }
+    if (pass == 0)
+      mu = total / (double)B.nch;
+    else
+      var = total / (double)B.nch;
+  }
+  const float sf = (float)sqrt(var);
+  return NormP{mu, 1.0 / (double)sf, 0.f, 0.f, 0};
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -373,7 +471,9 @@ __global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__r
     for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
 #pragma unroll
       for (int i = 0; i < h; ++i) p[i] += p[i + h];
-    B.ts[t] = (float)(p[0] / (double)B.nch);
+    const float tv = (float)(p[0] / (double)B.nch);
+  B.ts[t] = tv;
+  B.rts[t] = 1.0 / (double)tv;
   }
 }
 
@@ -444,13 +544,11 @@ __global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict
 // the workgroup builds the channel's table for this pass from the previous
 // table (lin < 0: the identity) and the current statistics, the workgroup of
 // the channel's first blocks stores it to slot lout, and every sample is
-// looked up.  TSD (FOP_TSDIV, FOP_NORM_QUANT): the value is then divided by
-// ts[t]; FOP_NORM_QUANT normalises and quantises into the ring.  The bytes
-// (and time series) arrive with 16-byte loads, one per thread and block, and
-// are read back from LDS in the canonical lane order; the NB blocks' partial
-// sums share one tree.  Samples, partial sums and flag counts are those of
-// k_apply.
-template <int NB, bool TSD>
+// looked up.  The bytes arrive with 16-byte loads, one per thread and block,
+// and are read back from LDS in the canonical lane order; the NB blocks'
+// partial sums share one tree.  Samples, partial sums and flag counts are
+// those of k_apply.
+template <int NB>
 __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict__ beams, int W,
                                                   int op, int part, int npart, float a0, float a1,
                                                   int lin, int lout)
@@ -464,7 +562,6 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
   __shared__ float tab[256];
   __shared__ uint8_t flg[256];
   __shared__ __align__(16) uint8_t bytes[NB][FLAG_BLOCK];
-  __shared__ __align__(16) float tsb[TSD ? FLAG_BLOCK : 4];
   __shared__ double sh[2 * NB * 128];
   const uint8_t *row = B.X8 + (int64_t)c * B.xs;
   // loads first: the blocks' bytes (the row holds xs >= W rounded up to 64
@@ -478,21 +575,6 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
     if (t < W) d[j] = *reinterpret_cast<const u32x4f *>(row + t);
   }
   const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
-  if (TSD) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int t = blk0 * FLAG_BLOCK + 4 * (l + NT * j);
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (t + 4 <= W) {
-        v = *reinterpret_cast<const float4 *>(B.ts + t);
-      } else if (t < W) {
-        v.x = B.ts[t];
-        if (t + 1 < W) v.y = B.ts[t + 1];
-        if (t + 2 < W) v.z = B.ts[t + 2];
-      }
-      *reinterpret_cast<float4 *>(&tsb[4 * (l + NT * j)]) = v;
-    }
-  }
   const NormP np = norm_params(B, c);
   const float bpv = B.bp[c];
   const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
@@ -512,7 +594,6 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
 #pragma unroll
   for (int j = 0; j < 2 * NB; ++j) acc[j] = 0.0;
   const bool sums = part != FPART_NONE;
-  const bool to_ring = TSD && op == FOP_NORM_QUANT;
   constexpr int KB = FLAG_BLOCK / NT;
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
@@ -531,19 +612,6 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
         cnt += flg[v[k]];
         cntw += base + l + NT * k < B.wp ? flg[v[k]] : 0u;
       }
-      if (TSD) {
-#pragma unroll
-        for (int k = 0; k < KB; ++k) x[k] = x[k] / tsb[l + NT * k];
-      }
-      if (to_ring) {
-#pragma unroll
-        for (int k = 0; k < KB; ++k) {
-          x[k] = normv(x[k], np);
-          int64_t col = B.col0 + base + l + NT * k;
-          if (col >= B.ring_len) col -= B.ring_len;
-          B.ring[(int64_t)c * B.ring_stride + col] = quant(x[k]);
-        }
-      }
       if (sums) {
 #pragma unroll
         for (int k = 0; k < KB; ++k) {
@@ -559,16 +627,9 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
       const int t = base + l + NT * k;
       if (t >= W) break;
       const int v = bytes[j][l + NT * k];
-      float x = tab[v];
+      const float x = tab[v];
       cnt += flg[v];
       cntw += t < B.wp ? flg[v] : 0u;
-      if (TSD) x = x / tsb[l + NT * k];
-      if (to_ring) {
-        x = normv(x, np);
-        int64_t col = B.col0 + t;
-        if (col >= B.ring_len) col -= B.ring_len;
-        B.ring[(int64_t)c * B.ring_stride + col] = quant(x);
-      }
       if (t < npart && part != FPART_NONE) {
         const double dd = (double)x;
         acc[2 * j] += dd;
@@ -590,54 +651,158 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
   }
 }
 
-// The quantising pass of the value-table path (FOP_NORM_QUANT: the value of
-// table slot lin divided by ts[t], normalised and quantised into the ring):
-// it has no sums and no flags, so a thread takes 16 consecutive samples of
-// one channel, with one 16-byte load of the bytes and, when the ring columns
-// are 16-aligned and do not wrap, one 16-byte store.  Each sample gets the
-// operations of k_apply_lut's quantising pass.
+// The time-series division of the value-table path, y = x / ts[t] with x the
+// channel's table value of the byte, formed as (float)((double)x * rts[t]),
+// rts[t] = 1 / (double)ts[t] rounded to double.  The double product is within
+// 2^-52 (relative) of the exact quotient, and the quotient of two floats is
+// never within 2^-49 (relative) of a boundary between two float roundings (a
+// midpoint of adjacent floats, or the overflow threshold): the quotient minus
+// such a boundary m is (x - ts m) / ts, a nonzero multiple of the last place
+// of ts m over ts.  So the product rounds to the correctly rounded quotient;
+// zeros, infinities and NaNs follow IEEE division as well.
+__device__ __forceinline__ float tsdiv(float x, double r) { return (float)((double)x * r); }
+
+// The statistics pass after the time-series division (FOP_TSDIV): per
+// channel and block the partial sums of y and y * y over the statistics
+// columns in the canonical lane order (k_apply's), TQCH channels per
+// workgroup sharing the block's reciprocals and one tree.
+constexpr int TQCH = 4;
+
+__global__ __launch_bounds__(NT) void k_tsdiv_stats(const FlagBeamDev *__restrict__ beams, int W,
+                                                    int lin)
+{
+  const FlagBeamDev B = beams[blockIdx.z];
+  const int c0 = blockIdx.y * TQCH;
+  if (c0 >= B.nch) return;
+  const int nc = min(TQCH, B.nch - c0);
+  const int blk = blockIdx.x;
+  const int l = threadIdx.x;
+  constexpr int KB = FLAG_BLOCK / NT;
+  __shared__ float tab[TQCH][256];
+  __shared__ __align__(16) uint8_t bytes[TQCH][FLAG_BLOCK];
+  __shared__ double sh[2 * TQCH * 128];
+  const int base = blk * FLAG_BLOCK;
+  // the rows hold xs >= W rounded up to 64 bytes: a 16-byte load that
+  // starts below W stays in the row
+  u32x4f d[TQCH];
+#pragma unroll
+  for (int j = 0; j < TQCH; ++j) {
+    d[j] = u32x4f{0u, 0u, 0u, 0u};
+    if (j < nc && base + 16 * l < W)
+      d[j] = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + base + 16 * l);
+  }
+  double r[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int t = base + l + NT * k;
+    r[k] = t < W ? B.rts[t] : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < TQCH; ++j)
+    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c0 + j) * 256 + l]);
+#pragma unroll
+  for (int j = 0; j < TQCH; ++j) *reinterpret_cast<u32x4f *>(&bytes[j][16 * l]) = d[j];
+  __syncthreads();
+  double acc[2 * TQCH];
+#pragma unroll
+  for (int j = 0; j < 2 * TQCH; ++j) acc[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < TQCH; ++j) {
+    if (j >= nc) break;
+    if (base + FLAG_BLOCK <= B.wp) {
+      int v[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) v[k] = bytes[j][l + NT * k];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        const double dd = (double)tsdiv(tab[j][v[k]], r[k]);
+        acc[2 * j] += dd;
+        acc[2 * j + 1] += dd * dd;
+      }
+    } else {
+#pragma unroll 4
+      for (int k = 0; k < KB; ++k) {
+        if (base + l + NT * k >= B.wp) break;
+        const double dd = (double)tsdiv(tab[j][bytes[j][l + NT * k]], r[k]);
+        acc[2 * j] += dd;
+        acc[2 * j + 1] += dd * dd;
+      }
+    }
+  }
+  tree256_multi(acc, sh);
+  if (l == 0) {
+#pragma unroll
+    for (int j = 0; j < TQCH; ++j) {
+      if (j >= nc) break;
+      B.part[(int64_t)(c0 + j) * B.nblk + blk] = acc[2 * j];
+      B.part2[(int64_t)(c0 + j) * B.nblk + blk] = acc[2 * j + 1];
+    }
+  }
+}
+
+// The quantising pass of the value-table path (FOP_NORM_QUANT): the value of
+// table slot lin divided by ts[t] (tsdiv), normalised and quantised into the
+// ring.  It has no sums and no flags, so a thread takes 16 consecutive
+// samples, with one 16-byte load of a channel's bytes and, when the ring
+// columns are 16-aligned and do not wrap, one 16-byte store; QQCH channels
+// per workgroup share the samples' reciprocals.
+constexpr int QQCH = 4;
+
 __global__ __launch_bounds__(NT) void k_quant_lut(const FlagBeamDev *__restrict__ beams, int W,
                                                   int lin)
 {
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
   const FlagBeamDev B = beams[blockIdx.z];
-  const int c = blockIdx.y;
-  if (c >= B.nch) return;
+  const int c0 = blockIdx.y * QQCH;
+  if (c0 >= B.nch) return;
+  const int nc = min(QQCH, B.nch - c0);
   const int l = threadIdx.x;
-  __shared__ float tab[256];
-  tab[l] = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
-  const NormP np = norm_params(B, c);
+  __shared__ float tab[QQCH][256];
+#pragma unroll
+  for (int j = 0; j < QQCH; ++j)
+    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c0 + j) * 256 + l]);
   __syncthreads();
   const int t0 = (blockIdx.x * NT + l) * 16;
   if (t0 >= W) return;
-  // the row holds xs >= W rounded up to 64 bytes and ts holds W rounded up
-  // to 64 floats: 16-byte loads that start below W stay inside both
-  const u32x4f d = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)c * B.xs + t0);
-  float ts[16];
+  // the rows hold xs >= W rounded up to 64 bytes and rts holds W rounded up
+  // to 64 doubles: 16-byte loads that start below W stay inside both
+  u32x4f d[QQCH];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const float4 v = *reinterpret_cast<const float4 *>(B.ts + t0 + 4 * j);
-    ts[4 * j] = v.x;
-    ts[4 * j + 1] = v.y;
-    ts[4 * j + 2] = v.z;
-    ts[4 * j + 3] = v.w;
+  for (int j = 0; j < QQCH; ++j) {
+    d[j] = u32x4f{0u, 0u, 0u, 0u};
+    if (j < nc) d[j] = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + t0);
   }
-  uint32_t q[4] = {0u, 0u, 0u, 0u};
+  double r[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t b = (d[i >> 2] >> (8 * (i & 3))) & 0xffu;
-    const float x = normv(tab[b] / ts[i], np);
-    q[i >> 2] |= (uint32_t)quant(x) << (8 * (i & 3));
+  for (int j = 0; j < 8; ++j) {
+    const f64x2 v = *reinterpret_cast<const f64x2 *>(B.rts + t0 + 2 * j);
+    r[2 * j] = v.x;
+    r[2 * j + 1] = v.y;
   }
   int64_t col = B.col0 + t0;
   if (col >= B.ring_len) col -= B.ring_len;
-  uint8_t *dst = B.ring + (int64_t)c * B.ring_stride;
-  if (t0 + 16 <= W && (col & 15) == 0 && col + 16 <= B.ring_len) {
-    *reinterpret_cast<u32x4f *>(dst + col) = u32x4f{q[0], q[1], q[2], q[3]};
-  } else {
-    for (int i = 0; i < 16 && t0 + i < W; ++i) {
-      int64_t ci = col + i;
-      if (ci >= B.ring_len) ci -= B.ring_len;
-      dst[ci] = (uint8_t)(q[i >> 2] >> (8 * (i & 3)));
+  const bool vec = t0 + 16 <= W && (col & 15) == 0 && col + 16 <= B.ring_len;
+#pragma unroll
+  for (int j = 0; j < QQCH; ++j) {
+    if (j >= nc) break;
+    const int c = c0 + j;
+    const NormP np = norm_params(B, c);
+    uint32_t q[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const uint32_t b = (d[j][i >> 2] >> (8 * (i & 3))) & 0xffu;
+      const float x = normv(tsdiv(tab[j][b], r[i]), np);
+      q[i >> 2] |= (uint32_t)quant(x) << (8 * (i & 3));
+    }
+    uint8_t *dst = B.ring + (int64_t)c * B.ring_stride;
+    if (vec) {
+      *reinterpret_cast<u32x4f *>(dst + col) = u32x4f{q[0], q[1], q[2], q[3]};
+    } else {
+      for (int i = 0; i < 16 && t0 + i < W; ++i) {
+        int64_t ci = col + i;
+        if (ci >= B.ring_len) ci -= B.ring_len;
+        dst[ci] = (uint8_t)(q[i >> 2] >> (8 * (i & 3)));
+      }
     }
   }
 }
@@ -662,19 +827,18 @@ __global__ __launch_bounds__(NT) void k_table(const FlagBeamDev *__restrict__ be
 
 // k_ts_sum on the value-table path, in two steps.  k_ts_part: workgroup
 // (time block, partition q) adds the values x / bp[c] (table slot lin,
-// already divided) over the partition's channels in channel order for 2048
-// samples, 8 consecutive ones per thread read with one 8-byte load per
+// already divided) over the partition's channels in channel order for 1024
+// samples, 4 consecutive ones per thread read with one 4-byte load per
 // channel; the tables of 32 channels at a time sit in LDS.
 // k_ts_combine: the 16 partition sums of a sample are combined pairwise
-// exactly as k_ts_sum does.
+// exactly as k_ts_sum does; it also stores the reciprocals 1 / ts[t].
 constexpr int TS_GROUP = 32;
-constexpr int TS_SPT = 8;                // samples per thread
+constexpr int TS_SPT = 4;                // samples per thread
 constexpr int TS_BLOCK = NT * TS_SPT;    // samples per workgroup
 
 __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ beams, int W,
                                                 int lin)
 {
-  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
   const FlagBeamDev B = beams[blockIdx.z];
   const int q = blockIdx.y;
   const int l = threadIdx.x;
@@ -686,7 +850,7 @@ __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ 
   double s[TS_SPT];
 #pragma unroll
   for (int j = 0; j < TS_SPT; ++j) s[j] = 0.0;
-  // the row holds xs >= W rounded up to 64 bytes: an 8-byte load that starts
+  // the row holds xs >= W rounded up to 64 bytes: a 4-byte load that starts
   // below W stays in the row
   const bool in = t0 < W;
   for (int g0 = c0; g0 < c1; g0 += TS_GROUP) {
@@ -696,11 +860,11 @@ __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ 
     __syncthreads();
     if (!in) continue;
     const uint8_t *row = B.X8 + (int64_t)g0 * B.xs + t0;
-#pragma unroll 4
+#pragma unroll 8
     for (int i = 0; i < ng; ++i) {
-      const u32x2 w = *reinterpret_cast<const u32x2 *>(row + (int64_t)i * B.xs);
+      const uint32_t w = *reinterpret_cast<const uint32_t *>(row + (int64_t)i * B.xs);
 #pragma unroll
-      for (int j = 0; j < TS_SPT; ++j) s[j] += (double)tab[i][(w[j >> 2] >> (8 * (j & 3))) & 0xffu];
+      for (int j = 0; j < TS_SPT; ++j) s[j] += (double)tab[i][(w >> (8 * j)) & 0xffu];
     }
   }
   double *out = B.tsp + (int64_t)q * B.xs + t0;
@@ -721,7 +885,9 @@ __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict
   for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
 #pragma unroll
     for (int i = 0; i < h; ++i) p[i] += p[i + h];
-  B.ts[t] = (float)(p[0] / (double)B.nch);
+  const float tv = (float)(p[0] / (double)B.nch);
+  B.ts[t] = tv;
+  B.rts[t] = 1.0 / (double)tv;
 }
 
 // ---- byte-histogram path (value-table path with mean statistics) ----------
@@ -788,13 +954,14 @@ __global__ __launch_bounds__(NT) void k_hist_u8(const FlagBeamDev *__restrict__ 
 //         bandpass, division;
 // STEP 2: normalisation and the 1 x 1 scrunch flag (a0, a1: +/- threshold),
 //         flag counts and spectrum, bandpass, division.
-// Statistics of the result (S1, S2 per channel into the partials' first
-// block, the other blocks 0) when `stats`; bp[c] ends as the last bandpass.
-// The values are those of the plane passes (k_apply_lut), the sums the
-// histogram order.
+// Steps 1 and 2 normalise with the statistics of the previous step's values,
+// formed from its per-channel sums in slot sin of hsum; a step with sout >= 0
+// stores the sums of its own values (S1, S2 of the statistics columns) in
+// slot sout.  bp[c] ends as the last bandpass.  The values are those of the
+// plane passes (k_apply_lut), the sums the histogram order.
 template <int STEP>
 __global__ __launch_bounds__(NT) void k_hstep(const FlagBeamDev *__restrict__ beams, int W, int lin,
-                                              int lout, float a0, float a1, int stats)
+                                              int lout, float a0, float a1, int sin, int sout)
 {
   const FlagBeamDev B = beams[blockIdx.z];
   const int c = blockIdx.y;
@@ -824,7 +991,7 @@ __global__ __launch_bounds__(NT) void k_hstep(const FlagBeamDev *__restrict__ be
     if (v == 0) B.bp[c] = bp;
     divide(bp);
   } else {
-    const NormP np = norm_params(B, c);
+    const NormP np = hstep_norm(B, B.hsum + (int64_t)sin * B.nch * 2, sh);
     x = normv(x, np);
     if (STEP == 1) {
       const float b2 = bandpass();
@@ -853,14 +1020,14 @@ __global__ __launch_bounds__(NT) void k_hstep(const FlagBeamDev *__restrict__ be
     divide(bp);
   }
   B.lut[((int64_t)lout * B.nch + c) * 256 + v] = x;
-  if (stats) {
+  if (sout >= 0) {
     const double d = (double)x;
     double s[2] = {hA * d, hA * (d * d)};
     tree256_multi(s, sh);
-    double *p1 = B.part + (int64_t)c * B.nblk, *p2 = B.part2 + (int64_t)c * B.nblk;
-    for (int b = v; b < B.nblk; b += NT) {
-      p1[b] = b == 0 ? s[0] : 0.0;
-      p2[b] = b == 0 ? s[1] : 0.0;
+    if (v == 0) {
+      double *hs = B.hsum + ((int64_t)sout * B.nch + c) * 2;
+      hs[0] = s[0];
+      hs[1] = s[1];
     }
   }
 }
@@ -1046,56 +1213,41 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
 {
   const int nblk = (W + FLAG_BLOCK - 1) / FLAG_BLOCK;
   const int n32 = (W / 32) * 32;
-  int wp = (W / 256) * 256;
-  if (wp == 0) wp = W;
+  const int wp = W >= 256 ? (W / 256) * 256 : W;
   const dim3 gplane((unsigned)nblk, (unsigned)max_nch, (unsigned)nbeam);
   const dim3 grow((unsigned)((max_nch + NT - 1) / NT), 1, (unsigned)nbeam);
-  const dim3 gbeam((unsigned)nbeam);
   int cur = -1;  // table slot holding the current values (-1: the input bytes)
   const dim3 gplane4((unsigned)((nblk + 3) / 4), (unsigned)max_nch, (unsigned)nbeam);
   const dim3 gplane2((unsigned)((nblk + 1) / 2), (unsigned)max_nch, (unsigned)nbeam);
+  // a pass of the robust path through the current tables (op before the
+  // time-series correction); NB blocks per workgroup: 4, or all of a short
+  // gulp's blocks (each block's partial sums are its own, whatever the
+  // grouping)
   auto apply = [&](int op, int part, int npart, float a0 = 0.f, float a1 = 0.f) {
-    const bool same = op == FOP_NONE || op == FOP_NORM_QUANT || op == FOP_TSDIV;
-    const int out = same ? cur : (cur == 0 ? 1 : 0);
-    // NB blocks per workgroup: 4, or all of a short gulp's blocks (each
-    // block's partial sums are its own, whatever the grouping)
-    if (op == FOP_NORM_QUANT && part == FPART_NONE)
-      hipLaunchKernelGGL(k_quant_lut, dim3((unsigned)((W + 16 * NT - 1) / (16 * NT)),
-                                           (unsigned)max_nch, (unsigned)nbeam),
-                         dim3(NT), 0, s, d_beams, W, cur);
-    else if (op == FOP_TSDIV || op == FOP_NORM_QUANT)
-      hipLaunchKernelGGL((k_apply_lut<1, true>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
-                         npart, a0, a1, cur, out);
-    else if (nblk >= 4)
-      hipLaunchKernelGGL((k_apply_lut<4, false>), gplane4, dim3(NT), 0, s, d_beams, W, op, part,
-                         npart, a0, a1, cur, out);
+    const int out = op == FOP_NONE ? cur : (cur == 0 ? 1 : 0);
+    if (nblk >= 4)
+      hipLaunchKernelGGL(k_apply_lut<4>, gplane4, dim3(NT), 0, s, d_beams, W, op, part, npart, a0,
+                         a1, cur, out);
     else if (nblk >= 2)
-      hipLaunchKernelGGL((k_apply_lut<2, false>), gplane2, dim3(NT), 0, s, d_beams, W, op, part,
-                         npart, a0, a1, cur, out);
+      hipLaunchKernelGGL(k_apply_lut<2>, gplane2, dim3(NT), 0, s, d_beams, W, op, part, npart, a0,
+                         a1, cur, out);
     else
-      hipLaunchKernelGGL((k_apply_lut<1, false>), gplane, dim3(NT), 0, s, d_beams, W, op, part,
-                         npart, a0, a1, cur, out);
+      hipLaunchKernelGGL(k_apply_lut<1>, gplane, dim3(NT), 0, s, d_beams, W, op, part, npart, a0,
+                         a1, cur, out);
     cur = out;
   };
   auto bp = [&](int box = 0) {
     hipLaunchKernelGGL(k_finish_bp, grow, dim3(NT), 0, s, d_beams, W, box);
   };
-  auto stats = [&]() { hipLaunchKernelGGL(k_finish_stats, gbeam, dim3(FS_NT), 0, s, d_beams, wp); };
   const dim3 gchan(1u, (unsigned)max_nch, (unsigned)nbeam);
-  // bandpass division and the statistics of its result: a pass with the
-  // partial sums of the plane's mean and sigma, or with robust statistics
-  // the division as a table and each channel's median and MAD through it
+  // robust statistics after a bandpass division: the division as a table
+  // and each channel's median and MAD through it
   auto divide_stats = [&]() {
-    if (P.robust) {
-      const int nxt = cur == 0 ? 1 : 0;
-      hipLaunchKernelGGL(k_table, gchan, dim3(NT), 0, s, d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur,
-                         nxt);
-      cur = nxt;
-      hipLaunchKernelGGL(k_robust<1>, gchan, dim3(NT), 0, s, d_beams, cur);
-    } else {
-      apply(FOP_DIVBP, FPART_SUM2, wp);
-      stats();
-    }
+    const int nxt = cur == 0 ? 1 : 0;
+    hipLaunchKernelGGL(k_table, gchan, dim3(NT), 0, s, d_beams, (int)FOP_DIVBP, 0.f, 0.f, cur,
+                       nxt);
+    cur = nxt;
+    hipLaunchKernelGGL(k_robust<1>, gchan, dim3(NT), 0, s, d_beams, cur);
   };
 
   hipLaunchKernelGGL(k_ingest_u8, dim3((unsigned)((W + 63) / 64), (unsigned)((max_nch + 63) / 64),
@@ -1108,19 +1260,21 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
       hipLaunchKernelGGL(k_hist_u8<true>, gchan, dim3(NT), 0, s, d_beams, W);
     else
       hipLaunchKernelGGL(k_hist_u8<false>, gchan, dim3(NT), 0, s, d_beams, W);
-    hipLaunchKernelGGL(k_hstep<0>, gchan, dim3(NT), 0, s, d_beams, W, -1, 0, 0.f, 0.f, 1);
-    stats();
+    // each step stores its values' per-channel sums (S1, S2) in a slot of
+    // hsum; the next step forms the plane's statistics from them
+    hipLaunchKernelGGL(k_hstep<0>, gchan, dim3(NT), 0, s, d_beams, W, -1, 0, 0.f, 0.f, -1, 0);
+    int slot = 1;
     hipLaunchKernelGGL(k_hstep<1>, gchan, dim3(NT), 0, s, d_beams, W, 0, 1, P.spec_min,
-                       P.spec_max, P.nscrunch > 0 ? 1 : 0);
+                       P.spec_max, 0, P.nscrunch > 0 ? slot : -1);
     cur = 1;
     for (int i = 0; i < P.nscrunch; ++i) {
       const FlagScrunch &S = P.scrunch[i];
-      stats();  // of the table the previous step divided (its S1, S2)
       const float thr = (float)((double)S.thresh / sqrt(1. * S.tscrunch * S.fscrunch));
       const float nthr = (float)(-1. * (double)thr);
       const int nxt = cur == 0 ? 1 : 0;
-      hipLaunchKernelGGL(k_hstep<2>, gchan, dim3(NT), 0, s, d_beams, W, cur, nxt, thr, nthr,
-                         i + 1 < P.nscrunch ? 1 : 0);
+      hipLaunchKernelGGL(k_hstep<2>, gchan, dim3(NT), 0, s, d_beams, W, cur, nxt, thr, nthr, slot,
+                         i + 1 < P.nscrunch ? 1 - slot : -1);
+      slot = 1 - slot;
       cur = nxt;
     }
   } else {
@@ -1152,10 +1306,14 @@ void FlagRunner::run_tables(const FlagBeamDev *d_beams, int nbeam, int max_nch, 
   if (P.robust) {
     hipLaunchKernelGGL(k_robust<2>, gchan, dim3(NT), 0, s, d_beams, cur);
   } else {
-    apply(FOP_TSDIV, FPART_SUM2, wp);
-    stats();
+    hipLaunchKernelGGL(k_tsdiv_stats, dim3((unsigned)nblk, (unsigned)((max_nch + TQCH - 1) / TQCH),
+                                           (unsigned)nbeam),
+                       dim3(NT), 0, s, d_beams, W, cur);
+    hipLaunchKernelGGL(k_finish_stats, dim3((unsigned)nbeam), dim3(FS_NT), 0, s, d_beams, wp);
   }
-  apply(FOP_NORM_QUANT, FPART_NONE, 0);
+  hipLaunchKernelGGL(k_quant_lut, dim3((unsigned)((W + 16 * NT - 1) / (16 * NT)),
+                                       (unsigned)((max_nch + QQCH - 1) / QQCH), (unsigned)nbeam),
+                     dim3(NT), 0, s, d_beams, W, cur);
   HAMD_CHECK(hipGetLastError());
 }
 

@@ -62,6 +62,10 @@ struct FlagBeamDev {
   int robust;
   float *rstat;        // [nch][2]
   float *ts;           // [W] time series of the final pass
+  double *rts;         // [W] 1 / (double)ts[t] (value-table path)
+  // value-table path, mean statistics: per channel the sums (S1, S2) of a
+  // table step's values over the statistics columns, two slots
+  double *hsum;        // [2][nch][2]
   uint8_t *mask;       // [nch][xs] threshold mask scratch
   // value-table path (uint8 input, 1 x 1 scrunches only): the input as a
   // uint8 channel-major plane, and two [nch][256] tables holding each

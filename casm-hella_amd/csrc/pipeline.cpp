@@ -247,8 +247,9 @@ static WorkSizes work_sizes(const WorkDims &d)
   z.D = (size_t)d.ndm * d.dstride * 4;
   z.planes = d.planes ? (size_t)d.nw * d.rows * round_up((size_t)d.nout, 64) * 4 : 0;
   z.fpart = (size_t)d.nch * nblk * 8;
-  // the final pass's time series, then the per-channel robust statistics
-  z.fts = round_up((size_t)d.W, 64) * 4 + (size_t)d.nch * 8;
+  // the final pass's time series, the per-channel robust statistics, the
+  // time series' reciprocals, then the table steps' channel sums
+  z.fts = round_up((size_t)d.W, 64) * 12 + (size_t)d.nch * 40;
   z.spart = ncnt * 8;
   z.scnt = ncnt * 4;
   z.stiles = ((ncnt + SEARCH_BLOCK - 1) / SEARCH_BLOCK + 1) * 4;
@@ -499,6 +500,8 @@ FlagBeamDev make_fdesc(const Config &cfg, int W, BeamState &B, WorkSet &w, const
   f.ts = w.f_ts.as<float>();
   f.robust = cfg.flag.robust ? 1 : 0;
   f.rstat = w.f_ts.as<float>() + round_up((size_t)W, 64);
+  f.rts = reinterpret_cast<double *>(f.rstat + 2 * (size_t)B.nch);
+  f.hsum = f.rts + round_up((size_t)W, 64);
   f.mask = w.mask.as<uint8_t>();
   if (flag_value_tables(cfg.input_fmt, cfg.flag)) {
     f.X8 = w.X.as<uint8_t>();
