@@ -61,6 +61,42 @@ struct Graph {
   int ntiles = 0;
 };
 
+// Results of a lane's chunks come back in one copy: the gather kernel puts
+// each work set's counts, first candidates and first clusters side by side,
+// and one device-to-host copy moves them to pinned memory.  Longer lists
+// fetch the rest after the counts.
+constexpr int SRV_KC = 256;  // candidates staged per chunk
+constexpr int SRV_KK = 64;   // clusters staged per chunk
+
+struct StageSlot {
+  const uint32_t *ncand;  // crossings, clusters, candidates taken
+  const Cand *cands;
+  const ClusterRec *clusters;
+  int cap_c, cap_k;  // entries the two lists hold
+};
+
+struct StageOut {
+  uint32_t n[4];
+  Cand c[SRV_KC];
+  ClusterRec k[SRV_KK];
+};
+
+__global__ __launch_bounds__(256) void k_stage_results(const StageSlot *__restrict__ slots,
+                                                       StageOut *__restrict__ out, int n)
+{
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const StageSlot S = slots[i];
+  StageOut &o = out[i];
+  const int l = threadIdx.x;
+  const uint32_t nc = S.ncand[0], nk = S.ncand[1];
+  if (l < 3) o.n[l] = S.ncand[l];
+  const int tc = min((int)min(nc, (uint32_t)SRV_KC), S.cap_c);
+  const int tk = min((int)min(nk, (uint32_t)SRV_KK), S.cap_k);
+  for (int j = l; j < tc; j += 256) o.c[j] = S.cands[j];
+  for (int j = l; j < tk; j += 256) o.k[j] = S.clusters[j];
+}
+
 struct Lane {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -68,8 +104,8 @@ struct Lane {
   FlagBeamDev *h_fdesc = nullptr;
   DDBeamDev *h_ddesc = nullptr;
   SearchBeamDev *h_sdesc = nullptr;
-  uint32_t *h_ncand = nullptr;
-  DevBuf d_fdesc, d_ddesc, d_sdesc, d_in;
+  StageOut *h_stage = nullptr;  // pinned: one StageOut per work set
+  DevBuf d_fdesc, d_ddesc, d_sdesc, d_in, d_slots, d_stage;
   std::map<int, Graph> flag_g, search_g;  // by batch size
   std::vector<Job> jobs;
   bool busy = false;
@@ -121,7 +157,7 @@ struct hella_server_s {
       if (L.h_fdesc) (void)hipHostFree(L.h_fdesc);
       if (L.h_ddesc) (void)hipHostFree(L.h_ddesc);
       if (L.h_sdesc) (void)hipHostFree(L.h_sdesc);
-      if (L.h_ncand) (void)hipHostFree(L.h_ncand);
+      if (L.h_stage) (void)hipHostFree(L.h_stage);
       if (L.done) (void)hipEventDestroy(L.done);
       if (L.stream) (void)hipStreamDestroy(L.stream);
     }
@@ -238,11 +274,21 @@ void setup_lanes(hella_server s)
     HAMD_CHECK(hipHostMalloc((void **)&L.h_fdesc, nb * sizeof(FlagBeamDev), hipHostMallocDefault));
     HAMD_CHECK(hipHostMalloc((void **)&L.h_ddesc, nb * sizeof(DDBeamDev), hipHostMallocDefault));
     HAMD_CHECK(hipHostMalloc((void **)&L.h_sdesc, nb * sizeof(SearchBeamDev), hipHostMallocDefault));
-    HAMD_CHECK(hipHostMalloc((void **)&L.h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
+    HAMD_CHECK(hipHostMalloc((void **)&L.h_stage, nb * sizeof(StageOut), hipHostMallocDefault));
     L.d_fdesc.ensure(nb * sizeof(FlagBeamDev));
     L.d_ddesc.ensure(nb * sizeof(DDBeamDev));
     L.d_sdesc.ensure(nb * sizeof(SearchBeamDev));
     L.d_in.ensure((size_t)nb * round_up(s->chunk_bytes, 256));
+    L.d_stage.ensure(nb * sizeof(StageOut));
+    std::vector<StageSlot> slots((size_t)nb);
+    for (int i = 0; i < nb; ++i) {
+      const WorkSet &w = L.ws[(size_t)i];
+      slots[(size_t)i] = StageSlot{w.s_ncand.as<uint32_t>(), w.s_cands.as<Cand>(),
+                                   w.c_out.as<ClusterRec>(), (int)(w.s_cands.bytes / sizeof(Cand)),
+                                   (int)(w.c_out.bytes / sizeof(ClusterRec))};
+    }
+    L.d_slots.ensure(nb * sizeof(StageSlot));
+    HAMD_CHECK(hipMemcpy(L.d_slots.p, slots.data(), nb * sizeof(StageSlot), hipMemcpyHostToDevice));
   }
   HAMD_CHECK(hipDeviceSynchronize());
 }
@@ -364,6 +410,17 @@ void record_flag(hella_server s, Lane &L, int n)
                   flag_value_tables(s->cfg.input_fmt, s->cfg.flag));
 }
 
+// the counts, first candidates and first clusters of work sets 0..n-1 to the
+// lane's pinned staging (stream-ordered; a graph may capture it)
+void stage_results(Lane &L, int n)
+{
+  hipLaunchKernelGGL(k_stage_results, dim3((unsigned)n), dim3(256), 0, L.stream,
+                     L.d_slots.as<StageSlot>(), L.d_stage.as<StageOut>(), n);
+  HAMD_CHECK(hipGetLastError());
+  HAMD_CHECK(hipMemcpyAsync(L.h_stage, L.d_stage.p, (size_t)n * sizeof(StageOut),
+                            hipMemcpyDeviceToHost, L.stream));
+}
+
 void record_search(hella_server s, Lane &L, int nr, const DDTileDesc *tiles, int ntiles)
 {
   const BeamState &g = s->geom;
@@ -379,9 +436,7 @@ void record_search(hella_server s, Lane &L, int nr, const DDTileDesc *tiles, int
   SearchRunner::run(L.d_sdesc.as<SearchBeamDev>(), nr, g.rows, s->common, s->d_kern.as<float>(),
                     s->kofs, L.stream);
   SearchRunner::cluster(L.d_sdesc.as<SearchBeamDev>(), nr, cluster_args(s->cfg, 0, false), L.stream);
-  for (int i = 0; i < nr; ++i)
-    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[4 * (size_t)i], L.ws[(size_t)i].s_ncand.p, 12,
-                              hipMemcpyDeviceToHost, L.stream));
+  stage_results(L, nr);
 }
 
 template <class Rec> Graph &get_graph(Lane &L, std::map<int, Graph> &m, int key, Rec &&rec)
@@ -481,7 +536,7 @@ void launch_plain(hella_server s, Lane &L)
                          s->kofs, L.stream);
     SearchRunner::peaks(L.d_sdesc.as<SearchBeamDev>(), 1, B.rows, s->common, L.stream);
     SearchRunner::cluster(L.d_sdesc.as<SearchBeamDev>(), 1, cluster_args(s->cfg, 0, false), L.stream);
-    HAMD_CHECK(hipMemcpyAsync(&L.h_ncand[0], w.s_ncand.p, 12, hipMemcpyDeviceToHost, L.stream));
+    stage_results(L, 1);
   }
   HAMD_CHECK(hipEventRecord(L.done, L.stream));
 }
@@ -501,18 +556,22 @@ void harvest(hella_server s, hella_ring ring, Lane &L)
     res.r.batch = (int)nb;
     res.r.t_arrival_ns = j.ck.t_arr;
     if (j.searched) {
-      const uint32_t nc = L.h_ncand[4 * i], ncl = L.h_ncand[4 * i + 1];
+      const StageOut &so = L.h_stage[i];
+      const uint32_t nc = so.n[0], ncl = so.n[1];
       size_t take = std::min<size_t>(nc, (size_t)s->cfg.cand_cap);
       take = std::min<size_t>(take, (size_t)s->cfg.max_giants);
       std::vector<Cand> cs(take);
       std::vector<ClusterRec> cl(ncl);
-      if (take)
-        HAMD_CHECK(hipMemcpyAsync(cs.data(), L.ws[i].s_cands.p, take * sizeof(Cand),
-                                  hipMemcpyDeviceToHost, L.stream));
-      if (ncl)
-        HAMD_CHECK(hipMemcpyAsync(cl.data(), L.ws[i].c_out.p, ncl * sizeof(ClusterRec),
-                                  hipMemcpyDeviceToHost, L.stream));
-      HAMD_CHECK(hipStreamSynchronize(L.stream));
+      const size_t c0 = std::min<size_t>(take, SRV_KC), k0 = std::min<size_t>(ncl, SRV_KK);
+      std::copy(so.c, so.c + c0, cs.begin());
+      std::copy(so.k, so.k + k0, cl.begin());
+      if (take > c0)
+        HAMD_CHECK(hipMemcpyAsync(cs.data() + c0, L.ws[i].s_cands.as<Cand>() + c0,
+                                  (take - c0) * sizeof(Cand), hipMemcpyDeviceToHost, L.stream));
+      if (ncl > k0)
+        HAMD_CHECK(hipMemcpyAsync(cl.data() + k0, L.ws[i].c_out.as<ClusterRec>() + k0,
+                                  (ncl - k0) * sizeof(ClusterRec), hipMemcpyDeviceToHost, L.stream));
+      if (take > c0 || ncl > k0) HAMD_CHECK(hipStreamSynchronize(L.stream));
       B.crossings = nc;
       res.r.crossings = nc;
       append_candidates(s->cfg, s->W, B, sb.id, cs, (size_t)s->cfg.max_giants, res.cands);
