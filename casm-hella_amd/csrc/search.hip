@@ -101,7 +101,14 @@ __device__ void block_prefix(uint32_t *P, int L, uint32_t *wtot)
   __syncthreads();
 }
 
-__device__ __forceinline__ float normed(double raw, const SScal &sc, int k)
+// The normalisation scalars, read through the constant address space: every
+// field a kernel reads at a wave-uniform index is a scalar load (the search
+// kernels that read them never write them; k_finish, which writes them, runs
+// before, in another launch).
+typedef __attribute__((address_space(4))) const SScal SScalK;
+
+template <class S>
+__device__ __forceinline__ float normed(double raw, const S &sc, int k)
 {
   return sc.sigf[k] > 0.f ? (float)((raw - sc.mu[k]) * sc.inv[k]) : (float)raw;
 }
@@ -136,6 +143,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 
 constexpr int MAXL = SB + MAXBOX;        // samples a block's boxes read
 constexpr int PF = (MAXL + NT - 1) / NT; // prefetch registers per thread
+
+// A load through the global address space: it counts on vmcnt alone, so the
+// waits of LDS accesses (lgkmcnt) never wait for it, as they wait for a flat
+// load.  Plane loads are issued unconditionally at clamped addresses and the
+// values outside the row are zeroed where they are used: a load that sits in
+// a branch of its own is waited for inside that branch, one at a time.
+template <class T>
+__device__ __forceinline__ T gld(const T *p)
+{
+  typedef __attribute__((address_space(1))) const T gt;
+  return *((gt *)p);
+}
 constexpr int PN = MAXL + 1;
 constexpr int TREE_WORDS = 2 * SEARCH_MAXW * 128 * 2;
 constexpr int LDSW = PN > TREE_WORDS ? PN : TREE_WORDS;
@@ -160,7 +179,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
   __shared__ uint32_t wtot[NT / 64];
   uint32_t *P = lds_u;
   double *shd = reinterpret_cast<double *>(lds_u);
-  const SScal &sc = *B.scal;
+  const SScalK &sc = *(SScalK *)B.scal;
   const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
   const int c0 = C.maxW / 2;
   const bool y0 = pass == P_Y0;  // launched with NW = 1
@@ -171,10 +190,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
   auto fetch = [&](int b) {
     const int base = b * SB;
 #pragma unroll
-    for (int t = 0; t < PF; ++t) {
-      const int i = l + NT * t;
-      nx[t] = (i < L && base + i < C.ntime_dd) ? drow[base + i] : 0u;
-    }
+    for (int t = 0; t < PF; ++t) nx[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
   };
   if (!HELLA) fetch(b0);
 
@@ -184,7 +200,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
 #pragma unroll
       for (int t = 0; t < PF; ++t) {
         const int i = l + NT * t;
-        if (i < L) P[1 + i] = nx[t];
+        if (i < L) P[1 + i] = base + i < C.ntime_dd ? nx[t] : 0u;
       }
       if (l == 0) P[0] = 0u;
       if (b + 1 < b1) fetch(b + 1);
@@ -192,7 +208,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
       if (!direct) block_prefix(P, L, wtot);
     }
     auto raw = [&](int k, int j) -> double {
-      if (HELLA) return (double)B.planes[((int64_t)k * B.rows + r) * B.ps + base + j];
+      if (HELLA) return (double)gld(B.planes + ((int64_t)k * B.rows + r) * B.ps + base + j);
       if (direct) return (double)P[1 + j + c0];
       const int w = C.minW << k;
       const int a = j + c0 - w / 2;
@@ -259,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_y0_direct(const SearchBeamDev *__restric
   if (b0 >= b1) return;
   const int l = threadIdx.x;
   __shared__ double shd[2 * 128];
-  const SScal &sc = *B.scal;
+  const SScalK &sc = *(SScalK *)B.scal;
   const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds + C.maxW / 2;
   constexpr int K = SB / NT;
   uint32_t nx[K];
@@ -267,7 +283,7 @@ __global__ __launch_bounds__(NT) void k_y0_direct(const SearchBeamDev *__restric
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = b * SB + l + NT * k;
-      nx[k] = j < C.nout ? drow[j] : 0u;
+      nx[k] = gld(drow + min(j, C.nout - 1));  // used only where j < nout
     }
   };
   fetch(b0);
@@ -335,16 +351,13 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   uint32_t mx[NW];
 #pragma unroll
   for (int k = 0; k < NW; ++k) s1[k] = s2lo[k] = s2hi[k] = q[k] = 0ull;
-  uint32_t nx[PF];
+  // W1: the widths are 1 .. 2^(NW-1), so L = SB + 2^(NW-1) is known here
+  constexpr int PFW = W1 ? (SB + (1 << (NW - 1)) + NT - 1) / NT : PF;
+  uint32_t nx[PFW];
   auto fetch = [&](int b) {
     const int base = b * SB;
 #pragma unroll
-    for (int t = 0; t < PF; ++t) {
-      const int i = l + NT * t;
-      const int col = min(base + i, C.ntime_dd - 1);
-      const uint32_t v = drow[col];
-      nx[t] = (i < L && base + i < C.ntime_dd) ? v : 0u;
-    }
+    for (int t = 0; t < PFW; ++t) nx[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
   };
   auto body = [&](int j, auto in_stats) {
 #pragma unroll
@@ -363,9 +376,9 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   for (int b = b0; b < b1; ++b) {
     const int base = b * SB;
 #pragma unroll
-    for (int t = 0; t < PF; ++t) {
+    for (int t = 0; t < PFW; ++t) {
       const int i = l + NT * t;
-      if (i < L) P[1 + i] = nx[t];
+      if (i < L) P[1 + i] = base + i < C.ntime_dd ? nx[t] : 0u;
     }
     if (l == 0) P[0] = 0u;
     if (b + 1 < b1) fetch(b + 1);
@@ -450,7 +463,7 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   __shared__ uint32_t wtot[NT / 64];
   __shared__ uint32_t shu[NT];
   __shared__ uint32_t shc[SEARCH_MAXW];
-  const SScal &sc = *B.scal;
+  const SScalK &sc = *(SScalK *)B.scal;
   const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
   const int c0 = C.maxW / 2;
   const float thr = sc.thr;
@@ -461,10 +474,11 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   uint32_t can = 0;
   if (pass == P_COUNT) {
     if (!sc.gated) {
-      uint4 bm[SEARCH_MAXW];
+      typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+      u32x4v bm[SEARCH_MAXW];
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k)
-        if (k < C.nw) bm[k] = *reinterpret_cast<const uint4 *>(B.bmax + slot(k) * 4);
+        bm[k] = gld(reinterpret_cast<const u32x4v *>(B.bmax + slot(min(k, C.nw - 1)) * 4));
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k) {
         if (k < C.nw) {
@@ -482,8 +496,7 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   } else {
     uint32_t cn[SEARCH_MAXW];
 #pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k)
-      if (k < C.nw) cn[k] = B.counts[slot(k)];
+    for (int k = 0; k < SEARCH_MAXW; ++k) cn[k] = gld(B.counts + slot(min(k, C.nw - 1)));
 #pragma unroll
     for (int k = 0; k < SEARCH_MAXW; ++k)
       if (k < C.nw) can |= cn[k] ? (1u << k) : 0u;
@@ -494,21 +507,18 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
     const int L = SB + C.maxW;
     uint32_t v[PF];
 #pragma unroll
-    for (int t = 0; t < PF; ++t) {
-      const int i = l + NT * t, col = base + i;
-      v[t] = (i < L && col < C.ntime_dd) ? drow[col] : 0u;
-    }
+    for (int t = 0; t < PF; ++t) v[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
 #pragma unroll
     for (int t = 0; t < PF; ++t) {
       const int i = l + NT * t;
-      if (i < L) P[i + 1] = v[t];
+      if (i < L) P[i + 1] = base + i < C.ntime_dd ? v[t] : 0u;
     }
     if (l == 0) P[0] = 0u;
     __syncthreads();
     block_prefix(P, L, wtot);
   }
   auto raw = [&](int k, int j) -> double {
-    if (HELLA) return (double)B.planes[((int64_t)k * B.rows + r) * B.ps + base + j];
+    if (HELLA) return (double)gld(B.planes + ((int64_t)k * B.rows + r) * B.ps + base + j);
     const int w = C.minW << k;
     const int a = j + c0 - w / 2;
     return (double)(uint32_t)(P[a + w] - P[a]);
@@ -567,7 +577,7 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
       if (lane == 0) shu[kk * (NT / 64) + wv] = (uint32_t)__popcll(m[kk]);
     }
     __syncthreads();
-    uint32_t pos = B.offs[slot(k)];
+    uint32_t pos = gld(B.offs + slot(k));
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
       uint32_t tot = 0, before = 0;
