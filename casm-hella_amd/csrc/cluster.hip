@@ -25,6 +25,8 @@
 // CU's L1).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "search.hpp"
 
 namespace hamd {
@@ -47,7 +49,7 @@ __device__ __forceinline__ unsigned long long ld64(const unsigned long long *p)
 }
 
 // root of x: parents only ever point to lower indices
-__device__ uint32_t find(uint32_t *par, uint32_t x)
+__device__ __forceinline__ uint32_t find(uint32_t *par, uint32_t x)
 {
   uint32_t p = ld(par + x);
   while (p != x) {
@@ -57,7 +59,7 @@ __device__ uint32_t find(uint32_t *par, uint32_t x)
   return x;
 }
 
-__device__ void unite(uint32_t *par, uint32_t a, uint32_t b)
+__device__ __forceinline__ void unite(uint32_t *par, uint32_t a, uint32_t b)
 {
   while (true) {
     a = find(par, a);
@@ -71,6 +73,15 @@ __device__ void unite(uint32_t *par, uint32_t a, uint32_t b)
     // hook the larger root under the smaller; retry if a was hooked meanwhile
     if (atomicCAS(par + a, a, b) == a) return;
   }
+}
+
+// p, known to the compiler as a global-memory address (a generic pointer read
+// from a descriptor is otherwise accessed by flat instructions, which also
+// count on the LDS counter and serialise with the LDS traffic around them)
+template <class T>
+__device__ __forceinline__ T *as_global(T *p)
+{
+  return (T *)(__attribute__((address_space(1))) T *)p;
 }
 
 // float -> u32 with the same order (every S/N here is finite)
@@ -341,20 +352,32 @@ __global__ __launch_bounds__(CT) void k_cluster(const SearchBeamDev *__restrict_
     if (l == 0) B.cl_n[0] = 0u;
     return;
   }
-  const bool lpar = n <= PCAP, lw = n <= WCAP;
+  // every placement of the words is a call site of its own, so each pointer
+  // has one known address space (LDS or global) in the code that uses it
+  SearchBeamDev G = B;
+  G.cands = as_global(B.cands);
+  G.cl_grid = as_global(B.cl_grid);
+  G.clusters = as_global(B.clusters);
   uint32_t *const lds_par = reinterpret_cast<uint32_t *>(lds_key + WCAP);
-  uint32_t *par = lpar ? lds_par : B.cl_work;
-  unsigned long long *key = lw ? lds_key : B.cl_key;
-  uint32_t *cnt = lw ? lds_par + PCAP : B.cl_work + B.cl_cap;
-  const int cap = lw ? WCAP : B.cl_cap;
-  uint32_t *smin = cnt + cap;
-  uint32_t *smax = smin + cap;
-  uint32_t *dmin = smax + cap;
-  uint32_t *dmax = dmin + cap;
-  if (n <= CT * RES_PT)
-    cluster_phases<true>(B, A, n, par, key, cnt, smin, smax, dmin, dmax, sh);
+  uint32_t *const gwork = as_global(B.cl_work);
+  unsigned long long *const gkey = as_global(B.cl_key);
+  auto run = [&](auto res, uint32_t *par, unsigned long long *key, uint32_t *cnt, int cap)
+                 __attribute__((always_inline)) {
+    uint32_t *smin = cnt + cap;
+    uint32_t *smax = smin + cap;
+    uint32_t *dmin = smax + cap;
+    uint32_t *dmax = dmin + cap;
+    cluster_phases<decltype(res)::value>(G, A, n, par, key, cnt, smin, smax, dmin, dmax, sh);
+  };
+  // WCAP <= CT * RES_PT < PCAP: a list whose words fit in LDS is held in registers
+  if (n <= WCAP)
+    run(std::true_type{}, lds_par, lds_key, lds_par + PCAP, WCAP);
+  else if (n <= CT * RES_PT)
+    run(std::true_type{}, lds_par, gkey, gwork + B.cl_cap, B.cl_cap);
+  else if (n <= PCAP)
+    run(std::false_type{}, lds_par, gkey, gwork + B.cl_cap, B.cl_cap);
   else
-    cluster_phases<false>(B, A, n, par, key, cnt, smin, smax, dmin, dmax, sh);
+    run(std::false_type{}, gwork, gkey, gwork + B.cl_cap, B.cl_cap);
 }
 
 }  // namespace

@@ -20,6 +20,16 @@ namespace {
 
 constexpr int NT = 256;
 
+// A load through the global address space: it counts on vmcnt alone, so the
+// waits of the LDS accesses between such loads and their use (lgkmcnt) do not
+// also wait for them, as they wait for a flat load.
+template <class T>
+__device__ __forceinline__ T gld(const T *p)
+{
+  typedef __attribute__((address_space(1))) const T gt;
+  return *((gt *)p);
+}
+
 __device__ __forceinline__ uint8_t quant(float x)
 {
   // transpose_output (src/hella/src/transpose.cu:65-75); NaN -> 0
@@ -40,13 +50,17 @@ struct NormP {
   int robust;
 };
 
+// read through the constant address space (scalar loads at wave-uniform c):
+// the pass that reads them never writes them
 __device__ __forceinline__ NormP norm_params(const FlagBeamDev &B, int c)
 {
-  const FScal sc = *B.scal;
+  typedef __attribute__((address_space(4))) const FScal FScalK;
+  typedef __attribute__((address_space(4))) const float FK;
+  const FScalK &sc = *(FScalK *)B.scal;
   NormP n{sc.mu, sc.inv, 0.f, 0.f, B.robust};
   if (B.robust) {
-    n.m = B.rstat[2 * c];
-    n.s = B.rstat[2 * c + 1];
+    n.m = ((FK *)B.rstat)[2 * c];
+    n.s = ((FK *)B.rstat)[2 * c + 1];
   }
   return n;
 }
@@ -329,12 +343,15 @@ __device__ NormP hstep_norm(const FlagBeamDev &B, const double *hs, double *sh)
 #pragma unroll 4
       for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
         const int r = r0 + l + NT * k;
+        // both words loaded at a clamped row, outside the branch
+        const int rc = min(r, B.nch - 1);
+        const double h1 = gld(hs + 2 * rc), h2 = gld(hs + 2 * rc + 1);
         if (r < B.nch) {
-          const double s1 = 0.0 + hs[2 * r];
+          const double s1 = 0.0 + h1;
           if (pass == 0) {
             acc[0] += s1 / dn;
           } else {
-            const double s2 = 0.0 + hs[2 * r + 1];
+            const double s2 = 0.0 + h2;
             acc[0] += ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
           }
         }
@@ -454,10 +471,11 @@ __global__ __launch_bounds__(64 * TS_PARTS) void k_ts_sum(const FlagBeamDev *__r
   double s = 0.0;
   if (t < W) {
     const float *col = B.X + t;
+    typedef __attribute__((address_space(4))) const float FK;
 #pragma unroll 8
     for (int c = c0; c < c1; ++c) {
-      const float b = B.bp[c];
-      const float x = col[(int64_t)c * B.xs];
+      const float b = ((FK *)B.bp)[c];  // wave-uniform: a scalar load
+      const float x = gld(col + (int64_t)c * B.xs);
       s += (double)(b > 0.f ? x / b : 0.f);
     }
   }
@@ -505,6 +523,7 @@ __device__ __forceinline__ float value_op(int op, float x, float bpv, bool bad, 
 // samples of one channel with one 16-byte store.
 typedef uint32_t u32x4f __attribute__((ext_vector_type(4)));
 
+
 __global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict__ beams, int W)
 {
   const FlagBeamDev B = beams[blockIdx.z];
@@ -519,7 +538,7 @@ __global__ __launch_bounds__(256) void k_ingest_u8(const FlagBeamDev *__restrict
     const uint8_t *src = in + (int64_t)t * B.in_stride + c;
     uint8_t v[16];
     if (t < W && c + 16 <= B.nch && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
-      const u32x4f w = *reinterpret_cast<const u32x4f *>(src);
+      const u32x4f w = gld(reinterpret_cast<const u32x4f *>(src));
 #pragma unroll
       for (int i = 0; i < 16; ++i) v[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
     } else {
@@ -689,17 +708,21 @@ __global__ __launch_bounds__(NT) void k_tsdiv_stats(const FlagBeamDev *__restric
   for (int j = 0; j < TQCH; ++j) {
     d[j] = u32x4f{0u, 0u, 0u, 0u};
     if (j < nc && base + 16 * l < W)
-      d[j] = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + base + 16 * l);
+      d[j] = gld(reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + base + 16 * l));
   }
   double r[KB];
 #pragma unroll
   for (int k = 0; k < KB; ++k) {
     const int t = base + l + NT * k;
-    r[k] = t < W ? B.rts[t] : 0.0;
+    r[k] = gld(B.rts + min(t, W - 1));  // used only for t < wp <= W
   }
 #pragma unroll
-  for (int j = 0; j < TQCH; ++j)
-    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c0 + j) * 256 + l]);
+  for (int j = 0; j < TQCH; ++j) {
+    // loaded for every j (the table exists for every channel and slot), kept
+    // only where it is the value
+    const float tv = gld(B.lut + ((int64_t)max(lin, 0) * B.nch + min(c0 + j, B.nch - 1)) * 256 + l);
+    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : tv);
+  }
 #pragma unroll
   for (int j = 0; j < TQCH; ++j) *reinterpret_cast<u32x4f *>(&bytes[j][16 * l]) = d[j];
   __syncthreads();
@@ -759,8 +782,12 @@ __global__ __launch_bounds__(NT) void k_quant_lut(const FlagBeamDev *__restrict_
   const int l = threadIdx.x;
   __shared__ float tab[QQCH][256];
 #pragma unroll
-  for (int j = 0; j < QQCH; ++j)
-    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c0 + j) * 256 + l]);
+  for (int j = 0; j < QQCH; ++j) {
+    // loaded for every j (the table exists for every channel and slot), kept
+    // only where it is the value
+    const float tv = gld(B.lut + ((int64_t)max(lin, 0) * B.nch + min(c0 + j, B.nch - 1)) * 256 + l);
+    tab[j][l] = j >= nc ? 0.f : (lin < 0 ? (float)l : tv);
+  }
   __syncthreads();
   const int t0 = (blockIdx.x * NT + l) * 16;
   if (t0 >= W) return;
@@ -770,12 +797,12 @@ __global__ __launch_bounds__(NT) void k_quant_lut(const FlagBeamDev *__restrict_
 #pragma unroll
   for (int j = 0; j < QQCH; ++j) {
     d[j] = u32x4f{0u, 0u, 0u, 0u};
-    if (j < nc) d[j] = *reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + t0);
+    if (j < nc) d[j] = gld(reinterpret_cast<const u32x4f *>(B.X8 + (int64_t)(c0 + j) * B.xs + t0));
   }
   double r[16];
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const f64x2 v = *reinterpret_cast<const f64x2 *>(B.rts + t0 + 2 * j);
+    const f64x2 v = gld(reinterpret_cast<const f64x2 *>(B.rts + t0 + 2 * j));
     r[2 * j] = v.x;
     r[2 * j + 1] = v.y;
   }
@@ -856,13 +883,13 @@ __global__ __launch_bounds__(NT) void k_ts_part(const FlagBeamDev *__restrict__ 
   for (int g0 = c0; g0 < c1; g0 += TS_GROUP) {
     const int ng = min(TS_GROUP, c1 - g0);
     __syncthreads();
-    for (int i = 0; i < ng; ++i) tab[i][l] = tabs[(int64_t)(g0 + i) * 256 + l];
+    for (int i = 0; i < ng; ++i) tab[i][l] = gld(tabs + (int64_t)(g0 + i) * 256 + l);
     __syncthreads();
     if (!in) continue;
     const uint8_t *row = B.X8 + (int64_t)g0 * B.xs + t0;
 #pragma unroll 8
     for (int i = 0; i < ng; ++i) {
-      const uint32_t w = *reinterpret_cast<const uint32_t *>(row + (int64_t)i * B.xs);
+      const uint32_t w = gld(reinterpret_cast<const uint32_t *>(row + (int64_t)i * B.xs));
 #pragma unroll
       for (int j = 0; j < TS_SPT; ++j) s[j] += (double)tab[i][(w >> (8 * j)) & 0xffu];
     }
@@ -880,7 +907,7 @@ __global__ __launch_bounds__(NT) void k_ts_combine(const FlagBeamDev *__restrict
   if (t >= W) return;
   double p[TS_PARTS];
 #pragma unroll
-  for (int i = 0; i < TS_PARTS; ++i) p[i] = B.tsp[(int64_t)i * B.xs + t];
+  for (int i = 0; i < TS_PARTS; ++i) p[i] = gld(B.tsp + (int64_t)i * B.xs + t);
 #pragma unroll
   for (int h = TS_PARTS / 2; h >= 1; h >>= 1)
 #pragma unroll
@@ -924,7 +951,7 @@ __global__ __launch_bounds__(NT) void k_hist_u8(const FlagBeamDev *__restrict__ 
   // the row holds xs >= W rounded up to 64 bytes: a 16-byte load that starts
   // below W stays in the row
   for (int t0 = 16 * l; t0 < W; t0 += 16 * NT) {
-    const u32x4f d = *reinterpret_cast<const u32x4f *>(row + t0);
+    const u32x4f d = gld(reinterpret_cast<const u32x4f *>(row + t0));
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int t = t0 + i;
@@ -971,10 +998,11 @@ __global__ __launch_bounds__(NT) void k_hstep(const FlagBeamDev *__restrict__ be
   __shared__ double bc[2];
   __shared__ unsigned long long cntsh[2];
   const uint32_t *H = B.hist + (int64_t)c * 768;
-  const double hA = (double)H[v];
-  const double h32 = hA + (double)H[256 + v];
-  const double hW = h32 + (double)H[512 + v];
-  float x = lin < 0 ? (float)v : B.lut[((int64_t)lin * B.nch + c) * 256 + v];
+  const double hA = (double)gld(H + v);
+  const double h32 = hA + (double)gld(H + 256 + v);
+  const double hW = h32 + (double)gld(H + 512 + v);
+  const float xl = gld(B.lut + ((int64_t)max(lin, 0) * B.nch + c) * 256 + v);
+  float x = lin < 0 ? (float)v : xl;
   // bandpass of the current values (k_finish_bp's (float)(sum / W))
   auto bandpass = [&]() -> float {
     double s[1] = {h32 * (double)x};
