@@ -72,19 +72,21 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 // the u32 (wrap-around) sum of the first i.  Thread l scans a contiguous
 // segment; segment totals are scanned within each wave by shuffles and across
 // the four waves through wtot.
+// MAXSEG: the largest segment; EXACT: L == MAXSEG * NT, so every thread's
+// segment is full and no access is predicated.
+template <int MAXSEG = (SB + MAXBOX + NT - 1) / NT, bool EXACT = false>
 __device__ void block_prefix(uint32_t *P, int L, uint32_t *wtot)
 {
   // thread l owns the seg consecutive words from 1 + l * seg; the loops run
   // over the largest segment with the tail predicated off, so every LDS read
   // of a segment is issued before the first addition (u32 sums wrap, and any
   // order of the additions gives the same words)
-  constexpr int MAXSEG = (SB + MAXBOX + NT - 1) / NT;
   const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
-  const int seg = (L + NT - 1) / NT;
-  const int i0 = 1 + l * seg, n = max(0, min(seg, L + 1 - i0));
+  const int seg = EXACT ? MAXSEG : (L + NT - 1) / NT;
+  const int i0 = 1 + l * seg, n = EXACT ? MAXSEG : max(0, min(seg, L + 1 - i0));
   uint32_t v[MAXSEG];
 #pragma unroll
-  for (int i = 0; i < MAXSEG; ++i) v[i] = i < n ? P[i0 + i] : 0u;
+  for (int i = 0; i < MAXSEG; ++i) v[i] = (EXACT || i < n) ? P[i0 + i] : 0u;
   uint32_t run = 0;
 #pragma unroll
   for (int i = 0; i < MAXSEG; ++i) run += v[i];
@@ -96,7 +98,7 @@ __device__ void block_prefix(uint32_t *P, int L, uint32_t *wtot)
 #pragma unroll
   for (int i = 0; i < MAXSEG; ++i) {
     acc += v[i];
-    if (i < n) P[i0 + i] = acc;
+    if (EXACT || i < n) P[i0 + i] = acc;
   }
   __syncthreads();
 }
@@ -351,8 +353,11 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   uint32_t mx[NW];
 #pragma unroll
   for (int k = 0; k < NW; ++k) s1[k] = s2lo[k] = s2hi[k] = q[k] = 0ull;
-  // W1: the widths are 1 .. 2^(NW-1), so L = SB + 2^(NW-1) is known here
-  constexpr int PFW = W1 ? (SB + (1 << (NW - 1)) + NT - 1) / NT : PF;
+  // W1: the widths are 1 .. 2^(NW-1), so L = SB + 2^(NW-1) is known here;
+  // when it is a multiple of NT every load, store and prefix segment is full
+  constexpr int LW = SB + (1 << (NW - 1));
+  constexpr int PFW = W1 ? (LW + NT - 1) / NT : PF;
+  constexpr bool EX = W1 && LW % NT == 0;
   uint32_t nx[PFW];
   auto fetch = [&](int b) {
     const int base = b * SB;
@@ -378,12 +383,15 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
 #pragma unroll
     for (int t = 0; t < PFW; ++t) {
       const int i = l + NT * t;
-      if (i < L) P[1 + i] = base + i < C.ntime_dd ? nx[t] : 0u;
+      if (EX || i < L) P[1 + i] = base + i < C.ntime_dd ? nx[t] : 0u;
     }
     if (l == 0) P[0] = 0u;
     if (b + 1 < b1) fetch(b + 1);
     __syncthreads();
-    block_prefix(P, L, wtot);
+    if constexpr (EX)
+      block_prefix<PFW, true>(P, L, wtot);
+    else
+      block_prefix(P, L, wtot);
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       if (!NARROW) q[k] = 0ull;
@@ -449,7 +457,9 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
 //  P_COUNT: crossings per width; widths whose block maximum cannot cross the
 //           threshold are 0 without reading D
 //  P_WRITE: candidates of the block, ordered by (width, row, sample)
-template <bool HELLA>
+// LX: 0, or the block's span SB + maxW when it is a multiple of NT (every
+// load, store and prefix segment full, nothing predicated)
+template <bool HELLA, int LX = 0>
 __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ beams,
                                              SearchCommon C, int pass)
 {
@@ -505,17 +515,21 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   if (!HELLA) {
     // the block and its halo: every load issued before the first LDS store
     const int L = SB + C.maxW;
-    uint32_t v[PF];
+    constexpr int PFX = LX ? LX / NT : PF;
+    uint32_t v[PFX];
 #pragma unroll
-    for (int t = 0; t < PF; ++t) v[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
+    for (int t = 0; t < PFX; ++t) v[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
 #pragma unroll
-    for (int t = 0; t < PF; ++t) {
+    for (int t = 0; t < PFX; ++t) {
       const int i = l + NT * t;
-      if (i < L) P[i + 1] = base + i < C.ntime_dd ? v[t] : 0u;
+      if (LX || i < L) P[i + 1] = base + i < C.ntime_dd ? v[t] : 0u;
     }
     if (l == 0) P[0] = 0u;
     __syncthreads();
-    block_prefix(P, L, wtot);
+    if constexpr (LX != 0)
+      block_prefix<PFX, true>(P, L, wtot);
+    else
+      block_prefix(P, L, wtot);
   }
   auto raw = [&](int k, int j) -> double {
     if (HELLA) return (double)gld(B.planes + ((int64_t)k * B.rows + r) * B.ps + base + j);
@@ -951,7 +965,16 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
     } else if (hella) {
       hipLaunchKernelGGL(k_hits<true>, gp, dim3(NT), 0, s, d_beams, C, p);
     } else {
-      hipLaunchKernelGGL(k_hits<false>, gp, dim3(NT), 0, s, d_beams, C, p);
+      // the usual widest boxes get the kernel with every block span full
+      const int lx = SB + C.maxW;
+      if (lx == SB + 256)
+        hipLaunchKernelGGL((k_hits<false, SB + 256>), gp, dim3(NT), 0, s, d_beams, C, p);
+      else if (lx == SB + 512)
+        hipLaunchKernelGGL((k_hits<false, SB + 512>), gp, dim3(NT), 0, s, d_beams, C, p);
+      else if (lx == SB + 1024)
+        hipLaunchKernelGGL((k_hits<false, SB + 1024>), gp, dim3(NT), 0, s, d_beams, C, p);
+      else
+        hipLaunchKernelGGL(k_hits<false>, gp, dim3(NT), 0, s, d_beams, C, p);
     }
   };
   auto fin = [&](int p) {
