@@ -81,7 +81,9 @@ __device__ __forceinline__ void unite(uint32_t *par, uint32_t a, uint32_t b)
 template <class T>
 __device__ __forceinline__ T *as_global(T *p)
 {
-  return (T *)(__attribute__((address_space(1))) T *)p;
+  // through an integer: a cast of the generic pointer itself would be folded away
+  typedef __attribute__((address_space(1))) T gT;
+  return (T *)(gT *)(uintptr_t)p;
 }
 
 // float -> u32 with the same order (every S/N here is finite)

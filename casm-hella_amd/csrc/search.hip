@@ -630,14 +630,31 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
 __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, int r, bool exact,
                          int ngroups, bool want2, double &s1, double &s2)
 {
+  // the partials are loaded RS at a time (clamped, unconditional) before any
+  // is added, and added in ascending order: one memory latency per RS
+  // partials instead of one per partial
+  constexpr int RS = 8;
   if (exact) {
     const uint64_t *p1 = reinterpret_cast<const uint64_t *>(B.part);
     const uint64_t *p2 = reinterpret_cast<const uint64_t *>(B.part2);
     const int64_t base = ((int64_t)k * B.rows + r) * ngroups;
     uint64_t a = 0, lo = 0, hi = 0;
-    for (int g = 0; g < ngroups; ++g) {
-      a += p1[base + g];
-      if (want2) add128(lo, hi, p2[2 * (base + g)], p2[2 * (base + g) + 1]);
+    for (int g0 = 0; g0 < ngroups; g0 += RS) {
+      uint64_t x1[RS], xl[RS], xh[RS];
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        const int64_t g = base + min(g0 + j, ngroups - 1);
+        x1[j] = gld(p1 + g);
+        xl[j] = want2 ? gld(p2 + 2 * g) : 0ull;
+        xh[j] = want2 ? gld(p2 + 2 * g + 1) : 0ull;
+      }
+#pragma unroll
+      for (int j = 0; j < RS; ++j) {
+        if (g0 + j < ngroups) {
+          a += x1[j];
+          add128(lo, hi, xl[j], xh[j]);
+        }
+      }
     }
     s1 = (double)a;
     s2 = want2 ? (double)hi * 18446744073709551616.0 + (double)lo : 0.0;
@@ -646,9 +663,23 @@ __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, i
   const int64_t base = ((int64_t)k * B.rows + r) * C.nblk;
   s1 = 0.0;
   s2 = 0.0;
-  for (int b = 0; b < C.nblk; ++b) s1 += B.part[base + b];
-  if (want2)
-    for (int b = 0; b < C.nblk; ++b) s2 += B.part2[base + b];
+  for (int b0 = 0; b0 < C.nblk; b0 += RS) {
+    double x1[RS], x2[RS];
+#pragma unroll
+    for (int j = 0; j < RS; ++j) {
+      const int64_t b = base + min(b0 + j, C.nblk - 1);
+      x1[j] = gld(B.part + b);
+      x2[j] = want2 ? gld(B.part2 + b) : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < RS; ++j)
+      if (b0 + j < C.nblk) s1 += x1[j];
+    if (want2) {
+#pragma unroll
+      for (int j = 0; j < RS; ++j)
+        if (b0 + j < C.nblk) s2 += x2[j];
+    }
+  }
 }
 
 // The row values of a block of SB rows are formed by all FIN_NT threads at
