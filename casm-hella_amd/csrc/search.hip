@@ -743,11 +743,17 @@ __global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restri
   const float thr = C.snr * sf;
   const int kw = threadIdx.x;
   if (kw < C.nw && C.mode == SEARCH_BOXCAR) {
-    // binary search for the first integer sum that crosses the threshold
+    // binary search for the first integer sum that crosses the threshold; the
+    // width's scalars are read once, not in each of the 33 steps (the steps
+    // evaluate normed() on these same values)
+    struct {
+      double mu[1], inv[1];
+      float sigf[1];
+    } one = {{sc.mu[kw]}, {sc.inv[kw]}, {sc.sigf[kw]}};
     uint64_t lo = 0, hi = 1ull << 32;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) >> 1;
-      if (normed((double)mid, sc, kw) > thr) hi = mid;
+      if (normed((double)mid, one, 0) > thr) hi = mid;
       else lo = mid + 1;
     }
     sc.rthr[kw] = lo;
