@@ -130,11 +130,21 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
   const bool fixed = c >= B.flag1 && c < B.flag2;
   uint32_t cnt = 0, cntw = 0;
   double acc[2] = {0.0, 0.0};
+  // every sample of the thread (and its time-series value) loaded before the
+  // first is used: the thread reads and writes only its own samples
+  constexpr int KB = FLAG_BLOCK / NT;
+  float xin[KB], tsv[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int tc = min(base + l + NT * k, W - 1);
+    xin[k] = gld(row + tc);
+    tsv[k] = op == FOP_DIVBP_TS ? gld(B.ts + tc) : 1.f;
+  }
 #pragma unroll 4
-  for (int k = 0; k < FLAG_BLOCK / NT; ++k) {
+  for (int k = 0; k < KB; ++k) {
     const int t = base + l + NT * k;
     if (t >= W) break;
-    float x = row[t];
+    float x = xin[k];
     switch (op) {
       case FOP_DIVBP: x = bpv > 0.f ? x / bpv : 0.f; row[t] = x; break;
       case FOP_NORM: x = normv(x, np); row[t] = x; break;
@@ -159,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
       }
       case FOP_DIVBP_TS: {
         x = bpv > 0.f ? x / bpv : 0.f;
-        x = x / B.ts[t];
+        x = x / tsv[k];
         row[t] = x;
         break;
       }
@@ -590,10 +600,12 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int t = (blk0 + j) * FLAG_BLOCK + 16 * l;
-    d[j] = u32x4f{0u, 0u, 0u, 0u};
-    if (t < W) d[j] = *reinterpret_cast<const u32x4f *>(row + t);
+    // unconditional at a clamped 16-byte column, zeroed past W
+    const u32x4f v = gld(reinterpret_cast<const u32x4f *>(row + min(t, ((W - 1) & ~15))));
+    d[j] = t < W ? v : u32x4f{0u, 0u, 0u, 0u};
   }
-  const float x0 = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  const float xl = gld(B.lut + ((int64_t)max(lin, 0) * B.nch + c) * 256 + l);
+  const float x0 = lin < 0 ? (float)l : xl;
   const NormP np = norm_params(B, c);
   const float bpv = B.bp[c];
   const bool bad = op == FOP_SPEC && ((bpv < a0) || (bpv > a1));
@@ -1086,9 +1098,19 @@ __device__ uint32_t rselect(int n, uint32_t k, F value, uint32_t *hist, uint32_t
   for (int shift = 24; shift >= 0; shift -= 8) {
     hist[l] = 0u;
     __syncthreads();
-    for (int t = l; t < n; t += NT) {
-      const uint32_t key = fkey(value(t));
-      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 0xffu], 1u);
+    // RU values per thread requested before the first is keyed (one memory
+    // latency per RU samples, not per sample)
+    constexpr int RU = 8;
+    for (int t0 = l; t0 < n; t0 += NT * RU) {
+      float v[RU];
+#pragma unroll
+      for (int u = 0; u < RU; ++u) v[u] = value(min(t0 + NT * u, n - 1));
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        if (t0 + NT * u >= n) break;
+        const uint32_t key = fkey(v[u]);
+        if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 0xffu], 1u);
+      }
     }
     __syncthreads();
     // inclusive scan of the counts (Hillis-Steele over 256 entries)
@@ -1130,14 +1152,17 @@ __global__ __launch_bounds__(NT) void k_robust(const FlagBeamDev *__restrict__ b
   __shared__ uint32_t hist[NT];
   __shared__ uint32_t pick[2];
   __shared__ float tab[256];
-  if (SRC >= 1) tab[l] = lin < 0 ? (float)l : B.lut[((int64_t)lin * B.nch + c) * 256 + l];
+  if (SRC >= 1) {
+    const float tv = gld(B.lut + ((int64_t)max(lin, 0) * B.nch + c) * 256 + l);
+    tab[l] = lin < 0 ? (float)l : tv;
+  }
   __syncthreads();
   const float *row = B.X + (int64_t)c * B.xs;
   const uint8_t *row8 = B.X8 + (int64_t)c * B.xs;
   auto value = [&](int t) -> float {
-    if (SRC == 0) return row[t];
-    const float x = tab[row8[t]];
-    return SRC == 2 ? x / B.ts[t] : x;
+    if (SRC == 0) return gld(row + t);
+    const float x = tab[gld(row8 + t)];
+    return SRC == 2 ? x / gld(B.ts + t) : x;
   };
   const int n = B.wp;
   const uint32_t k = (uint32_t)(n - 1) / 2u;
