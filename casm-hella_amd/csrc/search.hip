@@ -484,11 +484,14 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
   uint32_t can = 0;
   if (pass == P_COUNT) {
     if (!sc.gated) {
+      // the block's maxima are the same words for every lane: scalar loads
+      // (written by the statistics kernel, an earlier launch)
       typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+      typedef __attribute__((address_space(4))) const u32x4v u32x4k;
       u32x4v bm[SEARCH_MAXW];
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k)
-        bm[k] = gld(reinterpret_cast<const u32x4v *>(B.bmax + slot(min(k, C.nw - 1)) * 4));
+        bm[k] = *(u32x4k *)(B.bmax + slot(min(k, C.nw - 1)) * 4);
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k) {
         if (k < C.nw) {
@@ -504,9 +507,10 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
       return;
     }
   } else {
+    typedef __attribute__((address_space(4))) const uint32_t u32k;
     uint32_t cn[SEARCH_MAXW];
 #pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k) cn[k] = gld(B.counts + slot(min(k, C.nw - 1)));
+    for (int k = 0; k < SEARCH_MAXW; ++k) cn[k] = *(u32k *)(B.counts + slot(min(k, C.nw - 1)));
 #pragma unroll
     for (int k = 0; k < SEARCH_MAXW; ++k)
       if (k < C.nw) can |= cn[k] ? (1u << k) : 0u;
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
       if (lane == 0) shu[kk * (NT / 64) + wv] = (uint32_t)__popcll(m[kk]);
     }
     __syncthreads();
-    uint32_t pos = gld(B.offs + slot(k));
+    uint32_t pos = *(const __attribute__((address_space(4))) uint32_t *)(B.offs + slot(k));
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
       uint32_t tot = 0, before = 0;
