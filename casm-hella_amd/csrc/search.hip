@@ -253,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_stats(const SearchBeamDev *__restrict__ 
     if (l == 0) {
 #pragma unroll
       for (int k = 0; k < NW; ++k) {
-        const int64_t slot = ((int64_t)k * B.rows + r) * C.nblk + b;
+        const int64_t slot = ((int64_t)k * C.nblk + b) * B.rows + r;  // [k][block][row]
         B.part[slot] = acc[k];
         B.part2[slot] = acc[NW + k];
       }
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(NT) void k_y0_direct(const SearchBeamDev *__restric
     }
     tree256_multi(acc, shd);
     if (l == 0) {
-      const int64_t slot = (int64_t)r * C.nblk + b;
+      const int64_t slot = (int64_t)b * B.rows + r;  // [block][row]
       B.part[slot] = acc[0];
       B.part2[slot] = acc[1];
     }
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
       a += red[w][k][0];
       add128(lo, hi, red[w][k][1], red[w][k][2]);
     }
-    const int64_t slot = ((int64_t)k * B.rows + r) * ng + g;
+    const int64_t slot = ((int64_t)k * ng + g) * B.rows + r;  // [k][group][row]
     reinterpret_cast<uint64_t *>(B.part)[slot] = a;
     reinterpret_cast<uint64_t *>(B.part2)[2 * slot] = lo;
     reinterpret_cast<uint64_t *>(B.part2)[2 * slot + 1] = hi;
@@ -641,13 +641,14 @@ __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, i
   if (exact) {
     const uint64_t *p1 = reinterpret_cast<const uint64_t *>(B.part);
     const uint64_t *p2 = reinterpret_cast<const uint64_t *>(B.part2);
-    const int64_t base = ((int64_t)k * B.rows + r) * ngroups;
+    // [k][group][row]: the lanes (consecutive rows) read consecutive words
+    const int64_t base = (int64_t)k * ngroups * B.rows + r;
     uint64_t a = 0, lo = 0, hi = 0;
     for (int g0 = 0; g0 < ngroups; g0 += RS) {
       uint64_t x1[RS], xl[RS], xh[RS];
 #pragma unroll
       for (int j = 0; j < RS; ++j) {
-        const int64_t g = base + min(g0 + j, ngroups - 1);
+        const int64_t g = base + (int64_t)min(g0 + j, ngroups - 1) * B.rows;
         x1[j] = gld(p1 + g);
         xl[j] = want2 ? gld(p2 + 2 * g) : 0ull;
         xh[j] = want2 ? gld(p2 + 2 * g + 1) : 0ull;
@@ -664,14 +665,15 @@ __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, i
     s2 = want2 ? (double)hi * 18446744073709551616.0 + (double)lo : 0.0;
     return;
   }
-  const int64_t base = ((int64_t)k * B.rows + r) * C.nblk;
+  // [k][block][row]: the lanes (consecutive rows) read consecutive words
+  const int64_t base = (int64_t)k * C.nblk * B.rows + r;
   s1 = 0.0;
   s2 = 0.0;
   for (int b0 = 0; b0 < C.nblk; b0 += RS) {
     double x1[RS], x2[RS];
 #pragma unroll
     for (int j = 0; j < RS; ++j) {
-      const int64_t b = base + min(b0 + j, C.nblk - 1);
+      const int64_t b = base + (int64_t)min(b0 + j, C.nblk - 1) * B.rows;
       x1[j] = gld(B.part + b);
       x2[j] = want2 ? gld(B.part2 + b) : 0.0;
     }

@@ -39,9 +39,10 @@ struct SearchBeamDev {
   int row0;           // first D row searched
   float *planes;      // hella mode: raw smoothed planes [nw][rows][ps]
   int64_t ps;
-  double *part;       // [nw][rows][nblk] block sums of x (BOXCAR statistics:
-                      // exact u64 sums per group of blocks)
-  double *part2;      // [nw][rows][nblk] block sums of x*x (BOXCAR statistics:
+  double *part;       // [nw][nblk][rows] block sums of x, rows innermost so a
+                      // finisher thread per row reads them coalesced (BOXCAR
+                      // statistics: exact u64 sums, [nw][groups][rows])
+  double *part2;      // [nw][nblk][rows] block sums of x*x (BOXCAR statistics:
                       // exact u128 sums per group, lo/hi u64 pairs; 2x the size)
   SScal *scal;
   uint32_t *counts;   // [nw][rows][nblk]
