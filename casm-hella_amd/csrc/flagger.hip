@@ -30,6 +30,13 @@ __device__ __forceinline__ T gld(const T *p)
   return *((gt *)p);
 }
 
+// the partial sums of (channel c, block b), channels innermost: a thread per
+// channel reads a block's partials of consecutive channels coalesced
+__device__ __forceinline__ int64_t fslot(const FlagBeamDev &B, int c, int b)
+{
+  return (int64_t)b * B.nch + c;
+}
+
 __device__ __forceinline__ uint8_t quant(float x)
 {
   // transpose_output (src/hella/src/transpose.cu:65-75); NaN -> 0
@@ -186,8 +193,8 @@ __global__ __launch_bounds__(NT) void k_apply(const FlagBeamDev *__restrict__ be
   if (part == FPART_NONE) return;
   tree256_multi(acc, sh);
   if (l == 0) {
-    B.part[(int64_t)c * B.nblk + blk] = acc[0];
-    if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk] = acc[1];
+    B.part[fslot(B, c, blk)] = acc[0];
+    if (part == FPART_SUM2) B.part2[fslot(B, c, blk)] = acc[1];
   }
 }
 
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(NT) void k_finish_bp(const FlagBeamDev *__restrict_
   const int c = blockIdx.x * NT + threadIdx.x;
   if (c >= B.nch) return;
   double s = 0.0;
-  for (int b = 0; b < B.nblk; ++b) s += B.part[(int64_t)c * B.nblk + b];
+  for (int b = 0; b < B.nblk; ++b) s += gld(B.part + fslot(B, c, b));
   B.bp[c] = (float)(s / (double)W);
   if (box > 0) {
     const double g = (double)B.gsum[c];
@@ -232,17 +239,15 @@ __device__ double flag_rows_csum(const FlagBeamDev &B, int n, double mu, bool va
       const int r = r0 + i;
       double val = 0.0;
       if (r < B.nch) {
-        const double *p1 = B.part + (int64_t)r * B.nblk;
         double s1 = 0.0;
 #pragma unroll 8
-        for (int b = 0; b < B.nblk; ++b) s1 += p1[b];
+        for (int b = 0; b < B.nblk; ++b) s1 += gld(B.part + fslot(B, r, b));
         if (!var) {
           val = s1 / dn;
         } else {
-          const double *p2 = B.part2 + (int64_t)r * B.nblk;
           double s2 = 0.0;
 #pragma unroll 8
-          for (int b = 0; b < B.nblk; ++b) s2 += p2[b];
+          for (int b = 0; b < B.nblk; ++b) s2 += gld(B.part2 + fslot(B, r, b));
           val = ((s2 - (2.0 * mu) * s1) + (dn * mu) * mu) / dn;
         }
       }
@@ -307,12 +312,10 @@ __global__ __launch_bounds__(FS_NT) void k_finish_stats(const FlagBeamDev *__res
       s1[i] = 0.0;
       s2[i] = 0.0;
       if (r < B.nch) {
-        const double *p1 = B.part + (int64_t)r * B.nblk;
-        const double *p2 = B.part2 + (int64_t)r * B.nblk;
 #pragma unroll 8
         for (int b = 0; b < B.nblk; ++b) {
-          s1[i] += p1[b];
-          s2[i] += p2[b];
+          s1[i] += gld(B.part + fslot(B, r, b));
+          s2[i] += gld(B.part2 + fslot(B, r, b));
         }
       }
     }
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(NT) void k_grow_replace(const FlagBeamDev *__restri
   if (cnt) atomicAdd(&B.flagcount[c], cnt);
   if (nbox) atomicAdd(&B.gsum[c], nbox);
   tree256_multi(acc, sh);
-  if (l == 0) B.part[(int64_t)c * B.nblk + blockIdx.x] = acc[0];
+  if (l == 0) B.part[fslot(B, c, blockIdx.x)] = acc[0];
 }
 
 // ts_correct (src/hella/src/timeseries.cu:19-74) on the bandpass-corrected
@@ -676,8 +679,8 @@ __global__ __launch_bounds__(NT) void k_apply_lut(const FlagBeamDev *__restrict_
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (blk0 + j >= B.nblk) break;
-      B.part[(int64_t)c * B.nblk + blk0 + j] = acc[2 * j];
-      if (part == FPART_SUM2) B.part2[(int64_t)c * B.nblk + blk0 + j] = acc[2 * j + 1];
+      B.part[fslot(B, c, blk0 + j)] = acc[2 * j];
+      if (part == FPART_SUM2) B.part2[fslot(B, c, blk0 + j)] = acc[2 * j + 1];
     }
   }
 }
@@ -769,8 +772,8 @@ __global__ __launch_bounds__(NT) void k_tsdiv_stats(const FlagBeamDev *__restric
 #pragma unroll
     for (int j = 0; j < TQCH; ++j) {
       if (j >= nc) break;
-      B.part[(int64_t)(c0 + j) * B.nblk + blk] = acc[2 * j];
-      B.part2[(int64_t)(c0 + j) * B.nblk + blk] = acc[2 * j + 1];
+      B.part[fslot(B, c0 + j, blk)] = acc[2 * j];
+      B.part2[fslot(B, c0 + j, blk)] = acc[2 * j + 1];
     }
   }
 }

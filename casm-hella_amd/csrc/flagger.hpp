@@ -42,8 +42,8 @@ struct FlagBeamDev {
   int64_t ring_stride; // bytes between channel rows
   int64_t ring_len;    // columns (multiple of 16); column = (col0 + t) mod ring_len
   int64_t col0;
-  double *part;        // [nch][nblk] block partial sums of x
-  double *part2;       // [nch][nblk] block partial sums of x*x
+  double *part;        // [nblk][nch] block partial sums of x (channels innermost)
+  double *part2;       // [nblk][nch] block partial sums of x*x
   float *bp;           // [nch] bandpass
   FScal *scal;
   uint32_t *flagcount; // [nch] grown-mask counts (accumulated)
