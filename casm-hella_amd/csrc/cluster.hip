@@ -4,7 +4,8 @@
 // index on ties); clusters are listed by representative index.  One
 // workgroup per beam:
 //   1. every candidate marks its cell with the lowest candidate index in it
-//      (atomic minimum into the beam's cell grid);
+//      (atomic minimum into the beam's cell grid, issued only by the first
+//      lane of each run of a wave's lanes that share a cell);
 //   2. the first candidate of each cell joins the cells around it (union-find
 //      over candidate indices, the larger root hooked under the smaller by
 //      compare-and-swap, so a component's root is its lowest cell index);
@@ -174,7 +175,13 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
     float snr;
     int samp, dm, cl;
     cand(k, snr, samp, dm, cl);
-    atomicMin(B.cl_grid + cl, (uint32_t)i);
+    // the lanes of a wave hold consecutive candidates, and the list is
+    // ordered by (width, DM row, sample), so neighbours mostly share a cell:
+    // a lane whose predecessor is in the same cell leaves the cell's minimum
+    // to it (one atomic per run of a cell instead of one per candidate; the
+    // atomics of a cell's run all serialise on one grid word)
+    const int prev = __shfl_up(cl, 1, 64);
+    if (lane == 0 || prev != cl) atomicMin(B.cl_grid + cl, (uint32_t)i);
     par[i] = (uint32_t)i;
     key[i] = 0ull;
     cnt[i] = 0u;
