@@ -453,173 +453,201 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   }
 }
 
-// Threshold passes over one (row, block):
+// Threshold passes.  A workgroup owns HIT_IPW consecutive (row, block) items
+// (item = row * nblk + block, so a width's slots of the items are
+// consecutive words):
 //  P_COUNT: crossings per width; widths whose block maximum cannot cross the
 //           threshold are 0 without reading D
 //  P_WRITE: candidates of the block, ordered by (width, row, sample)
+// One thread per item first finds the widths that can have crossings (the
+// block maxima, or the counts of the count pass); the workgroup then walks
+// only the items that have one.  Around a few bright pulses most blocks have
+// none, and they cost one load round per HIT_IPW items instead of a
+// workgroup each.
 // LX: 0, or the block's span SB + maxW when it is a multiple of NT (every
 // load, store and prefix segment full, nothing predicated)
+constexpr int HIT_IPW = 16;
+static_assert(HIT_IPW <= 64, "one wave selects the items");
+
 template <bool HELLA, int LX = 0>
 __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ beams,
                                              SearchCommon C, int pass)
 {
   const SearchBeamDev B = beams[blockIdx.z];
-  const int r = blockIdx.y;
-  if (r >= B.rows) return;
-  const int b = blockIdx.x;
-  const int base = b * SB;
+  const int64_t plane = (int64_t)B.rows * C.nblk;  // slots of one width
+  const int64_t it0 = (int64_t)blockIdx.x * HIT_IPW;
+  if (it0 >= plane) return;
   const int l = threadIdx.x;
   __shared__ __align__(16) uint32_t P[PN];
   __shared__ uint32_t wtot[NT / 64];
   __shared__ uint32_t shu[NT];
   __shared__ uint32_t shc[SEARCH_MAXW];
+  __shared__ uint32_t s_it[HIT_IPW], s_can[HIT_IPW], s_n;
   const SScalK &sc = *(SScalK *)B.scal;
-  const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
   const int c0 = C.maxW / 2;
   const float thr = sc.thr;
-  auto slot = [&](int k) { return ((int64_t)k * B.rows + r) * C.nblk + b; };
 
-  // widths of this block that can have crossings (every width's words are
-  // requested before the first is used)
-  uint32_t can = 0;
-  if (pass == P_COUNT) {
-    if (!sc.gated) {
-      // the block's maxima are the same words for every lane: scalar loads
-      // (written by the statistics kernel, an earlier launch)
-      typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
-      typedef __attribute__((address_space(4))) const u32x4v u32x4k;
-      u32x4v bm[SEARCH_MAXW];
+  // the items' widths that can have crossings (every width's words are
+  // requested before the first is used); the items with any are listed in
+  // item order
+  if (l < 64) {
+    const int64_t it = it0 + l;
+    const bool mine = l < HIT_IPW && it < plane;
+    const int64_t itc = mine ? it : it0;  // loads stay inside the slots
+    uint32_t can = 0;
+    if (pass == P_COUNT) {
+      if (!sc.gated) {
+        // each item's four wave maxima of a width in one 16-byte load
+        typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+        u32x4v bm[SEARCH_MAXW];
+#pragma unroll
+        for (int k = 0; k < SEARCH_MAXW; ++k)
+          bm[k] = gld(reinterpret_cast<const u32x4v *>(B.bmax) + min(k, C.nw - 1) * plane + itc);
+#pragma unroll
+        for (int k = 0; k < SEARCH_MAXW; ++k) {
+          if (k < C.nw) {
+            const uint32_t m = max(max(bm[k].x, bm[k].y), max(bm[k].z, bm[k].w));
+            const bool c = HELLA ? normed((double)__uint_as_float(m), sc, k) > thr
+                                 : (uint64_t)m >= sc.rthr[k];
+            can |= c ? (1u << k) : 0u;
+          }
+        }
+      }
+      if (mine && !can)
+        for (int k = 0; k < C.nw; ++k) B.counts[k * plane + it] = 0u;
+    } else {
+      uint32_t cn[SEARCH_MAXW];
+#pragma unroll
+      for (int k = 0; k < SEARCH_MAXW; ++k) cn[k] = gld(B.counts + min(k, C.nw - 1) * plane + itc);
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k)
-        bm[k] = *(u32x4k *)(B.bmax + slot(min(k, C.nw - 1)) * 4);
+        if (k < C.nw) can |= cn[k] ? (1u << k) : 0u;
+    }
+    can = mine ? can : 0u;
+    const uint64_t act = __ballot(can != 0u);
+    if (can) {
+      const int q = __popcll(act & ((1ull << l) - 1ull));
+      s_it[q] = (uint32_t)l;
+      s_can[q] = can;
+    }
+    if (l == 0) s_n = (uint32_t)__popcll(act);
+  }
+  __syncthreads();
+  const int nact = (int)__builtin_amdgcn_readfirstlane(s_n);
+
+  for (int q = 0; q < nact; ++q) {
+    const int64_t it = it0 + (int64_t)__builtin_amdgcn_readfirstlane(s_it[q]);
+    const uint32_t can = __builtin_amdgcn_readfirstlane(s_can[q]);
+    const int r = (int)(it / C.nblk);
+    const int b = (int)(it - (int64_t)r * C.nblk);
+    const int base = b * SB;
+    const uint32_t *drow = B.D + (int64_t)(r + B.row0) * B.ds;
+    if (!HELLA) {
+      // the block and its halo: every load issued before the first LDS store
+      const int L = SB + C.maxW;
+      constexpr int PFX = LX ? LX / NT : PF;
+      uint32_t v[PFX];
+#pragma unroll
+      for (int t = 0; t < PFX; ++t) v[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
+#pragma unroll
+      for (int t = 0; t < PFX; ++t) {
+        const int i = l + NT * t;
+        if (LX || i < L) P[i + 1] = base + i < C.ntime_dd ? v[t] : 0u;
+      }
+      if (l == 0) P[0] = 0u;
+      __syncthreads();
+      if constexpr (LX != 0)
+        block_prefix<PFX, true>(P, L, wtot);
+      else
+        block_prefix(P, L, wtot);
+    }
+    auto raw = [&](int k, int j) -> double {
+      if (HELLA) return (double)gld(B.planes + ((int64_t)k * B.rows + r) * B.ps + base + j);
+      const int w = C.minW << k;
+      const int a = j + c0 - w / 2;
+      return (double)(uint32_t)(P[a + w] - P[a]);
+    };
+
+    if (pass == P_COUNT) {
+      if (l < SEARCH_MAXW) shc[l] = 0u;
+      __syncthreads();
+      uint32_t cnt[SEARCH_MAXW];
+#pragma unroll
+      for (int k = 0; k < SEARCH_MAXW; ++k) cnt[k] = 0;
+      for (int kk = 0; kk < SB / NT; ++kk) {
+        const int j = l + NT * kk;
+        if (base + j >= C.nout) break;
+#pragma unroll
+        for (int k = 0; k < SEARCH_MAXW; ++k)
+          if (k < C.nw && ((can >> k) & 1u)) {
+            if (HELLA)
+              cnt[k] += normed(raw(k, j), sc, k) > thr ? 1u : 0u;
+            else
+              cnt[k] += (uint64_t)raw(k, j) >= sc.rthr[k] ? 1u : 0u;
+          }
+      }
+      // integer totals are order free: wave sums, then LDS atomics
 #pragma unroll
       for (int k = 0; k < SEARCH_MAXW; ++k) {
         if (k < C.nw) {
-          const uint32_t m = max(max(bm[k].x, bm[k].y), max(bm[k].z, bm[k].w));
-          const bool c = HELLA ? normed((double)__uint_as_float(m), sc, k) > thr
-                               : (uint64_t)m >= sc.rthr[k];
-          can |= c ? (1u << k) : 0u;
+          const uint32_t x = wave_sum(cnt[k]);
+          if ((l & 63) == 0 && x) atomicAdd(&shc[k], x);
         }
       }
-    }
-    if (!can) {
-      for (int k = l; k < C.nw; k += NT) B.counts[slot(k)] = 0u;
-      return;
-    }
-  } else {
-    typedef __attribute__((address_space(4))) const uint32_t u32k;
-    uint32_t cn[SEARCH_MAXW];
+      __syncthreads();
+      if (l < C.nw) B.counts[l * plane + it] = shc[l];
+    } else {
+      // P_WRITE: ordered by (width, row, sample).  Per width, every wave
+      // keeps the ballot of each 256-sample step and publishes its hit
+      // count; after one barrier a hit's rank is the hits of the earlier
+      // steps, of the earlier waves of its step, and of the lower lanes of
+      // its wave.
+      constexpr int KS = SB / NT;
+      const int lane = l & 63, wv = l >> 6;
+      const uint64_t below = (1ull << lane) - 1ull;
+      for (int k = 0; k < C.nw; ++k) {
+        if (!((can >> k) & 1u)) continue;
+        uint64_t m[KS];
 #pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k) cn[k] = *(u32k *)(B.counts + slot(min(k, C.nw - 1)));
-#pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k)
-      if (k < C.nw) can |= cn[k] ? (1u << k) : 0u;
-    if (!can) return;
-  }
-  if (!HELLA) {
-    // the block and its halo: every load issued before the first LDS store
-    const int L = SB + C.maxW;
-    constexpr int PFX = LX ? LX / NT : PF;
-    uint32_t v[PFX];
-#pragma unroll
-    for (int t = 0; t < PFX; ++t) v[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
-#pragma unroll
-    for (int t = 0; t < PFX; ++t) {
-      const int i = l + NT * t;
-      if (LX || i < L) P[i + 1] = base + i < C.ntime_dd ? v[t] : 0u;
-    }
-    if (l == 0) P[0] = 0u;
-    __syncthreads();
-    if constexpr (LX != 0)
-      block_prefix<PFX, true>(P, L, wtot);
-    else
-      block_prefix(P, L, wtot);
-  }
-  auto raw = [&](int k, int j) -> double {
-    if (HELLA) return (double)gld(B.planes + ((int64_t)k * B.rows + r) * B.ps + base + j);
-    const int w = C.minW << k;
-    const int a = j + c0 - w / 2;
-    return (double)(uint32_t)(P[a + w] - P[a]);
-  };
-
-  if (pass == P_COUNT) {
-    if (l < SEARCH_MAXW) shc[l] = 0u;
-    __syncthreads();
-    uint32_t cnt[SEARCH_MAXW];
-#pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k) cnt[k] = 0;
-    for (int kk = 0; kk < SB / NT; ++kk) {
-      const int j = l + NT * kk;
-      if (base + j >= C.nout) break;
-#pragma unroll
-      for (int k = 0; k < SEARCH_MAXW; ++k)
-        if (k < C.nw && ((can >> k) & 1u)) {
-          if (HELLA)
-            cnt[k] += normed(raw(k, j), sc, k) > thr ? 1u : 0u;
-          else
-            cnt[k] += (uint64_t)raw(k, j) >= sc.rthr[k] ? 1u : 0u;
-        }
-    }
-    // integer totals are order free: wave sums, then LDS atomics
-#pragma unroll
-    for (int k = 0; k < SEARCH_MAXW; ++k) {
-      if (k < C.nw) {
-        const uint32_t x = wave_sum(cnt[k]);
-        if ((l & 63) == 0 && x) atomicAdd(&shc[k], x);
-      }
-    }
-    __syncthreads();
-    if (l < C.nw) B.counts[slot(l)] = shc[l];
-    return;
-  }
-
-  // P_WRITE: ordered by (width, row, sample).  Per width, every wave keeps
-  // the ballot of each 256-sample step and publishes its hit count; after
-  // one barrier a hit's rank is the hits of the earlier steps, of the
-  // earlier waves of its step, and of the lower lanes of its wave.
-  constexpr int KS = SB / NT;
-  const int lane = l & 63, wv = l >> 6;
-  const uint64_t below = (1ull << lane) - 1ull;
-  for (int k = 0; k < C.nw; ++k) {
-    if (!((can >> k) & 1u)) continue;
-    uint64_t m[KS];
-#pragma unroll
-    for (int kk = 0; kk < KS; ++kk) {
-      const int j = l + NT * kk;
-      bool hit = false;
-      if (base + j < C.nout) {
-        const double v = raw(k, j);
-        hit = HELLA ? normed(v, sc, k) > thr : (uint64_t)v >= sc.rthr[k];
-      }
-      m[kk] = __ballot(hit);
-      if (lane == 0) shu[kk * (NT / 64) + wv] = (uint32_t)__popcll(m[kk]);
-    }
-    __syncthreads();
-    uint32_t pos = *(const __attribute__((address_space(4))) uint32_t *)(B.offs + slot(k));
-#pragma unroll
-    for (int kk = 0; kk < KS; ++kk) {
-      uint32_t tot = 0, before = 0;
-#pragma unroll
-      for (int w = 0; w < NT / 64; ++w) {
-        const uint32_t cw = shu[kk * (NT / 64) + w];
-        tot += cw;
-        before += w < wv ? cw : 0u;
-      }
-      if ((m[kk] >> lane) & 1ull) {
-        const uint32_t rank = before + (uint32_t)__popcll(m[kk] & below);
-        if (pos + rank < (uint32_t)C.cap) {
+        for (int kk = 0; kk < KS; ++kk) {
           const int j = l + NT * kk;
-          Cand c;
-          c.snr = normed(raw(k, j), sc, k);
-          c.samp = base + j;
-          c.width = k;
-          c.dm_idx = r + B.row0;
-          B.cands[pos + rank] = c;
+          bool hit = false;
+          if (base + j < C.nout) {
+            const double v = raw(k, j);
+            hit = HELLA ? normed(v, sc, k) > thr : (uint64_t)v >= sc.rthr[k];
+          }
+          m[kk] = __ballot(hit);
+          if (lane == 0) shu[kk * (NT / 64) + wv] = (uint32_t)__popcll(m[kk]);
         }
+        __syncthreads();
+        uint32_t pos = *(const __attribute__((address_space(4))) uint32_t *)(B.offs + k * plane + it);
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+          uint32_t tot = 0, before = 0;
+#pragma unroll
+          for (int w = 0; w < NT / 64; ++w) {
+            const uint32_t cw = shu[kk * (NT / 64) + w];
+            tot += cw;
+            before += w < wv ? cw : 0u;
+          }
+          if ((m[kk] >> lane) & 1ull) {
+            const uint32_t rank = before + (uint32_t)__popcll(m[kk] & below);
+            if (pos + rank < (uint32_t)C.cap) {
+              const int j = l + NT * kk;
+              Cand c;
+              c.snr = normed(raw(k, j), sc, k);
+              c.samp = base + j;
+              c.width = k;
+              c.dm_idx = r + B.row0;
+              B.cands[pos + rank] = c;
+            }
+          }
+          pos += tot;
+        }
+        __syncthreads();  // shu is rewritten by the next width
       }
-      pos += tot;
     }
-    __syncthreads();  // shu is rewritten by the next width
+    __syncthreads();  // P and shc are rewritten by the next item
   }
 }
 
@@ -995,6 +1023,9 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
   const int bpw = stats_bpw(C, max_rows, nbeam);
   const dim3 gs((unsigned)((C.nblk + bpw - 1) / bpw), (unsigned)max_rows, (unsigned)nbeam);
   const dim3 gp((unsigned)C.nblk, (unsigned)max_rows, (unsigned)nbeam);
+  // threshold passes: HIT_IPW (row, block) items per workgroup
+  const int64_t nitem = (int64_t)max_rows * C.nblk;
+  const dim3 gh((unsigned)((nitem + HIT_IPW - 1) / HIT_IPW), 1u, (unsigned)nbeam);
   auto pass = [&](int p) {
     if (p == P_Y0) {
       if (hella)
@@ -1006,18 +1037,18 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
       else
         launch_stats<false>(d_beams, C, p, bpw, gs, s);
     } else if (hella) {
-      hipLaunchKernelGGL(k_hits<true>, gp, dim3(NT), 0, s, d_beams, C, p);
+      hipLaunchKernelGGL(k_hits<true>, gh, dim3(NT), 0, s, d_beams, C, p);
     } else {
       // the usual widest boxes get the kernel with every block span full
       const int lx = SB + C.maxW;
       if (lx == SB + 256)
-        hipLaunchKernelGGL((k_hits<false, SB + 256>), gp, dim3(NT), 0, s, d_beams, C, p);
+        hipLaunchKernelGGL((k_hits<false, SB + 256>), gh, dim3(NT), 0, s, d_beams, C, p);
       else if (lx == SB + 512)
-        hipLaunchKernelGGL((k_hits<false, SB + 512>), gp, dim3(NT), 0, s, d_beams, C, p);
+        hipLaunchKernelGGL((k_hits<false, SB + 512>), gh, dim3(NT), 0, s, d_beams, C, p);
       else if (lx == SB + 1024)
-        hipLaunchKernelGGL((k_hits<false, SB + 1024>), gp, dim3(NT), 0, s, d_beams, C, p);
+        hipLaunchKernelGGL((k_hits<false, SB + 1024>), gh, dim3(NT), 0, s, d_beams, C, p);
       else
-        hipLaunchKernelGGL(k_hits<false>, gp, dim3(NT), 0, s, d_beams, C, p);
+        hipLaunchKernelGGL(k_hits<false>, gh, dim3(NT), 0, s, d_beams, C, p);
     }
   };
   auto fin = [&](int p) {
