@@ -21,9 +21,12 @@
 // MAX_GIANTS in all (the beams before this one counted in order).
 // The union-find parents live in LDS for lists of up to PCAP candidates, the
 // other per-candidate words (keys, counts, bounds) for up to WCAP; beyond,
-// the work set's buffers hold them; the same generic pointers serve both.  Reads of words other waves change by
-// atomics use device-scope atomic loads (for global words they bypass the
-// CU's L1).
+// the work set's buffers hold them; the same generic pointers serve both.
+// A beam's words and cell grid are touched by its workgroup alone during the
+// launch, so every atomic, and every read of a word other waves change by
+// atomics, is a workgroup-scope atomic (a device-scope one would have to be
+// made coherent across the chip's L2s), and the phases are ordered by the
+// workgroup barriers alone.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -39,14 +42,41 @@ constexpr int WCAP = 3072;   // candidates whose other words fit in LDS (28 B ea
 constexpr size_t CLUSTER_LDS = (size_t)WCAP * 8 + (size_t)PCAP * 4 + (size_t)5 * WCAP * 4;
 constexpr uint32_t NONE = 0xffffffffu;
 
+constexpr int WG = __HIP_MEMORY_SCOPE_WORKGROUP;
+
 __device__ __forceinline__ uint32_t ld(const uint32_t *p)
 {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, WG);
 }
 
 __device__ __forceinline__ unsigned long long ld64(const unsigned long long *p)
 {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, WG);
+}
+
+template <class T>
+__device__ __forceinline__ void amin(T *p, T v)
+{
+  (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, WG);
+}
+
+template <class T>
+__device__ __forceinline__ void amax(T *p, T v)
+{
+  (void)__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, WG);
+}
+
+template <class T>
+__device__ __forceinline__ void aadd(T *p, T v)
+{
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, WG);
+}
+
+// compare-and-swap; the word's value before the operation
+__device__ __forceinline__ uint32_t acas(uint32_t *p, uint32_t expect, uint32_t want)
+{
+  __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, WG);
+  return expect;
 }
 
 // root of x: parents only ever point to lower indices
@@ -72,7 +102,7 @@ __device__ __forceinline__ void unite(uint32_t *par, uint32_t a, uint32_t b)
       b = t;
     }
     // hook the larger root under the smaller; retry if a was hooked meanwhile
-    if (atomicCAS(par + a, a, b) == a) return;
+    if (acas(par + a, a, b) == a) return;
   }
 }
 
@@ -181,7 +211,7 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
     // to it (one atomic per run of a cell instead of one per candidate; the
     // atomics of a cell's run all serialise on one grid word)
     const int prev = __shfl_up(cl, 1, 64);
-    if (lane == 0 || prev != cl) atomicMin(B.cl_grid + cl, (uint32_t)i);
+    if (lane == 0 || prev != cl) amin(B.cl_grid + cl, (uint32_t)i);
     par[i] = (uint32_t)i;
     key[i] = 0ull;
     cnt[i] = 0u;
@@ -190,7 +220,6 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
     dmin[i] = NONE;
     dmax[i] = 0u;
   }
-  __threadfence();
   __syncthreads();
   // the head of every candidate's cell (fixed from here on)
   auto head = [&](int k, int cl) -> uint32_t {
@@ -230,7 +259,6 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
     for (int j = 0; j < 8; ++j)
       if (g[j] != NONE) unite(par, (uint32_t)i, g[j]);
   }
-  __threadfence();
   __syncthreads();
   // 3. every candidate into its root (all lanes take part in every round)
 #pragma unroll UNR
@@ -271,25 +299,24 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
         d1 = max(d1, (uint32_t)__shfl_xor((int)d1, o, 64));
       }
       if (lane == lead) {
-        atomicMax(key + r, kk);
-        atomicAdd(cnt + r, cc);
-        atomicMin(smin + r, s0);
-        atomicMax(smax + r, s1);
-        atomicMin(dmin + r, d0);
-        atomicMax(dmax + r, d1);
+        amax(key + r, kk);
+        aadd(cnt + r, cc);
+        amin(smin + r, s0);
+        amax(smax + r, s1);
+        amin(dmin + r, d0);
+        amax(dmax + r, d1);
       }
       act = act && !mine;
     }
     if (act) {
-      atomicMax(key + root, kv);
-      atomicAdd(cnt + root, 1u);
-      atomicMin(smin + root, sm);
-      atomicMax(smax + root, sm);
-      atomicMin(dmin + root, dmv);
-      atomicMax(dmax + root, dmv);
+      amax(key + root, kv);
+      aadd(cnt + root, 1u);
+      amin(smin + root, sm);
+      amax(smax + root, sm);
+      amin(dmin + root, dmv);
+      amax(dmax + root, dmv);
     }
   }
-  __threadfence();
   __syncthreads();
   // 4. representatives in index order
   uint32_t run = 0;
