@@ -328,9 +328,13 @@ __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo,
 // squares of one block fit a u64 before they are folded into the u128.
 // W1: widths 1, 2, 4, ... (WIDTH_MIN 1), compile-time LDS offsets.
 // NARROW: every box sum is so small that a thread's squares over all its
-// blocks fit a u64 (box_max^2 x 16 x bpw < 2^64, checked on the host); the
-// thread then keeps one u64 per width and only the reduction widens to u128,
-// which frees the registers of the per-thread u128 sums.
+// blocks fit a u64 (box_max^2 x 16 x bpw < 2^64) and its 16 box sums of one
+// block fit a u32 (box_max < 2^28), both checked on the host; the thread then
+// keeps one u64 of squares per width, adds a block's boxes in a u32, and only
+// the reduction widens to u128, which frees the registers of the per-thread
+// u128 sums.
+// The row is read as a buffer of ntime_dd words, so a load past its end
+// returns 0 without a clamp or a select.
 template <int NW, bool W1, bool NARROW>
 __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C, int bpw)
@@ -359,11 +363,14 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
   constexpr int PFW = W1 ? (LW + NT - 1) / NT : PF;
   constexpr bool EX = W1 && LW % NT == 0;
   uint32_t nx[PFW];
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)drow, (short)0, C.ntime_dd * 4, 0x00020000);
   auto fetch = [&](int b) {
-    const int base = b * SB;
+    const int vo = (b * SB + l) * 4;  // the whole offset in the range-checked part
 #pragma unroll
-    for (int t = 0; t < PFW; ++t) nx[t] = gld(drow + min(base + l + NT * t, C.ntime_dd - 1));
+    for (int t = 0; t < PFW; ++t) nx[t] = __builtin_amdgcn_raw_buffer_load_b32(rs, vo + NT * 4 * t, 0, 0);
   };
+  uint32_t s1b[NW];  // NARROW: this block's box sums
   auto body = [&](int j, auto in_stats) {
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
@@ -372,7 +379,10 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
       const uint32_t box = P[a + w] - P[a];
       mx[k] = max(mx[k], box);
       if constexpr (decltype(in_stats)::value) {
-        s1[k] += box;
+        if (NARROW)
+          s1b[k] += box;
+        else
+          s1[k] += box;
         q[k] += (uint64_t)box * box;
       }
     }
@@ -383,7 +393,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
 #pragma unroll
     for (int t = 0; t < PFW; ++t) {
       const int i = l + NT * t;
-      if (EX || i < L) P[1 + i] = base + i < C.ntime_dd ? nx[t] : 0u;
+      if (EX || i < L) P[1 + i] = nx[t];
     }
     if (l == 0) P[0] = 0u;
     if (b + 1 < b1) fetch(b + 1);
@@ -395,6 +405,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
       if (!NARROW) q[k] = 0ull;
+      s1b[k] = 0u;
       mx[k] = 0u;
     }
     // iterations whose 256 samples are all searched / all in the statistics
@@ -416,6 +427,7 @@ __global__ __launch_bounds__(NT) void k_stats_box(const SearchBeamDev *__restric
     }
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
+      if (NARROW) s1[k] += s1b[k];
       if (!NARROW) add128(s2lo[k], s2hi[k], q[k], 0ull);
       const uint32_t m = wave_max(mx[k]);
       if (lane == 0) B.bmax[(((int64_t)k * B.rows + r) * C.nblk + b) * 4 + wv] = m;
@@ -948,9 +960,10 @@ static void launch_stats(const SearchBeamDev *d_beams, const SearchCommon &C, in
 static void launch_stats_box(const SearchBeamDev *d_beams, const SearchCommon &C, int bpw,
                              dim3 grid, hipStream_t s)
 {
-  // a thread adds at most 16 squares per block over bpw blocks
+  // a thread adds at most 16 box sums and 16 squares per block, over bpw blocks
   const unsigned __int128 bm = (unsigned __int128)(uint64_t)std::max<int64_t>(C.box_max, 0);
-  const bool narrow = C.box_max > 0 && bm * bm * 16u * (unsigned)bpw < ((unsigned __int128)1 << 64);
+  const bool narrow = C.box_max > 0 && C.box_max < (1ll << 28) &&
+                      bm * bm * 16u * (unsigned)bpw < ((unsigned __int128)1 << 64);
   switch (C.nw) {
 #define HAMD_BOX_CASE(N)                                                                        \
   case N:                                                                                       \
