@@ -424,6 +424,35 @@ void cluster_kernel_setup()
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)CLUSTER_LDS));
 }
 
+namespace {
+
+// A beam's results into pinned host memory, HV_WG workgroups striding over
+// the lists: only what exists is written (one pass over PCIe, no copy-engine
+// transfers)
+constexpr int HV_WG = 8;
+
+__global__ __launch_bounds__(256) void k_harvest(const SearchBeamDev *__restrict__ beam,
+                                                 uint32_t *h_n, Cand *h_c, int cap_c,
+                                                 ClusterRec *h_k, int cap_k)
+{
+  const SearchBeamDev B = *beam;
+  const uint32_t nc = B.ncand[0], nk = B.cl_n[0];
+  const int tc = (int)min(nc, (uint32_t)cap_c), tk = (int)min(nk, (uint32_t)cap_k);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  if (g < 3) h_n[g] = B.ncand[g];
+  for (int j = g; j < tc; j += stride) h_c[j] = B.cands[j];
+  for (int j = g; j < tk; j += stride) h_k[j] = B.clusters[j];
+}
+
+}  // namespace
+
+void SearchRunner::harvest(const SearchBeamDev *d_beam, uint32_t *h_n, Cand *h_c, int cap_c,
+                           ClusterRec *h_k, int cap_k, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_harvest, dim3(HV_WG), dim3(256), 0, s, d_beam, h_n, h_c, cap_c, h_k, cap_k);
+  HAMD_CHECK(hipGetLastError());
+}
+
 void SearchRunner::cluster(const SearchBeamDev *d_beams, int nbeam, const ClusterArgs &A,
                            hipStream_t s)
 {

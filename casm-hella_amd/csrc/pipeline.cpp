@@ -711,16 +711,12 @@ int ctx_find_peaks(hella_ctx c, int b)
     SearchRunner::peaks(sd, 1, B.rows, c->common, c->stream);
     SearchRunner::cluster(sd, 1, cluster_args(c->cfg, rep0, false), c->stream);
     // the counts, the candidates the host can keep and the first KSTAGE_N
-    // clusters in one round trip (pinned staging); longer cluster lists
-    // fetch the rest after the counts
+    // clusters in one round trip: the device writes them into the pinned
+    // staging; longer cluster lists fetch the rest after the counts
     uint32_t *hn = c->h_ncand + 4 * (size_t)b;
     const size_t cap = std::min<size_t>(c->cstage_n, (size_t)w.s_cands.bytes / sizeof(Cand));
     const size_t kcap = std::min<size_t>(KSTAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
-    HAMD_CHECK(hipMemcpyAsync(hn, w.s_ncand.p, 12, hipMemcpyDeviceToHost, c->stream));
-    HAMD_CHECK(hipMemcpyAsync(c->h_cstage, w.s_cands.p, cap * sizeof(Cand), hipMemcpyDeviceToHost,
-                              c->stream));
-    HAMD_CHECK(hipMemcpyAsync(c->h_kstage, w.c_out.p, kcap * sizeof(ClusterRec),
-                              hipMemcpyDeviceToHost, c->stream));
+    SearchRunner::harvest(sd, hn, c->h_cstage, (int)cap, c->h_kstage, (int)kcap, c->stream);
     HAMD_CHECK(hipStreamSynchronize(c->stream));
     nc = hn[0];
     cs.resize(beam_take(c->cfg, nc));

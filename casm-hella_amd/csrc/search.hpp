@@ -107,6 +107,12 @@ struct SearchRunner {
                     hipStream_t s);
   // clusters of every beam's taken candidates (one workgroup per beam)
   static void cluster(const SearchBeamDev *d_beams, int nbeam, const ClusterArgs &A, hipStream_t s);
+  // one beam's results written by the device straight into pinned host
+  // memory: its three counts (crossings, clusters, candidates taken) into
+  // h_n[0..2], its first min(crossings, cap_c) candidates into h_c and its
+  // first min(clusters, cap_k) clusters into h_k
+  static void harvest(const SearchBeamDev *d_beam, uint32_t *h_n, Cand *h_c, int cap_c,
+                      ClusterRec *h_k, int cap_k, hipStream_t s);
 };
 
 }  // namespace hamd
