@@ -433,8 +433,11 @@ void ctx_build(hella_ctx c)
   HAMD_CHECK(hipHostMalloc((void **)&c->h_ncand, 4 * nb * sizeof(uint32_t), hipHostMallocDefault));
   c->cstage_n = std::max<size_t>(1, std::min<size_t>((size_t)std::max(cfg.max_giants, 0),
                                                       (size_t)std::max(cfg.cand_cap, 0)));
-  HAMD_CHECK(hipHostMalloc((void **)&c->h_cstage, c->cstage_n * sizeof(Cand), hipHostMallocDefault));
-  HAMD_CHECK(hipHostMalloc((void **)&c->h_kstage, KSTAGE_N * sizeof(ClusterRec), hipHostMallocDefault));
+  // one staging slot per beam (the batched path harvests every beam at once)
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_cstage, nb * c->cstage_n * sizeof(Cand),
+                           hipHostMallocDefault));
+  HAMD_CHECK(hipHostMalloc((void **)&c->h_kstage, nb * KSTAGE_N * sizeof(ClusterRec),
+                           hipHostMallocDefault));
   c->d_fdesc.ensure(nb * sizeof(FlagBeamDev));
   c->d_ddesc.ensure(nb * sizeof(DDBeamDev));
   c->d_sdesc.ensure(nb * sizeof(SearchBeamDev));
@@ -580,11 +583,13 @@ ClusterArgs cluster_args(const Config &cfg, int prefix, bool cumulative)
   return a;
 }
 
-void append_clusters(const std::vector<ClusterRec> &cl, int rep0, int beam_id,
+void append_clusters(const ClusterRec *cl, size_t n, int rep0, int beam_id,
                      std::vector<hella_cluster> &out)
 {
   static_assert(sizeof(ClusterRec) == sizeof(hella_cluster), "cluster record layout");
-  for (const ClusterRec &x : cl) {
+  out.reserve(out.size() + n);
+  for (size_t i = 0; i < n; ++i) {
+    const ClusterRec &x = cl[i];
     hella_cluster h;
     h.rep = rep0 + x.rep;
     h.count = x.count;
@@ -679,11 +684,13 @@ static size_t beam_take(const Config &cfg, uint32_t ncand)
 }
 
 // append beam b's candidates to a gulp list holding at most cap entries
-void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id,
-                       const std::vector<Cand> &cs, size_t cap, std::vector<hella_candidate> &out)
+void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id, const Cand *cs,
+                       size_t n, size_t cap, std::vector<hella_candidate> &out)
 {
   const long long a0 = window_abs0(cfg, W, B) + cfg.width_max / 2;
-  for (const Cand &x : cs) {
+  if (out.size() < cap) out.reserve(std::min(cap, out.size() + n));
+  for (size_t i = 0; i < n; ++i) {
+    const Cand &x = cs[i];
     if (out.size() >= cap) break;
     hella_candidate h;
     h.snr = x.snr;
@@ -697,13 +704,27 @@ void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id
   }
 }
 
+// records of a work set's lists that fit one staging slot
+static size_t stage_cap(hella_ctx c, const WorkSet &w)
+{
+  return std::min<size_t>(c->cstage_n, (size_t)w.s_cands.bytes / sizeof(Cand));
+}
+
+static size_t kstage_cap(const WorkSet &w)
+{
+  return std::min<size_t>(KSTAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
+}
+
 int ctx_find_peaks(hella_ctx c, int b)
 {
   BeamState &B = c->beams[(size_t)b];
   WorkSet &w = c->ws[0];
   uint32_t nc = 0;
-  std::vector<Cand> cs;
-  std::vector<ClusterRec> cl;
+  size_t ntake = 0, ncl = 0;
+  const Cand *cs = nullptr;
+  const ClusterRec *cl = nullptr;
+  std::vector<Cand> spill_c;
+  std::vector<ClusterRec> spill_k;
   const int rep0 = (int)c->peaks.size();
   {
     StageTimer t(c, 3);
@@ -714,29 +735,20 @@ int ctx_find_peaks(hella_ctx c, int b)
     // clusters in one round trip: the device writes them into the pinned
     // staging; longer cluster lists fetch the rest after the counts
     uint32_t *hn = c->h_ncand + 4 * (size_t)b;
-    const size_t cap = std::min<size_t>(c->cstage_n, (size_t)w.s_cands.bytes / sizeof(Cand));
-    const size_t kcap = std::min<size_t>(KSTAGE_N, (size_t)w.c_out.bytes / sizeof(ClusterRec));
+    const size_t cap = stage_cap(c, w), kcap = kstage_cap(w);
     SearchRunner::harvest(sd, hn, c->h_cstage, (int)cap, c->h_kstage, (int)kcap, c->stream);
     HAMD_CHECK(hipStreamSynchronize(c->stream));
     nc = hn[0];
-    cs.resize(beam_take(c->cfg, nc));
-    cl.resize(hn[1]);
-    const size_t c0 = std::min(cs.size(), cap), k0 = std::min(cl.size(), kcap);
-    std::copy(c->h_cstage, c->h_cstage + c0, cs.begin());
-    std::copy(c->h_kstage, c->h_kstage + k0, cl.begin());
-    if (cs.size() > c0)
-      HAMD_CHECK(hipMemcpyAsync(cs.data() + c0, w.s_cands.as<Cand>() + c0,
-                                (cs.size() - c0) * sizeof(Cand), hipMemcpyDeviceToHost, c->stream));
-    if (cl.size() > k0)
-      HAMD_CHECK(hipMemcpyAsync(cl.data() + k0, w.c_out.as<ClusterRec>() + k0,
-                                (cl.size() - k0) * sizeof(ClusterRec), hipMemcpyDeviceToHost,
-                                c->stream));
-    if (cs.size() > c0 || cl.size() > k0) HAMD_CHECK(hipStreamSynchronize(c->stream));
+    ntake = beam_take(c->cfg, nc);
+    ncl = hn[1];
+    cs = results_view<Cand>(c->h_cstage, cap, w.s_cands.as<Cand>(), ntake, spill_c, c->stream);
+    cl = results_view<ClusterRec>(c->h_kstage, kcap, w.c_out.as<ClusterRec>(), ncl, spill_k,
+                                  c->stream);
   }
   B.crossings = nc;
-  append_candidates(c->cfg, c->W, B, b, cs, (size_t)c->cfg.max_giants, c->peaks);
-  append_clusters(cl, rep0, b, c->clusters);
-  return (int)cs.size();
+  append_candidates(c->cfg, c->W, B, b, cs, ntake, (size_t)c->cfg.max_giants, c->peaks);
+  append_clusters(cl, ncl, rep0, b, c->clusters);
+  return (int)ntake;
 }
 
 void ctx_end_gulp(hella_ctx c)
@@ -843,10 +855,12 @@ static void batch_record(hella_ctx c)
   SearchRunner::run(c->d_sdesc.as<SearchBeamDev>(), nr, max_rows, c->common,
                     c->d_kern.as<float>(), c->kofs, s);
   SearchRunner::cluster(c->d_sdesc.as<SearchBeamDev>(), nr, cluster_args(c->cfg, 0, true), s);
+  // every beam's counts, candidates and first clusters into its staging slot
   for (int i = 0; i < nr; ++i) {
-    const int b = bt.beams[(size_t)i];
-    HAMD_CHECK(hipMemcpyAsync(&c->h_ncand[4 * (size_t)i], c->ws[(size_t)b].s_ncand.p, 12,
-                              hipMemcpyDeviceToHost, s));
+    const WorkSet &w = c->ws[(size_t)bt.beams[(size_t)i]];
+    SearchRunner::harvest(c->d_sdesc.as<SearchBeamDev>() + i, &c->h_ncand[4 * (size_t)i],
+                          c->h_cstage + (size_t)i * c->cstage_n, (int)stage_cap(c, w),
+                          c->h_kstage + (size_t)i * KSTAGE_N, (int)kstage_cap(w), s);
   }
 }
 
@@ -900,32 +914,25 @@ void ctx_process_batched(hella_ctx c, const void *gulp, bool host_ptr)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
-  // candidates and clusters: only beams with crossings are copied back
-  std::vector<std::vector<Cand>> per(bt.beams.size());
-  std::vector<std::vector<ClusterRec>> pcl(bt.beams.size());
-  for (size_t i = 0; i < bt.beams.size(); ++i) {
-    const uint32_t *hn = c->h_ncand + 4 * i;
-    const WorkSet &w = c->ws[(size_t)bt.beams[i]];
-    const size_t n = beam_take(c->cfg, hn[0]);
-    if (n) {
-      per[i].resize(n);
-      HAMD_CHECK(hipMemcpyAsync(per[i].data(), w.s_cands.p, n * sizeof(Cand),
-                                hipMemcpyDeviceToHost, c->stream));
-    }
-    if (hn[1]) {
-      pcl[i].resize(hn[1]);
-      HAMD_CHECK(hipMemcpyAsync(pcl[i].data(), w.c_out.p, hn[1] * sizeof(ClusterRec),
-                                hipMemcpyDeviceToHost, c->stream));
-    }
-  }
-  HAMD_CHECK(hipStreamSynchronize(c->stream));
+  // candidates and clusters from the staging slots the graph filled (lists
+  // longer than a slot fetch the rest)
+  std::vector<Cand> spill_c;
+  std::vector<ClusterRec> spill_k;
   for (size_t i = 0; i < bt.beams.size(); ++i) {
     if ((int)c->peaks.size() >= c->cfg.max_giants) break;
     BeamState &B = c->beams[(size_t)bt.beams[i]];
-    B.crossings = c->h_ncand[4 * i];
+    const WorkSet &w = c->ws[(size_t)bt.beams[i]];
+    const uint32_t *hn = c->h_ncand + 4 * i;
+    B.crossings = hn[0];
+    const size_t n = beam_take(c->cfg, hn[0]), ncl = hn[1];
+    const Cand *cs = results_view<Cand>(c->h_cstage + i * c->cstage_n, stage_cap(c, w),
+                                        w.s_cands.as<Cand>(), n, spill_c, c->stream);
+    const ClusterRec *cl = results_view<ClusterRec>(c->h_kstage + i * KSTAGE_N, kstage_cap(w),
+                                                    w.c_out.as<ClusterRec>(), ncl, spill_k,
+                                                    c->stream);
     const int rep0 = (int)c->peaks.size();
-    append_candidates(c->cfg, c->W, B, bt.beams[i], per[i], (size_t)c->cfg.max_giants, c->peaks);
-    append_clusters(pcl[i], rep0, bt.beams[i], c->clusters);
+    append_candidates(c->cfg, c->W, B, bt.beams[i], cs, n, (size_t)c->cfg.max_giants, c->peaks);
+    append_clusters(cl, ncl, rep0, bt.beams[i], c->clusters);
   }
   ctx_end_gulp(c);
 }

@@ -161,15 +161,30 @@ bool beam_ready(const Config &cfg, const BeamState &B);
 SearchCommon search_common(const Config &cfg, const BeamState &B0, int nw);
 // append a beam's search results (first entries of its ordered list) to a
 // candidate list holding at most cap entries
-void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id,
-                       const std::vector<Cand> &cs, size_t cap, std::vector<hella_candidate> &out);
+void append_candidates(const Config &cfg, int W, const BeamState &B, int beam_id, const Cand *cs,
+                       size_t n, size_t cap, std::vector<hella_candidate> &out);
 // clustering arguments (SearchRunner::cluster): prefix candidates already in
 // the list; cumulative: the launch's beams share one list, in order
 ClusterArgs cluster_args(const Config &cfg, int prefix, bool cumulative);
 // append a beam's clusters (device records; rep0 = the beam's first index in
 // the candidate list)
-void append_clusters(const std::vector<ClusterRec> &cl, int rep0, int beam_id,
+void append_clusters(const ClusterRec *cl, size_t n, int rep0, int beam_id,
                      std::vector<hella_cluster> &out);
+// the first n records of a device list whose first `staged` are already in
+// pinned staging: a pointer into the staging when they all are, else the
+// staged ones plus the rest copied from the device into `spill` (synchronous)
+template <class T>
+const T *results_view(const T *staged_h, size_t staged, const T *dev, size_t n,
+                      std::vector<T> &spill, hipStream_t s)
+{
+  if (n <= staged) return staged_h;
+  spill.resize(n);
+  std::copy(staged_h, staged_h + staged, spill.begin());
+  HAMD_CHECK(hipMemcpyAsync(spill.data() + staged, dev + staged, (n - staged) * sizeof(T),
+                            hipMemcpyDeviceToHost, s));
+  HAMD_CHECK(hipStreamSynchronize(s));
+  return spill.data();
+}
 // default dedispersion tile shape for a set of beams
 void choose_default_tile(const std::vector<BeamState *> &beams, DDTile &tile, int &order);
 void build_plain_tiles(BeamState &B, const DDTile &t, int order);

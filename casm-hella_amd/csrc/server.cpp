@@ -61,10 +61,10 @@ struct Graph {
   int ntiles = 0;
 };
 
-// Results of a lane's chunks come back in one copy: the gather kernel puts
-// each work set's counts, first candidates and first clusters side by side,
-// and one device-to-host copy moves them to pinned memory.  Longer lists
-// fetch the rest after the counts.
+// Results of a lane's chunks come back in one pass: the gather kernel writes
+// each work set's counts, first candidates and first clusters straight into
+// the lane's pinned staging (only the entries that exist; no copy-engine
+// transfer).  Longer lists fetch the rest after the counts.
 constexpr int SRV_KC = 256;  // candidates staged per chunk
 constexpr int SRV_KK = 64;   // clusters staged per chunk
 
@@ -105,7 +105,7 @@ struct Lane {
   DDBeamDev *h_ddesc = nullptr;
   SearchBeamDev *h_sdesc = nullptr;
   StageOut *h_stage = nullptr;  // pinned: one StageOut per work set
-  DevBuf d_fdesc, d_ddesc, d_sdesc, d_in, d_slots, d_stage;
+  DevBuf d_fdesc, d_ddesc, d_sdesc, d_in, d_slots;
   std::map<int, Graph> flag_g, search_g;  // by batch size
   std::vector<Job> jobs;
   bool busy = false;
@@ -279,7 +279,6 @@ void setup_lanes(hella_server s)
     L.d_ddesc.ensure(nb * sizeof(DDBeamDev));
     L.d_sdesc.ensure(nb * sizeof(SearchBeamDev));
     L.d_in.ensure((size_t)nb * round_up(s->chunk_bytes, 256));
-    L.d_stage.ensure(nb * sizeof(StageOut));
     std::vector<StageSlot> slots((size_t)nb);
     for (int i = 0; i < nb; ++i) {
       const WorkSet &w = L.ws[(size_t)i];
@@ -415,10 +414,8 @@ void record_flag(hella_server s, Lane &L, int n)
 void stage_results(Lane &L, int n)
 {
   hipLaunchKernelGGL(k_stage_results, dim3((unsigned)n), dim3(256), 0, L.stream,
-                     L.d_slots.as<StageSlot>(), L.d_stage.as<StageOut>(), n);
+                     L.d_slots.as<StageSlot>(), L.h_stage, n);
   HAMD_CHECK(hipGetLastError());
-  HAMD_CHECK(hipMemcpyAsync(L.h_stage, L.d_stage.p, (size_t)n * sizeof(StageOut),
-                            hipMemcpyDeviceToHost, L.stream));
 }
 
 void record_search(hella_server s, Lane &L, int nr, const DDTileDesc *tiles, int ntiles)
@@ -560,22 +557,16 @@ void harvest(hella_server s, hella_ring ring, Lane &L)
       const uint32_t nc = so.n[0], ncl = so.n[1];
       size_t take = std::min<size_t>(nc, (size_t)s->cfg.cand_cap);
       take = std::min<size_t>(take, (size_t)s->cfg.max_giants);
-      std::vector<Cand> cs(take);
-      std::vector<ClusterRec> cl(ncl);
-      const size_t c0 = std::min<size_t>(take, SRV_KC), k0 = std::min<size_t>(ncl, SRV_KK);
-      std::copy(so.c, so.c + c0, cs.begin());
-      std::copy(so.k, so.k + k0, cl.begin());
-      if (take > c0)
-        HAMD_CHECK(hipMemcpyAsync(cs.data() + c0, L.ws[i].s_cands.as<Cand>() + c0,
-                                  (take - c0) * sizeof(Cand), hipMemcpyDeviceToHost, L.stream));
-      if (ncl > k0)
-        HAMD_CHECK(hipMemcpyAsync(cl.data() + k0, L.ws[i].c_out.as<ClusterRec>() + k0,
-                                  (ncl - k0) * sizeof(ClusterRec), hipMemcpyDeviceToHost, L.stream));
-      if (take > c0 || ncl > k0) HAMD_CHECK(hipStreamSynchronize(L.stream));
+      std::vector<Cand> spill_c;
+      std::vector<ClusterRec> spill_k;
+      const Cand *cs = results_view<Cand>(so.c, SRV_KC, L.ws[i].s_cands.as<Cand>(), take, spill_c,
+                                          L.stream);
+      const ClusterRec *cl = results_view<ClusterRec>(so.k, SRV_KK, L.ws[i].c_out.as<ClusterRec>(),
+                                                      ncl, spill_k, L.stream);
       B.crossings = nc;
       res.r.crossings = nc;
-      append_candidates(s->cfg, s->W, B, sb.id, cs, (size_t)s->cfg.max_giants, res.cands);
-      append_clusters(cl, 0, sb.id, res.clusters);
+      append_candidates(s->cfg, s->W, B, sb.id, cs, take, (size_t)s->cfg.max_giants, res.cands);
+      append_clusters(cl, ncl, 0, sb.id, res.clusters);
     }
     res.r.ncand = (int)res.cands.size();
     res.r.nclusters = (int)res.clusters.size();
