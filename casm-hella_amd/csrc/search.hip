@@ -57,14 +57,19 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total)
   return before + incl - v;
 }
 
+// inclusive prefix sum over the wave (u32, wrapping), all in the vector ALU:
+// a Hillis-Steele scan within each 16-lane row by DPP row shifts (a lane
+// with no source lane adds 0), then the row totals carried forward by the
+// row broadcasts of lane 15 (into rows 1 and 3) and of lane 31 (into rows 2
+// and 3).  (Shuffles through ds_bpermute would wait on LDS latency six times.)
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
   return v;
 }
 
