@@ -12,8 +12,8 @@
 //   3. every candidate folds itself into its root: the (S/N, -index) key by
 //      atomic maximum, the count, and the sample and DM bounds; lanes of a
 //      wave that share a root (neighbouring candidates mostly do) first
-//      combine their values by shuffles, so a large cluster does not
-//      serialise one atomic per candidate on its root;
+//      combine their values by DPP reductions over the wave, so a large
+//      cluster does not serialise one atomic per candidate on its root;
 //   4. the candidates that represent their cluster are compacted in index
 //      order (wave ballots, workgroup scan).
 // The candidates taken are those the host keeps: the first min(crossings,
@@ -31,6 +31,7 @@
 
 #include <type_traits>
 
+#include "reduce.hpp"
 #include "search.hpp"
 
 namespace hamd {
@@ -127,12 +128,7 @@ __device__ __forceinline__ uint32_t ordered(float f)
 __device__ uint32_t scan_excl(uint32_t v, uint32_t *sh, uint32_t *total)
 {
   const int l = threadIdx.x, lane = l & 63, wv = l >> 6;
-  uint32_t incl = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
-  }
+  const uint32_t incl = wave_incl_scan(v);
   if (lane == 63) sh[wv] = incl;
   __syncthreads();
   uint32_t before = 0, tot = 0;
@@ -210,7 +206,7 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
     // a lane whose predecessor is in the same cell leaves the cell's minimum
     // to it (one atomic per run of a cell instead of one per candidate; the
     // atomics of a cell's run all serialise on one grid word)
-    const int prev = __shfl_up(cl, 1, 64);
+    const int prev = __builtin_amdgcn_update_dpp(0, cl, 0x138, 0xF, 0xF, false);  // wave_shr:1
     if (lane == 0 || prev != cl) amin(B.cl_grid + cl, (uint32_t)i);
     par[i] = (uint32_t)i;
     key[i] = 0ull;
@@ -277,27 +273,19 @@ __device__ __forceinline__ void cluster_phases(const SearchBeamDev &B, const Clu
       sm = (uint32_t)samp;
       dmv = (uint32_t)dm;
     }
-    // up to four groups of lanes with a common root, combined by shuffles
-    // (identities from the lanes outside the group); the rest add directly
+    // up to four groups of lanes with a common root, each combined over the
+    // wave by DPP reductions (identities from the lanes outside the group);
+    // the rest add directly
     for (int g = 0; g < 4; ++g) {
       const unsigned long long pend = __ballot(act);
       if (!pend) break;
       const int lead = __ffsll((long long)pend) - 1;
-      const uint32_t r = (uint32_t)__shfl((int)root, lead, 64);
+      const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)root, lead);
       const bool mine = act && root == r;
-      unsigned long long kk = mine ? kv : 0ull;
-      uint32_t cc = mine ? 1u : 0u, s0 = mine ? sm : NONE, s1 = mine ? sm : 0u;
-      uint32_t d0 = mine ? dmv : NONE, d1 = mine ? dmv : 0u;
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        const unsigned long long ko = __shfl_xor(kk, o, 64);
-        kk = ko > kk ? ko : kk;
-        cc += (uint32_t)__shfl_xor((int)cc, o, 64);
-        s0 = min(s0, (uint32_t)__shfl_xor((int)s0, o, 64));
-        s1 = max(s1, (uint32_t)__shfl_xor((int)s1, o, 64));
-        d0 = min(d0, (uint32_t)__shfl_xor((int)d0, o, 64));
-        d1 = max(d1, (uint32_t)__shfl_xor((int)d1, o, 64));
-      }
+      const unsigned long long kk = wave_max_u64(mine ? kv : 0ull);
+      const uint32_t cc = (uint32_t)__popcll(__ballot(mine));
+      const uint32_t s0 = wave_min(mine ? sm : NONE), s1 = wave_max(mine ? sm : 0u);
+      const uint32_t d0 = wave_min(mine ? dmv : NONE), d1 = wave_max(mine ? dmv : 0u);
       if (lane == lead) {
         amax(key + r, kk);
         aadd(cnt + r, cc);

@@ -34,8 +34,6 @@ constexpr int MAXBOX = 1024;
 
 enum Pass { P_STATS = 0, P_Y0 = 1, P_COUNT = 2, P_WRITE = 3 };
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v);
-
 // exclusive scan of one u32 per thread over the 256-thread workgroup: wave
 // scans, then the four wave totals through LDS (u32 sums: any order gives the
 // same words)
@@ -57,25 +55,9 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total)
   return before + incl - v;
 }
 
-// inclusive prefix sum over the wave (u32, wrapping), all in the vector ALU:
-// a Hillis-Steele scan within each 16-lane row by DPP row shifts (a lane
-// with no source lane adds 0), then the row totals carried forward by the
-// row broadcasts of lane 15 (into rows 1 and 3) and of lane 31 (into rows 2
-// and 3).  (Shuffles through ds_bpermute would wait on LDS latency six times.)
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
-{
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
-  return v;
-}
-
 // P[1..L] holds a block's dedispersed samples (P[0] = 0); afterwards P[i] is
 // the u32 (wrap-around) sum of the first i.  Thread l scans a contiguous
-// segment; segment totals are scanned within each wave by shuffles and across
+// segment; segment totals are scanned within each wave by DPP steps and across
 // the four waves through wtot.
 // MAXSEG: the largest segment; EXACT: L == MAXSEG * NT, so every thread's
 // segment is full and no access is predicated.
@@ -118,34 +100,6 @@ template <class S>
 __device__ __forceinline__ float normed(double raw, const S &sc, int k)
 {
   return sc.sigf[k] > 0.f ? (float)((raw - sc.mu[k]) * sc.inv[k]) : (float)raw;
-}
-
-// largest value of the wave, in every lane: the maximum within each 16-lane
-// row by four DPP steps (lane pairs, quads, half rows, rows), then the four
-// row maxima read into scalar registers
-__device__ __forceinline__ uint32_t wave_max(uint32_t x)
-{
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));  // row_half_mirror
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));  // row_mirror
-  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);
-  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x, 16);
-  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)x, 32);
-  const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
-  return max(max(a, b), max(c, d));
-}
-
-// sum of the wave (u32, wrapping), in every lane: DPP steps within each
-// 16-lane row, then the four row sums read into scalar registers
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
-{
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
-  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
-  return (uint32_t)__builtin_amdgcn_readlane((int)x, 0) + (uint32_t)__builtin_amdgcn_readlane((int)x, 16) +
-         (uint32_t)__builtin_amdgcn_readlane((int)x, 32) + (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
 }
 
 constexpr int MAXL = SB + MAXBOX;        // samples a block's boxes read
