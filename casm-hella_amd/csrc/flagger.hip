@@ -48,13 +48,16 @@ __device__ __forceinline__ uint8_t quant(float x)
   return (uint8_t)v;
 }
 
-// The normalisation of a pass: the plane's mean and 1/sigma (normalize_data,
-// flagger.cu:198-214), or with robust statistics the channel's median m and
-// scale s (0 when s is 0: a channel whose samples are mostly one value)
+// The normalisation of a pass, (float)(((double)x - a) * b): the plane's
+// mean and 1/sigma (normalize_data, flagger.cu:198-214), or with robust
+// statistics the channel's median m and 1/s for its scale s; zero when s is
+// 0 (a channel whose samples are mostly one value), and then every value
+// is 0.  Both forms are the same two double operations per value, so a and
+// b are formed once per channel (a per-value branch between the forms was
+// compiled into both paths and a select).
 struct NormP {
-  double mu, inv;
-  float m, s;
-  int robust;
+  double a, b;
+  bool zero;
 };
 
 // read through the constant address space (scalar loads at wave-uniform c):
@@ -64,18 +67,14 @@ __device__ __forceinline__ NormP norm_params(const FlagBeamDev &B, int c)
   typedef __attribute__((address_space(4))) const FScal FScalK;
   typedef __attribute__((address_space(4))) const float FK;
   const FScalK &sc = *(FScalK *)B.scal;
-  NormP n{sc.mu, sc.inv, 0.f, 0.f, B.robust};
-  if (B.robust) {
-    n.m = ((FK *)B.rstat)[2 * c];
-    n.s = ((FK *)B.rstat)[2 * c + 1];
-  }
-  return n;
+  if (!B.robust) return NormP{sc.mu, sc.inv, false};
+  const float m = ((FK *)B.rstat)[2 * c], sd = ((FK *)B.rstat)[2 * c + 1];
+  return NormP{(double)m, 1.0 / (double)sd, !(sd > 0.f)};
 }
 
 __device__ __forceinline__ float normv(float x, const NormP &n)
 {
-  if (n.robust) return n.s > 0.f ? (float)(((double)x - (double)n.m) * (1.0 / (double)n.s)) : 0.f;
-  return (float)(((double)x - n.mu) * n.inv);
+  return n.zero ? 0.f : (float)(((double)x - n.a) * n.b);
 }
 
 // ---- ingest: input gulp -> X (transpose_input, src/hella/src/transpose.cu:23-42)
@@ -381,7 +380,7 @@ __device__ NormP hstep_norm(const FlagBeamDev &B, const double *hs, double *sh)
       var = total / (double)B.nch;
   }
   const float sf = (float)sqrt(var);
-  return NormP{mu, 1.0 / (double)sf, 0.f, 0.f, 0};
+  return NormP{mu, 1.0 / (double)sf, false};
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
