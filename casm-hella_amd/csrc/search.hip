@@ -630,8 +630,8 @@ __global__ __launch_bounds__(NT) void k_hits(const SearchBeamDev *__restrict__ b
 // (S1 u64, S2 u128 as lo/hi), summed exactly and rounded to double once
 // (S2 = hi * 2^64 + lo); otherwise the double block partials are added in
 // ascending block order.
-__device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, int r, bool exact,
-                         int ngroups, bool want2, double &s1, double &s2)
+__device__ __forceinline__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, int r,
+                                         bool exact, int ngroups, bool want2, double &s1, double &s2)
 {
   // the partials are loaded RS at a time (clamped, unconditional) before any
   // is added, and added in ascending order: one memory latency per RS
@@ -692,6 +692,28 @@ __device__ void row_sums(const SearchBeamDev &B, const SearchCommon &C, int k, i
 // lanes are folded by the canonical tree.
 constexpr int FIN_NT = 1024;
 
+// the canonical sum of rowv[0 .. nr) (one chunk of at most SB rows): lane
+// l < 256 adds rows l + 256k in ascending k, then the lanes are folded by the
+// canonical tree; every thread returns it
+__device__ double fold_rows(const double *rowv, int nr, double *sh)
+{
+  const int l = threadIdx.x;
+  __syncthreads();
+  double acc[1] = {0.0};
+  if (l < NT) {
+    for (int kk = 0; kk < SB / NT; ++kk) {
+      if (l + NT * kk >= nr) break;
+      acc[0] += rowv[l + NT * kk];
+    }
+  }
+  tree256_multi(acc, sh);
+  if (l == 0) sh[256] = acc[0];
+  __syncthreads();
+  const double t = sh[256];
+  __syncthreads();
+  return t;
+}
+
 __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k, double mu,
                             bool var, bool exact, int ngroups, double *rowv, double *sh)
 {
@@ -709,21 +731,40 @@ __device__ double rows_csum(const SearchBeamDev &B, const SearchCommon &C, int k
       }
       rowv[i] = val;
     }
-    __syncthreads();
-    double acc[1] = {0.0};
-    if (l < NT) {
-      for (int kk = 0; kk < SB / NT; ++kk) {
-        if (r0 + l + NT * kk >= B.rows) break;
-        acc[0] += rowv[l + NT * kk];
-      }
-    }
-    tree256_multi(acc, sh);
-    if (l == 0) sh[256] = acc[0];
-    __syncthreads();
-    total += sh[256];
-    __syncthreads();
+    total += fold_rows(rowv, min(SB, B.rows - r0), sh);
   }
   return total / (double)B.rows;
+}
+
+// mu and the variance about it (calculate_stddev's two passes).  When every
+// row fits one chunk, each thread loads its rows' S1 and S2 once and keeps
+// them for the second pass; the row values and the folds are those of
+// rows_csum, so the results are the same.
+__device__ void rows_mu_var(const SearchBeamDev &B, const SearchCommon &C, int k, bool exact,
+                            int ngroups, double *rowv, double *sh, double &mu, double &var)
+{
+  if (B.rows > SB) {
+    mu = rows_csum(B, C, k, 0.0, false, exact, ngroups, rowv, sh);
+    var = rows_csum(B, C, k, mu, true, exact, ngroups, rowv, sh);
+    return;
+  }
+  constexpr int RPT = SB / FIN_NT;  // rows per thread
+  const int l = threadIdx.x;
+  const double n = (double)C.wp;
+  double s1[RPT], s2[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    s1[q] = s2[q] = 0.0;
+    if (l + FIN_NT * q < B.rows) row_sums(B, C, k, l + FIN_NT * q, exact, ngroups, true, s1[q], s2[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) rowv[l + FIN_NT * q] = l + FIN_NT * q < B.rows ? s1[q] / n : 0.0;
+  mu = (0.0 + fold_rows(rowv, B.rows, sh)) / (double)B.rows;  // rows_csum's total, from 0.0
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    rowv[l + FIN_NT * q] =
+        l + FIN_NT * q < B.rows ? ((s2[q] - (2.0 * mu) * s1[q]) + (n * mu) * mu) / n : 0.0;
+  var = (0.0 + fold_rows(rowv, B.rows, sh)) / (double)B.rows;
 }
 
 __global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restrict__ beams,
@@ -734,8 +775,8 @@ __global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restri
   __shared__ double rowv[SB];
   __shared__ double sh[260];  // 128 tree slots + the broadcast slot 256
   const bool exact = which == P_STATS && C.mode == SEARCH_BOXCAR;
-  const double mu = rows_csum(B, C, k, 0.0, false, exact, ngroups, rowv, sh);
-  const double var = rows_csum(B, C, k, mu, true, exact, ngroups, rowv, sh);
+  double mu, var;
+  rows_mu_var(B, C, k, exact, ngroups, rowv, sh, mu, var);
   SScal &sc = *B.scal;
   const float sf = (float)sqrt(var);
   if (which == P_STATS) {
