@@ -126,15 +126,21 @@ def test_server_clusters_match_oracle(hamd, orc):
     assert checked > 0
 
 
-@pytest.mark.parametrize("snr,lo,hi", [(2.5, 2049, 16384), (2.0, 16385, 10 ** 9)])
-def test_large_lists(hamd, orc, snr, lo, hi):
+@pytest.mark.parametrize("batched", [False, True])
+@pytest.mark.parametrize("snr,lo,hi,min_cl", [(2.5, 2049, 16384, 1025), (2.0, 16385, 10 ** 9, 0)])
+def test_large_lists(hamd, orc, snr, lo, hi, min_cl, batched):
     """lists beyond the LDS capacities (parents in LDS and the other words
-    in the work set's buffers; then everything in the buffers)"""
+    in the work set's buffers; then everything in the buffers); at S/N 2.5 a
+    beam has more clusters than the 1024 its pinned staging slot holds, so
+    the rest of its list is fetched after the counts"""
     cfg = _cfg(GULP=2048, NBEAM=3, SNR=snr, MAX_GIANTS=60000, CAND_CAP=60000, CLUSTER_DT=8,
                CLUSTER_DDM=2)
-    seen = 0
-    for peaks, cl in _run(hamd, cfg, False, gulps=4):
+    seen, most = 0, 0
+    for peaks, cl in _run(hamd, cfg, batched, gulps=4):
         assert _device_clusters(cl) == _oracle_clusters(orc, peaks, 8, 2)
         for b in set(peaks["beam"].tolist()):
             seen += int(lo <= int(np.sum(peaks["beam"] == b)) <= hi)
+        for b in set(cl["beam"].tolist()):
+            most = max(most, int(np.sum(cl["beam"] == b)))
     assert seen > 0
+    assert most >= min_cl
