@@ -812,7 +812,8 @@ __global__ __launch_bounds__(FIN_NT) void k_finish(const SearchBeamDev *__restri
   sc.thr = thr;
 }
 
-// ---- ordered compaction offsets: exclusive scan of counts per beam
+// ---- ordered compaction offsets: exclusive scan of counts per beam (tile
+// totals, then each tile's scan from the totals before it)
 __device__ __forceinline__ int64_t scan_len(const SearchBeamDev &B, const SearchCommon &C)
 {
   return (int64_t)C.nw * B.rows * C.nblk;
@@ -834,35 +835,42 @@ __global__ __launch_bounds__(NT) void k_scan_tiles(const SearchBeamDev *__restri
   if (threadIdx.x == 0) B.tiles[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(NT) void k_scan_top(const SearchBeamDev *__restrict__ beams,
-                                                 SearchCommon C)
-{
-  const SearchBeamDev B = beams[blockIdx.x];
-  const int64_t n = scan_len(B, C);
-  const int nt = (int)((n + SB - 1) / SB);
-  __shared__ uint32_t sh[NT];
-  const int seg = (nt + NT - 1) / NT;
-  const int i0 = threadIdx.x * seg, i1 = min(i0 + seg, nt);
-  uint32_t s = 0;
-  for (int i = i0; i < i1; ++i) s += B.tiles[i];
-  uint32_t tot;
-  uint32_t run = block_excl_scan(s, sh, &tot);
-  for (int i = i0; i < i1; ++i) {
-    const uint32_t v = B.tiles[i];
-    B.tiles[i] = run;
-    run += v;
-  }
-  if (threadIdx.x == 0) *B.ncand = tot;
-}
-
 __global__ __launch_bounds__(NT) void k_scan_down(const SearchBeamDev *__restrict__ beams,
                                                   SearchCommon C)
 {
   const SearchBeamDev B = beams[blockIdx.y];
   const int64_t n = scan_len(B, C);
   const int64_t t0 = (int64_t)blockIdx.x * SB;
-  if (t0 >= n) return;
+  if (t0 >= n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.ncand = 0u;  // a beam with nothing to scan
+    return;
+  }
   __shared__ uint32_t sh[NT];
+  __shared__ uint32_t red[2][NT / 64];
+  // this tile's offset is the sum of the earlier tiles' totals (k_scan_tiles);
+  // the first tile's workgroup also stores the beam's crossings (u32 sums:
+  // any order gives the same words)
+  const int nt = (int)((n + SB - 1) / SB);
+  uint32_t pre = 0, all = 0;
+  for (int i = threadIdx.x; i < nt; i += NT) {
+    const uint32_t v = B.tiles[i];
+    all += v;
+    pre += i < (int)blockIdx.x ? v : 0u;
+  }
+  pre = wave_sum(pre);
+  all = wave_sum(all);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = pre;
+    red[1][threadIdx.x >> 6] = all;
+  }
+  __syncthreads();
+  uint32_t tile_off = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    tile_off += red[0][w];
+    total += red[1][w];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *B.ncand = total;
   constexpr int PER = SB / NT;
   const int64_t i0 = t0 + (int64_t)threadIdx.x * PER;
   uint32_t v[PER];
@@ -873,7 +881,7 @@ __global__ __launch_bounds__(NT) void k_scan_down(const SearchBeamDev *__restric
     s += v[i];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan(s, sh, &tot) + B.tiles[blockIdx.x];
+  uint32_t run = block_excl_scan(s, sh, &tot) + tile_off;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     if (i0 + i < n) B.offs[i0 + i] = run;
@@ -1073,7 +1081,6 @@ void SearchRunner::peaks(const SearchBeamDev *d_beams, int nbeam, int max_rows,
   const int64_t n = (int64_t)C.nw * max_rows * C.nblk;
   const unsigned ntiles = (unsigned)((n + SB - 1) / SB);
   hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C);
-  hipLaunchKernelGGL(k_scan_top, dim3((unsigned)nbeam), dim3(NT), 0, s, d_beams, C);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles, (unsigned)nbeam), dim3(NT), 0, s, d_beams, C);
   pass(P_WRITE);
   HAMD_CHECK(hipGetLastError());
